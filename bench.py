@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark of the Hush denoiser hot path on one MI355X.
+
+Workload: 1920x1080, 64-frame sequence of the seeded "boxes" scene (200 boxes, 40 moving, 1% emissive
+pixels, ~10% sky, camera pan 3 px/frame), FP32 plain path, built-in weights.  All 64 frames of inputs are
+generated on the host and resident in HBM before timing.  A step is one denoised frame (the 8 passes).
+
+Prints one JSON line: ms/frame (lower is better), the per-pass times, the roofline of the dominant kernel
+(temporal) from its algorithmic bytes and its hipEvent-measured duration, and the CPU baseline (the
+oracle's FP32-mirror restatement, OpenMP on the host cores, on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms/frame @1080p (plain FP32 vs FP16-MFMA MLP) at stated PSNR; %HBM peak"
+HBM_PEAK_GBS = 8000.0
+INPUT_KEYS = ("radiance", "normalDepth", "motion", "material", "emission")
+
+
+def algorithmic_bytes(w: int, h: int) -> dict:
+    """Bytes per pass at the reference's pass boundaries: each bound texture once per texel per pass,
+    rgba16float 8 B, rgba32uint 16 B (reads and writes)."""
+    w2, h2 = (w + 1) // 2, (h + 1) // 2
+    w4, h4 = (w2 + 1) // 2, (h2 + 1) // 2
+    N, N2, N4 = w * h, w2 * h2, w4 * h4
+    return {"temporal": 80 * N + 72 * N2, "infer+mid1": 96 * N2, "mid2": 48 * N2, "mid3": 48 * N2,
+            "coarse1": 32 * N2 + 16 * N4, "coarse2": 24 * N4, "fine+final": 48 * N + 64 * N2 + 16 * N4,
+            "taa": 32 * N}
+
+
+def cpu_baseline(frames_g, w, h, n_frames):
+    """The oracle (FP32 mirror, C + OpenMP) timed on the host on the first `n_frames` frames."""
+    from oracle.ho import Oracle
+    import hush_amd
+    o = Oracle(w, h, hush_amd.default_weights())
+    o.denoise(frames_g[0])        # warm-up (page-in)
+    o.reset()
+    o.set_state(0, True)
+    t = time.perf_counter()
+    for f in range(n_frames):
+        o.denoise(frames_g[f])
+    dt = time.perf_counter() - t
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(dt / n_frames * 1e3, 3), "unit": "ms/frame", "cores": threads, "kind": "port",
+            "sample": f"{n_frames} frames of the same 1920x1080 boxes sequence, FP32 mirror oracle (C, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=128)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--frames", type=int, default=64)
+    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--cpu-frames", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the denoiser path runs whole frames on one device); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import hush_amd
+    from hush_amd import hip
+    from hush_amd.scene import SceneGenerator
+
+    W, H, F = args.width, args.height, args.frames
+    t0 = time.perf_counter()
+    gen = SceneGenerator("boxes", W, H, seed=0)
+    frames_g = [gen.render(f) for f in range(F)]
+    print(f"[bench] generated {F} frames {W}x{H} in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    dev = []
+    for g in frames_g:
+        dev.append({k: hip.Surface(W, H).upload(g[k].view(np.float16)) for k in INPUT_KEYS})
+    hip.synchronize()
+    print(f"[bench] inputs resident in HBM ({F * 5 * W * H * 8 / 1e9:.2f} GB)", file=sys.stderr, flush=True)
+
+    hs = hush_amd.Hush(0, {"width": W, "height": H, "precision": args.precision})
+    stream = hip.Stream()
+
+    def run(n, start=0):
+        for i in range(n):
+            f = (start + i) % F
+            if f == 0:
+                hs.reset()
+            hs.encode(dev[f], stream=stream.handle)
+
+    run(args.warmup)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    hip.synchronize()
+    e0.record(stream.handle)
+    t = time.perf_counter()
+    run(args.steps)
+    e1.record(stream.handle)
+    e1.synchronize()
+    wall = time.perf_counter() - t
+    ms_frame = e0.elapsed_ms(e1) / args.steps
+    print(f"[bench] timed {args.steps} frames: {ms_frame:.4f} ms/frame (wall {wall * 1e3 / args.steps:.4f})",
+          file=sys.stderr, flush=True)
+
+    # per-pass durations with hipEvents on the launch stream (separate loop, one frame at a time)
+    timer = hush_amd.HushTimer()
+    acc = np.zeros(8)
+    n_t = min(args.steps, 64)
+    for i in range(n_t):
+        f = i % F
+        if f == 0:
+            hs.reset()
+        hs.encode(dev[f], timer=timer, stream=stream.handle)
+        acc += np.array([ms for _, ms in timer.read()])
+    pass_ms = acc / n_t
+    names = ["temporal", "infer+mid1", "mid2", "mid3", "coarse1", "coarse2", "fine+final", "taa"]
+    ab = algorithmic_bytes(W, H)
+    total_bytes = sum(ab.values())
+    passes = {n: {"ms": round(float(m), 5), "GB/s": round(ab[n] / (m * 1e-3) / 1e9, 1),
+                  "hbm_frac": round(ab[n] / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)} for n, m in zip(names, pass_ms)}
+    tk = float(pass_ms[0])
+    achieved = ab["temporal"] / (tk * 1e-3) / 1e9
+    roofline = {"kernel": "temporal", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": ab["temporal"]}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(frames_g, W, H, args.cpu_frames)
+        print(f"[bench] cpu baseline {cpu['value']} ms/frame", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC, "value": round(ms_frame, 5), "unit": "ms/frame", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_frame, 5), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{W}x{H} boxes scene, {F}-frame sequence, {args.precision} MLP, built-in weights",
+                   "width": W, "height": H, "frames": F, "mlp": args.precision},
+        "frame_GB/s_algorithmic": round(total_bytes / (ms_frame * 1e-3) / 1e9, 1),
+        "frame_hbm_frac": round(total_bytes / (ms_frame * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "passes": passes, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    hs.destroy()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,70 @@
+"""ctypes declarations of the C ABI in include/hush.h (libhush.so, built in-tree by hush_amd/build.py).
+
+Loading fails loudly when the library is missing: there is no CPU fallback for the denoiser.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhush.so")
+SCENE_LIB_PATH = os.path.join(HERE, "libhush_scene.so")
+
+# every symbol include/hush.h declares
+EXPORTS = [
+    "hush_hidden_of", "hush_create", "hush_destroy", "hush_resize", "hush_reset", "hush_set_exposure",
+    "hush_get_exposure", "hush_get_size", "hush_hidden", "hush_frame", "hush_set_precision", "hush_get_precision",
+    "hush_denoise", "hush_encode", "hush_output", "hush_last_error", "hush_set_timing", "hush_pass_times",
+    "hush_debug_surface", "hush_debug_set_state", "hush_debug_run_pass", "hush_set_debug_mask",
+]
+
+
+class hush_surface(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("pitch", ctypes.c_size_t), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("format", ctypes.c_int32)]
+
+
+class hush_inputs(ctypes.Structure):
+    _fields_ = [("radiance", hush_surface), ("normal_depth", hush_surface), ("motion", hush_surface),
+                ("material", hush_surface), ("emission", hush_surface)]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+                          "or `python hush_amd/build.py`")
+    L = ctypes.CDLL(LIB_PATH)
+    P, i32, f32, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_size_t
+    S = ctypes.POINTER(hush_surface)
+    I = ctypes.POINTER(hush_inputs)
+    L.hush_hidden_of.argtypes = [sz]; L.hush_hidden_of.restype = i32
+    L.hush_create.argtypes = [i32, i32, i32, f32, P, sz, i32, ctypes.POINTER(P)]; L.hush_create.restype = i32
+    L.hush_destroy.argtypes = [P]; L.hush_destroy.restype = None
+    L.hush_resize.argtypes = [P, i32, i32]; L.hush_resize.restype = i32
+    L.hush_reset.argtypes = [P]; L.hush_reset.restype = i32
+    L.hush_set_exposure.argtypes = [P, f32]; L.hush_set_exposure.restype = i32
+    L.hush_get_exposure.argtypes = [P]; L.hush_get_exposure.restype = f32
+    L.hush_get_size.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]; L.hush_get_size.restype = i32
+    L.hush_hidden.argtypes = [P]; L.hush_hidden.restype = i32
+    L.hush_frame.argtypes = [P]; L.hush_frame.restype = ctypes.c_int64
+    L.hush_set_precision.argtypes = [P, i32]; L.hush_set_precision.restype = i32
+    L.hush_get_precision.argtypes = [P]; L.hush_get_precision.restype = i32
+    L.hush_denoise.argtypes = [P, I, P, S]; L.hush_denoise.restype = i32
+    L.hush_encode.argtypes = [P, I, P, S]; L.hush_encode.restype = i32
+    L.hush_output.argtypes = [P, S]; L.hush_output.restype = i32
+    L.hush_last_error.argtypes = [P]; L.hush_last_error.restype = ctypes.c_char_p
+    L.hush_set_timing.argtypes = [P, i32]; L.hush_set_timing.restype = i32
+    L.hush_pass_times.argtypes = [P, ctypes.POINTER(f32), ctypes.POINTER(ctypes.c_char_p)]; L.hush_pass_times.restype = i32
+    L.hush_debug_surface.argtypes = [P, i32, S]; L.hush_debug_surface.restype = i32
+    L.hush_debug_set_state.argtypes = [P, ctypes.c_int64, i32]; L.hush_debug_set_state.restype = i32
+    L.hush_debug_run_pass.argtypes = [P, i32, I, P]; L.hush_debug_run_pass.restype = i32
+    L.hush_set_debug_mask.argtypes = [P, i32]; L.hush_set_debug_mask.restype = i32
+    _lib = L
+    return L

@@ -1,0 +1,406 @@
+// Host side of the Hush C ABI (include/hush.h): context, surface arena, pass sequencing.
+// Mirrors the Hush class of src/denoise/hush.js:725-875 with HIP device buffers in place of GPUTextures.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "hush.h"
+#include "hush_internal.h"
+
+using namespace hush;
+
+namespace {
+
+thread_local char g_err[512];
+
+enum {
+  S_A0, S_X, S_BH0, S_BH1, S_MOM0, S_MOM1, S_KEY0, S_KEY1, S_HIST0, S_HIST1, S_TAA0, S_TAA1,
+  S_PAR0_0, S_PAR0_1, S_PAR1, S_PAR2, S_F0, S_F1, S_F2, S_F3, S_C0, S_C1, S_CG, S_MASK, S_COUNT
+};
+
+const char *kPassLabel[8] = {"temporal", "infer+mid", "infer+mid", "infer+mid", "coarse", "coarse", "fine+final", "taa"};
+const char *kInputName[5] = {"radiance", "normalDepth", "motion", "material", "emission"};
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// halfToFloat (hush.js:706-711)
+float half_to_float(uint16_t h) {
+  int s = h >> 15, e = (h >> 10) & 31, m = h & 1023;
+  float v;
+  if (e == 0) v = std::ldexp((float)m, -24);
+  else if (e == 31) v = m ? NAN : INFINITY;
+  else v = std::ldexp((float)(1024 + m), e - 25);
+  return s ? -v : v;
+}
+
+}  // namespace
+
+struct hush_ctx {
+  int device = 0;
+  int W = 0, H = 0, W2 = 0, H2 = 0, W4 = 0, H4 = 0;
+  int64_t frame = 0;
+  bool reset = true;
+  float exposure = 1.0f;
+  int hidden = 0;
+  int precision = HUSH_MLP_FP32;
+  float *d_w = nullptr;
+  void *d_wq = nullptr;
+  float *d_wscale = nullptr;
+  void *arena = nullptr;
+  Tex tex[S_COUNT] = {};
+  TexU F[4] = {};
+  uint8_t *mask = nullptr;
+  int mask_pitch = 0;
+  bool mask_on = false;
+  bool timing = false;
+  bool ev_ready = false;
+  hipEvent_t ev[9] = {};
+  char err[512] = {0};
+};
+
+static int fail(hush_t *h, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+static int fail(hush_t *h, int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(h ? h->err : g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+static int hip_fail(hush_t *h, hipError_t e, const char *what) {
+  return fail(h, HUSH_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+static bool precision_built(int p) { return p == HUSH_MLP_FP32 || p == HUSH_MLP_F16; }
+
+extern "C" int32_t hush_hidden_of(size_t n) {
+  for (long h = 1; h <= 256; h++)
+    if ((size_t)(h * h + 34 * h + 12) == n) return (int32_t)h;
+  return 0;
+}
+
+static void free_surfaces(hush_t *h) {
+  if (h->arena) (void)hipFree(h->arena);
+  h->arena = nullptr;
+  std::memset(h->tex, 0, sizeof(h->tex));
+  std::memset(h->F, 0, sizeof(h->F));
+  h->mask = nullptr;
+}
+
+// allocates the 22 surfaces of hush.js:780-793 (plus the debug mask) in one HBM arena
+static int alloc_surfaces(hush_t *h, int W, int H) {
+  const int W2 = (W + 1) / 2, H2 = (H + 1) / 2, W4 = (W2 + 1) / 2, H4 = (H2 + 1) / 2;
+  struct Spec { int id, w, h, bpt; };
+  std::vector<Spec> specs;
+  for (int id : {S_A0, S_X, S_MOM0, S_MOM1, S_KEY0, S_KEY1, S_HIST0, S_HIST1, S_TAA0, S_TAA1}) specs.push_back({id, W, H, 8});
+  for (int id : {S_BH0, S_BH1, S_PAR0_0, S_PAR0_1, S_PAR1, S_PAR2}) specs.push_back({id, W2, H2, 8});
+  for (int id : {S_F0, S_F1, S_F2, S_F3}) specs.push_back({id, W2, H2, 16});
+  for (int id : {S_C0, S_C1, S_CG}) specs.push_back({id, W4, H4, 8});
+  specs.push_back({S_MASK, W, H, 1});
+  size_t total = 0;
+  std::vector<size_t> offs;
+  for (auto &s : specs) {
+    size_t pitch = (size_t)round_up(s.w, 64) * s.bpt;
+    offs.push_back(total);
+    total += (pitch * s.h + 4095) / 4096 * 4096;
+  }
+  void *arena = nullptr;
+  hipError_t e = hipMalloc(&arena, total);
+  if (e != hipSuccess) return hip_fail(h, e, "hipMalloc of the denoiser surfaces");
+  e = hipMemset(arena, 0, total);
+  if (e != hipSuccess) { (void)hipFree(arena); return hip_fail(h, e, "hipMemset of the denoiser surfaces"); }
+  h->arena = arena;
+  for (size_t i = 0; i < specs.size(); i++) {
+    auto &s = specs[i];
+    char *p = (char *)arena + offs[i];
+    int pitch = round_up(s.w, 64);
+    if (s.id >= S_F0 && s.id <= S_F3) h->F[s.id - S_F0] = TexU{(uint4 *)p, pitch};
+    else if (s.id == S_MASK) { h->mask = (uint8_t *)p; h->mask_pitch = pitch; }
+    else h->tex[s.id] = Tex{(uint2 *)p, pitch};
+  }
+  h->W = W; h->H = H; h->W2 = W2; h->H2 = H2; h->W4 = W4; h->H4 = H4;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_resize(hush_t *h, int32_t width, int32_t height) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_resize: null context");
+  if (!(width > 0 && height > 0)) return fail(h, HUSH_ERR_SIZE, "Hush: bad size %dx%d", width, height);
+  if (width == h->W && height == h->H) return HUSH_OK;
+  hipError_t e = hipSetDevice(h->device);
+  if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+  (void)hipDeviceSynchronize();
+  free_surfaces(h);
+  h->W = h->H = 0;
+  int r = alloc_surfaces(h, width, height);
+  if (r != HUSH_OK) return r;
+  h->frame = 0;
+  h->reset = true;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, float exposure, const uint16_t *weights,
+                               size_t count, int32_t precision, hush_t **out) {
+  if (!out) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_create: null output pointer");
+  *out = nullptr;
+  if (!weights) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_create: null weights");
+  int hid = hush_hidden_of(count);
+  if (!hid) return fail(nullptr, HUSH_ERR_WEIGHTS, "hush weights: %zu f16 values fit no MLP size", count);
+  if (!(width > 0 && height > 0)) return fail(nullptr, HUSH_ERR_SIZE, "Hush: bad size %dx%d", width, height);
+  if (precision < HUSH_MLP_FP32 || precision > HUSH_MLP_FP8_MFMA) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_create: unknown precision %d", precision);
+  if (!precision_built(precision)) return fail(nullptr, HUSH_ERR_UNSUPPORTED, "hush_create: precision %d is not built", precision);
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipSetDevice");
+  hush_t *h = new (std::nothrow) hush_ctx();
+  if (!h) return fail(nullptr, HUSH_ERR_DEVICE, "hush_create: out of host memory");
+  h->device = device;
+  h->exposure = exposure;
+  h->hidden = hid;
+  h->precision = precision;
+  std::vector<float> wf(count);
+  for (size_t i = 0; i < count; i++) wf[i] = half_to_float(weights[i]);
+  e = hipMalloc((void **)&h->d_w, count * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(h->d_w, wf.data(), count * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hip_fail(nullptr, e, "weights upload");
+    hush_destroy(h);
+    return HUSH_ERR_DEVICE;
+  }
+  for (int i = 0; i < 9; i++) (void)hipEventCreate(&h->ev[i]);
+  int r = hush_resize(h, width, height);
+  if (r != HUSH_OK) {
+    std::snprintf(g_err, sizeof(g_err), "%s", h->err);
+    hush_destroy(h);
+    return r;
+  }
+  *out = h;
+  return HUSH_OK;
+}
+
+extern "C" void hush_destroy(hush_t *h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  (void)hipDeviceSynchronize();
+  free_surfaces(h);
+  if (h->d_w) (void)hipFree(h->d_w);
+  if (h->d_wq) (void)hipFree(h->d_wq);
+  if (h->d_wscale) (void)hipFree(h->d_wscale);
+  for (int i = 0; i < 9; i++)
+    if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+  delete h;
+}
+
+extern "C" int32_t hush_reset(hush_t *h) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_reset: null context");
+  h->reset = true;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_set_exposure(hush_t *h, float e) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_exposure: null context");
+  h->exposure = e;
+  return HUSH_OK;
+}
+extern "C" float hush_get_exposure(const hush_t *h) { return h ? h->exposure : NAN; }
+
+extern "C" int32_t hush_get_size(const hush_t *h, int32_t *w, int32_t *hh) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_get_size: null context");
+  if (w) *w = h->W;
+  if (hh) *hh = h->H;
+  return HUSH_OK;
+}
+extern "C" int32_t hush_hidden(const hush_t *h) { return h ? h->hidden : 0; }
+extern "C" int64_t hush_frame(const hush_t *h) { return h ? h->frame : -1; }
+
+extern "C" int32_t hush_set_precision(hush_t *h, int32_t p) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_precision: null context");
+  if (p < HUSH_MLP_FP32 || p > HUSH_MLP_FP8_MFMA) return fail(h, HUSH_ERR_ARGUMENT, "hush_set_precision: unknown precision %d", p);
+  if (!precision_built(p)) return fail(h, HUSH_ERR_UNSUPPORTED, "hush_set_precision: precision %d is not built", p);
+  h->precision = p;
+  return HUSH_OK;
+}
+extern "C" int32_t hush_get_precision(const hush_t *h) { return h ? h->precision : -1; }
+
+extern "C" const char *hush_last_error(const hush_t *h) { return h ? h->err : g_err; }
+
+static Tex in_tex(const hush_surface &s) { return Tex{(uint2 *)s.ptr, (int)(s.pitch / 8)}; }
+
+static hush_surface surf_of(const hush_t *h, int id) {
+  hush_surface s{};
+  if (id >= S_F0 && id <= S_F3) {
+    s.ptr = h->F[id - S_F0].p; s.pitch = (size_t)h->F[id - S_F0].pitch * 16;
+    s.width = h->W2; s.height = h->H2; s.format = HUSH_RGBA32U;
+  } else if (id == S_MASK) {
+    s.ptr = h->mask; s.pitch = (size_t)h->mask_pitch; s.width = h->W; s.height = h->H; s.format = HUSH_R8U;
+  } else {
+    s.ptr = h->tex[id].p; s.pitch = (size_t)h->tex[id].pitch * 8; s.format = HUSH_RGBA16F;
+    bool half = id == S_BH0 || id == S_BH1 || id == S_PAR0_0 || id == S_PAR0_1 || id == S_PAR1 || id == S_PAR2;
+    bool quarter = id == S_C0 || id == S_C1 || id == S_CG;
+    s.width = half ? h->W2 : quarter ? h->W4 : h->W;
+    s.height = half ? h->H2 : quarter ? h->H4 : h->H;
+  }
+  return s;
+}
+
+// one pass of the frame at the context's current state (hush.js:858-870)
+static hipError_t run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream_t s) {
+  const int cur = (int)(h->frame & 1), prev = cur ^ 1;
+  const Tex *t = h->tex;
+  switch (pass) {
+    case 0: {
+      TemporalArgs a;
+      a.rad = in_tex(in->radiance); a.nd = in_tex(in->normal_depth); a.mv = in_tex(in->motion); a.mat = in_tex(in->material);
+      a.pkey = t[S_KEY0 + prev]; a.pcol = t[S_HIST0 + prev]; a.pmom = t[S_MOM0 + prev]; a.ppar0 = t[S_PAR0_0 + prev];
+      a.outA = t[S_A0]; a.outMom = t[S_MOM0 + cur]; a.outKey = t[S_KEY0 + cur];
+      a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.F3 = h->F[3];
+      a.mask = h->mask_on ? h->mask : nullptr; a.mask_pitch = h->mask_pitch;
+      a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.reset = h->reset ? 1 : 0; a.E = h->exposure;
+      return launch_temporal(a, s);
+    }
+    case 1: {
+      InferArgs a;
+      a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.F3 = h->F[3];
+      a.out0 = t[S_PAR0_0 + cur]; a.out1 = t[S_PAR1]; a.out2 = t[S_PAR2]; a.outMid = t[S_BH0];
+      a.W2 = h->W2; a.H2 = h->H2; a.hidden = h->hidden; a.w = h->d_w; a.wq = h->d_wq; a.wscale = h->d_wscale;
+      return launch_infer(a, h->precision, s);
+    }
+    case 2:
+    case 3: {
+      AtrousArgs a{};
+      a.inA = t[pass == 2 ? S_BH0 : S_BH1]; a.guideF = h->F[3]; a.out = t[pass == 2 ? S_BH1 : S_BH0];
+      a.par1 = t[S_PAR1]; a.par2 = t[S_PAR2]; a.w = h->W2; a.h = h->H2; a.s = 0.0f;
+      return pass == 2 ? launch_mid2(a, s) : launch_mid3(a, s);
+    }
+    case 4: {
+      Coarse1Args a;
+      a.F2 = h->F[2]; a.F3 = h->F[3]; a.outC = t[S_C1]; a.outG = t[S_CG];
+      a.W4 = h->W4; a.H4 = h->H4; a.W2 = h->W2; a.H2 = h->H2; a.s = 0.25f;   // COARSE_S (hush.js:26)
+      return launch_coarse1(a, s);
+    }
+    case 5: {
+      AtrousArgs a{};
+      a.inA = t[S_C1]; a.guideT = t[S_CG]; a.out = t[S_C0]; a.w = h->W4; a.h = h->H4; a.s = 0.25f;
+      return launch_coarse2(a, s);
+    }
+    case 6: {
+      FineArgs a;
+      a.A0 = t[S_A0]; a.key = t[S_KEY0 + cur]; a.par0 = t[S_PAR0_0 + cur]; a.par1 = t[S_PAR1]; a.par2 = t[S_PAR2];
+      a.F2 = h->F[2]; a.F3 = h->F[3]; a.mid = t[S_BH0]; a.coarse = t[S_C0]; a.cguide = t[S_CG];
+      a.mat = in_tex(in->material); a.emi = in_tex(in->emission);
+      a.outHist = t[S_HIST0 + cur]; a.outX = t[S_X];
+      a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.W4 = h->W4; a.H4 = h->H4;
+      return launch_fine_final(a, s);
+    }
+    case 7: {
+      TaaArgs a;
+      a.X = t[S_X]; a.mv = in_tex(in->motion); a.prev = t[S_TAA0 + prev]; a.out = t[S_TAA0 + cur];
+      a.W = h->W; a.H = h->H; a.reset = h->reset ? 1 : 0;
+      return launch_taa(a, s);
+    }
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+static int check_inputs(hush_t *h, const hush_inputs *in) {
+  const hush_surface *s[5] = {&in->radiance, &in->normal_depth, &in->motion, &in->material, &in->emission};
+  for (int k = 0; k < 5; k++) {
+    if (!s[k]->ptr) return fail(h, HUSH_ERR_INPUT, "Hush.denoise: missing input '%s'", kInputName[k]);
+    if (s[k]->width != h->W || s[k]->height != h->H)
+      return fail(h, HUSH_ERR_INPUT, "Hush.denoise: '%s' is %dx%d, the denoiser %dx%d (call resize)", kInputName[k],
+                  s[k]->width, s[k]->height, h->W, h->H);
+    if (s[k]->format != HUSH_RGBA16F)
+      return fail(h, HUSH_ERR_INPUT, "Hush.denoise: '%s' must be rgba16float", kInputName[k]);
+    if (s[k]->pitch % 8 != 0 || s[k]->pitch < (size_t)h->W * 8 || ((uintptr_t)s[k]->ptr & 7) != 0)
+      return fail(h, HUSH_ERR_INPUT, "Hush.denoise: '%s' has pitch %zu bytes; rows need 8-byte alignment and >= %d bytes",
+                  kInputName[k], s[k]->pitch, h->W * 8);
+  }
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_encode(hush_t *h, const hush_inputs *in, void *stream, hush_surface *out) {
+  if (!h || !in) return fail(h, HUSH_ERR_ARGUMENT, "hush_encode: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSetDevice(h->device);
+  if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+  for (int p = 0; p < 8; p++) {
+    if (h->timing) (void)hipEventRecord(h->ev[p], s);
+    e = run_pass(h, p, in, s);
+    if (e != hipSuccess) return hip_fail(h, e, kPassLabel[p]);
+  }
+  if (h->timing) { (void)hipEventRecord(h->ev[8], s); h->ev_ready = true; }
+  const int cur = (int)(h->frame & 1);
+  h->reset = false;
+  h->frame++;
+  if (out) *out = surf_of(h, S_TAA0 + cur);
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_denoise(hush_t *h, const hush_inputs *in, void *stream, hush_surface *out) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_denoise: null context");
+  if (!in) return fail(h, HUSH_ERR_INPUT, "Hush.denoise: missing input '%s'", kInputName[0]);
+  int r = check_inputs(h, in);
+  if (r != HUSH_OK) return r;
+  return hush_encode(h, in, stream, out);
+}
+
+extern "C" int32_t hush_output(const hush_t *h, hush_surface *out) {
+  if (!h || !out) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_output: null argument");
+  *out = surf_of(h, S_TAA0 + (int)((h->frame + 1) & 1));
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_set_timing(hush_t *h, int32_t enabled) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_timing: null context");
+  h->timing = enabled != 0;
+  h->ev_ready = false;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_pass_times(hush_t *h, float *ms, const char **labels) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_pass_times: null context");
+  if (labels)
+    for (int i = 0; i < 8; i++) labels[i] = kPassLabel[i];
+  if (!ms) return HUSH_OK;
+  if (!h->ev_ready) return fail(h, HUSH_ERR_ARGUMENT, "hush_pass_times: no timed frame (enable timing first)");
+  hipError_t e = hipEventSynchronize(h->ev[8]);
+  if (e != hipSuccess) return hip_fail(h, e, "hipEventSynchronize");
+  for (int i = 0; i < 8; i++) {
+    e = hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]);
+    if (e != hipSuccess) return hip_fail(h, e, "hipEventElapsedTime");
+  }
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_debug_surface(hush_t *h, int32_t id, hush_surface *out) {
+  if (!h || !out) return fail(h, HUSH_ERR_ARGUMENT, "hush_debug_surface: null argument");
+  if (id < 0 || id >= S_COUNT) return fail(h, HUSH_ERR_ARGUMENT, "hush_debug_surface: unknown surface %d", id);
+  *out = surf_of(h, id);
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_debug_set_state(hush_t *h, int64_t frame, int32_t reset) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_debug_set_state: null context");
+  h->frame = frame;
+  h->reset = reset != 0;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_debug_run_pass(hush_t *h, int32_t pass, const hush_inputs *in, void *stream) {
+  if (!h || !in) return fail(h, HUSH_ERR_ARGUMENT, "hush_debug_run_pass: null argument");
+  if (pass < 0 || pass > 7) return fail(h, HUSH_ERR_ARGUMENT, "hush_debug_run_pass: unknown pass %d", pass);
+  int r = check_inputs(h, in);
+  if (r != HUSH_OK) return r;
+  hipError_t e = hipSetDevice(h->device);
+  if (e == hipSuccess) e = run_pass(h, pass, in, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(h, e, kPassLabel[pass]);
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_set_debug_mask(hush_t *h, int32_t enabled) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_debug_mask: null context");
+  h->mask_on = enabled != 0;
+  return HUSH_OK;
+}

@@ -1,0 +1,778 @@
+// Hush denoiser hot path: the eight per-frame passes of src/denoise/hush.js as gfx950 HIP kernels.
+//
+// Launch shape: every pass runs 256-thread workgroups (four wave64s) over 32x8 output texels; each wave
+// covers a 32x2 strip, so a row segment is 32 texels = 256 contiguous bytes of an rgba16float surface.
+// Neighbourhood passes stage a clamp-to-edge tile plus apron in LDS (the semantics of the reference's
+// clampP-staged workgroup tiles, hush.js:287-295, 332-339, 407-415, 512-517, 563-568, 642-645).
+//
+// The arithmetic follows the reference's operation order with no FMA contraction (build flag
+// -ffp-contract=off); the MLP's multiply-adds are fused explicitly (__builtin_fmaf).
+#include "hush_device.h"
+#include "hush_internal.h"
+
+namespace hush {
+
+constexpr int TW = 32, TH = 8, NT = 256;
+
+// ============================================================================ pass 1: temporal
+// TEMPORAL (hush.js:79-308): demodulate, 3x3 stats, reproject (bilinear + 3x3 fallback), clamp and blend,
+// store A0 / mom / key; then per 2x2 block the 20 MLP features and the half-res guide (F0..F3).
+
+__device__ __forceinline__ bool check(float4 k, float3 n, float zprev, float obj) {  // hush.js:103-105
+  if (!(k.z >= 0.0f)) return false;
+  if (!(k.w == obj)) return false;
+  float tol = 0.05f * zprev;
+  tol = tol + 0.01f;
+  if (!(fabsf(k.z - zprev) <= tol)) return false;
+  return dot3(oct_dec(k.x, k.y), n) >= 0.9f;
+}
+
+// per-pixel feature inputs kept in LDS for the block phase (structure-of-arrays, conflict-free)
+enum { P0X, P0Y, P0Z, P0W, P1X, P1Y, P1Z, P1W, P2Z, P2W, NSX, NSY, NSZ, P3X, P3Y, P3Z, P3W, P4X, P4Y, P4Z, P4W, NP };
+
+__global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
+  constexpr int SW = TW + 2, SH = TH + 2, SN = SW * SH;
+  __shared__ float4 sI[SN];        // demodulated illumination, valid flag
+  __shared__ float sL[SN];         // lum(illum)
+  __shared__ float sQ[SN];         // E * qf16(lum(illum))
+  __shared__ float sP[NP][NT];     // feature inputs P0..P4 (+ decoded normal) per pixel
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const float E = a.E;
+  for (int i = tid; i < SN; i += NT) {
+    int qx = x0 - 1 + i % SW, qy = y0 - 1 + i / SW;
+    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (inside(qx, qy, a.W, a.H)) {
+      float4 nd = ld(a.nd, qx, qy);
+      if (nd.w >= 0.0f) {
+        float4 r = ld(a.rad, qx, qy), m = ld(a.mat, qx, qy);
+        e = make_float4(r.x / mx(m.x, 1e-3f), r.y / mx(m.y, 1e-3f), r.z / mx(m.z, 1e-3f), 1.0f);
+      }
+    }
+    sI[i] = e;
+    float l = lum(e.x, e.y, e.z);
+    sL[i] = l;
+    sQ[i] = E * qf16(l);
+  }
+  __syncthreads();
+
+  const int lx = tid & 31, ly = tid >> 5;
+  const int px = x0 + lx, py = y0 + ly;
+  float P[NP];
+#pragma unroll
+  for (int k = 0; k < NP; k++) P[k] = 0.0f;
+  P[P2Z] = -1.0f;
+  if (inside(px, py, a.W, a.H)) {
+    float4 nd = ld(a.nd, px, py);
+    float4 mv = ld(a.mv, px, py);
+    uint32_t mbits = 0;
+    if (nd.w < 0.0f) {                                   // sky (hush.js:115-119)
+      st(a.outA, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+      st(a.outMom, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+      st(a.outKey, px, py, make_float4(0.0f, 0.0f, -1.0f, mv.z));
+      mbits = 0x80;
+    } else {
+      const float3 n = make_float3(nd.x, nd.y, nd.z);
+      const float4 mr = ld(a.mat, px, py);
+      const int c0 = (ly + 1) * SW + lx + 1;
+      const float4 ce = sI[c0];
+      const float l = sL[c0];
+      // 3x3 stats (hush.js:125-142)
+      float s1x = 0.0f, s1y = 0.0f, s1z = 0.0f, cnt = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 9; j++) {
+        float4 t = sI[c0 + (j / 3 - 1) * SW + (j % 3 - 1)];
+        s1x += t.x * t.w; s1y += t.y * t.w; s1z += t.z * t.w;
+        cnt += t.w;
+      }
+      cnt = mx(cnt, 1.0f);
+      const float mux = s1x / cnt, muy = s1y / cnt, muz = s1z / cnt;
+      const float slm = lum(mux, muy, muz);
+      float svx = 0.0f, svy = 0.0f, svz = 0.0f, svar = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 9; j++) {
+        int idx = c0 + (j / 3 - 1) * SW + (j % 3 - 1);
+        float4 t = sI[idx];
+        float dx = t.x - mux, dy = t.y - muy, dz = t.z - muz;
+        float dl = sL[idx] - slm;
+        svx += t.w * dx * dx; svy += t.w * dy * dy; svz += t.w * dz * dz;
+        svar += t.w * dl * dl;
+      }
+      const float sdx = sqrtf(mx(svx / cnt, 1e-12f)), sdy = sqrtf(mx(svy / cnt, 1e-12f)), sdz = sqrtf(mx(svz / cnt, 1e-12f));
+      svar /= cnt;
+      // reprojection (hush.js:144-187)
+      bool has = false;
+      float hcx = 0.0f, hcy = 0.0f, hcz = 0.0f;
+      float4 hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      float v = 0.0f, alpha = 1.0f, gamma = 1.0f;
+      if (!a.reset) {
+        const int W = a.W, H = a.H;
+        float ppx = (float)px + mv.x, ppy = (float)py + mv.y;
+        float bx = floorf(ppx), by = floorf(ppy);
+        float fx = ppx - bx, fy = ppy - by;
+        int bix = (int)bx, biy = (int)by;
+        float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+        int cx = (int)floorf(ppx + 0.5f), cy = (int)floorf(ppy + 0.5f);
+        // all 12 bilinear loads and the par0 load issued before the tests
+        float4 k4[4], c4[4], m4[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          int qx = clampi(bix + (i & 1), 0, W - 1), qy = clampi(biy + (i >> 1), 0, H - 1);
+          k4[i] = ld(a.pkey, qx, qy); c4[i] = ld(a.pcol, qx, qy); m4[i] = ld(a.pmom, qx, qy);
+        }
+        float4 pq = ld(a.ppar0, clampi(cx, 0, W - 1) >> 1, clampi(cy, 0, H - 1) >> 1);
+        float sumW = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          if (wts[i] > 0.0f && inside(bix + (i & 1), biy + (i >> 1), W, H) && check(k4[i], n, mv.w, mv.z)) {
+            mbits |= 1u << i;
+            sumW += wts[i];
+            hcx += wts[i] * c4[i].x; hcy += wts[i] * c4[i].y; hcz += wts[i] * c4[i].z;
+            hm.x += wts[i] * m4[i].x; hm.y += wts[i] * m4[i].y; hm.z += wts[i] * m4[i].z; hm.w += wts[i] * m4[i].w;
+          }
+        }
+        if (sumW >= 0.01f) {
+          hcx /= sumW; hcy /= sumW; hcz /= sumW;
+          hm.x /= sumW; hm.y /= sumW; hm.z /= sumW; hm.w /= sumW;
+          v = sumW; has = true;
+        } else {
+          float n3 = 0.0f;
+          hcx = hcy = hcz = 0.0f;
+          hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          for (int j = 0; j < 9; j++) {
+            int qx = cx + j % 3 - 1, qy = cy + j / 3 - 1;
+            if (inside(qx, qy, W, H) && check(ld(a.pkey, qx, qy), n, mv.w, mv.z)) {
+              n3 += 1.0f;
+              float4 cc = ld(a.pcol, qx, qy), mm = ld(a.pmom, qx, qy);
+              hcx += cc.x; hcy += cc.y; hcz += cc.z;
+              hm.x += mm.x; hm.y += mm.y; hm.z += mm.z; hm.w += mm.w;
+            }
+          }
+          if (n3 > 0.0f) {
+            hcx /= n3; hcy /= n3; hcz /= n3;
+            hm.x /= n3; hm.y /= n3; hm.z /= n3; hm.w /= n3;
+            v = 0.1f; has = true;
+            mbits |= 0x10;
+          }
+        }
+        alpha = pq.x; gamma = pq.y;
+      }
+      has = has && hm.w > 0.0f;
+      if (has) mbits |= 0x20;
+      // temporal blend (hush.js:189-211)
+      float cr = ce.x, cg = ce.y, cb = ce.z;
+      float nmx = l, nmy = l * l, nfast = l, len1 = 1.0f, lag = 0.0f;
+      if (has) {
+        float gx = gamma * sdx, gy = gamma * sdy, gz = gamma * sdz;
+        float hlx = mn(mx(hcx, mux - gx), mux + gx), hly = mn(mx(hcy, muy - gy), muy + gy), hlz = mn(mx(hcz, muz - gz), muz + gz);
+        len1 = mn(hm.w + 1.0f, 32.0f);
+        float aa = mx(alpha, 1.0f / len1);
+        cr = hlx + (ce.x - hlx) * aa; cg = hly + (ce.y - hly) * aa; cb = hlz + (ce.z - hlz) * aa;
+        nmx = hm.x + (l - hm.x) * aa;
+        nmy = hm.y + (l * l - hm.y) * aa;
+        nfast = hm.z + (l - hm.z) * mx(1.0f / 6.0f, 1.0f / len1);
+        float lh = lum(hcx, hcy, hcz);
+        lag = fabsf(slm - lh) / (sqrtf(mx(svar / cnt, 1e-12f)) + 0.02f * lh + 1e-3f);
+      } else {
+        v = 0.0f;
+      }
+      float tvar = mx(nmy - nmx * nmx, 0.0f);
+      float4 A = make_float4(cr, cg, cb, svar + (tvar - svar) * mn(len1 / 4.0f, 1.0f));
+      float2 oc = oct_enc(n);
+      st(a.outA, px, py, A);
+      st(a.outMom, px, py, make_float4(nmx, nmy, nfast, len1));
+      st(a.outKey, px, py, make_float4(oc.x, oc.y, nd.w, mv.z));
+      // feature inputs from the stored values (hush.js:212-222)
+      float aqx = qf16(A.x), aqy = qf16(A.y), aqz = qf16(A.z), aqw = qf16(A.w);
+      float m1 = E * qf16(nmx);
+      float m2 = E * E * qf16(nmy);
+      P[P0X] = E * lum(aqx, aqy, aqz); P[P0Y] = E * qf16(l); P[P0Z] = mx(m2 - m1 * m1, 0.0f); P[P0W] = len1;
+      P[P1X] = qf16(v); P[P1Y] = qf16(lag); P[P1Z] = E * qf16(nfast); P[P1W] = m1;
+      float ocx = qf16(oc.x), ocy = qf16(oc.y);
+      P[P2Z] = qf16(nd.w); P[P2W] = qf16(sqrtf(mv.x * mv.x + mv.y * mv.y));
+      float3 ns = oct_dec(ocx, ocy);
+      P[NSX] = ns.x; P[NSY] = ns.y; P[NSZ] = ns.z;
+      P[P3X] = mr.x; P[P3Y] = mr.y; P[P3Z] = mr.z; P[P3W] = mr.w;
+      P[P4X] = aqx; P[P4Y] = aqy; P[P4Z] = aqz; P[P4W] = aqw;
+    }
+    if (a.mask) a.mask[(size_t)py * a.mask_pitch + px] = (uint8_t)mbits;
+  }
+#pragma unroll
+  for (int k = 0; k < NP; k++) sP[k][tid] = P[k];
+  __syncthreads();
+
+  // block phase (hush.js:226-282, 301-306): wave 0 computes the 16x4 half-res texels of this tile
+  if (tid >= 64) return;
+  const int bxl = tid & 15, byl = tid >> 4;
+  const int X = (x0 >> 1) + bxl, Y = (y0 >> 1) + byl;
+  if (X >= a.W2 || Y >= a.H2) return;
+  float nv = 0.0f;
+  float S0[4] = {0, 0, 0, 0}, S1[4] = {0, 0, 0, 0}, S3[4] = {0, 0, 0, 0}, AC[4] = {0, 0, 0, 0};
+  float Nx = 0.0f, Ny = 0.0f, Nz = 0.0f, MV = 0.0f, Z = 0.0f;
+  bool ok[4];
+  float la[4], zs[4], nsx[4], nsy[4], nsz[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int t = byl * 64 + (j >> 1) * 32 + 2 * bxl + (j & 1);
+    ok[j] = sP[P2Z][t] >= 0.0f;
+    la[j] = zs[j] = nsx[j] = nsy[j] = nsz[j] = 0.0f;
+    if (!ok[j]) continue;
+    nv += 1.0f;
+    la[j] = sP[P0X][t]; zs[j] = sP[P2Z][t];
+    nsx[j] = sP[NSX][t]; nsy[j] = sP[NSY][t]; nsz[j] = sP[NSZ][t];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      S0[c] += sP[P0X + c][t]; S1[c] += sP[P1X + c][t]; S3[c] += sP[P3X + c][t]; AC[c] += sP[P4X + c][t];
+    }
+    Nx += nsx[j]; Ny += nsy[j]; Nz += nsz[j];
+    MV += sP[P2W][t]; Z += sP[P2Z][t];
+  }
+  if (nv == 0.0f) {
+    stu(a.F0, X, Y, make_uint4(0, 0, 0, 0));
+    stu(a.F1, X, Y, make_uint4(0, 0, 0, 0));
+    stu(a.F2, X, Y, make_uint4(0, 0, 0, 0));
+    stu(a.F3, X, Y, make_uint4(pk(0.0f, 0.0f), pk(-1.0f, 0.0f), 0u, 0u));
+    return;
+  }
+  const float k = 1.0f / nv;
+#pragma unroll
+  for (int c = 0; c < 4; c++) { S0[c] *= k; S1[c] *= k; S3[c] *= k; AC[c] *= k; }
+  Nx *= k; Ny *= k; Nz *= k; MV *= k; Z *= k;
+  const float Lacc = S0[0];
+  float VA = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 4; j++) if (ok[j]) VA += (la[j] - Lacc) * (la[j] - Lacc);
+  VA *= k;
+  // 4x4 neighbourhood (block +-1) of the exposed, truncated luminance (hush.js:253-265)
+  float s1 = 0.0f, n4 = 0.0f;
+  const int b0 = (2 * byl) * SW + 2 * bxl;      // tile index of pixel (2*bxl - 1, 2*byl - 1) incl. apron
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    int idx = b0 + (j >> 2) * SW + (j & 3);
+    float w = sI[idx].w;
+    s1 += w * E * qf16(sL[idx]);
+    n4 += w;
+  }
+  const float mu4 = s1 / mx(n4, 1.0f);
+  float var4 = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    int idx = b0 + (j >> 2) * SW + (j & 3);
+    float d = sQ[idx] - mu4;
+    var4 += sI[idx].w * d * d;
+  }
+  var4 /= mx(n4, 1.0f);
+  const float ln = mx(sqrtf(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
+  const float3 Nn = make_float3(Nx / ln, Ny / ln, Nz / ln);
+  float zmax = -1e30f, zmin = 1e30f, mind = 1e30f;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (ok[j]) {
+      zmax = mx(zmax, zs[j]); zmin = mn(zmin, zs[j]);
+      mind = mn(mind, dot3(make_float3(nsx[j], nsy[j], nsz[j]), Nn));
+    }
+  const float dz = mn(mx((zmax - zmin) / mx(zmin, 1e-6f), 0.0f), 1.0f);
+  const float dn = mn(mx(1.0f - mind, 0.0f), 2.0f);
+  const float2 oc = oct_enc(Nn);
+  float f[20] = {log1p_(Lacc), log1p_(S0[1]), log1p_(S0[2]), log1p_(var4), S0[3] / 32.0f, S1[0], log1p_(S1[1]),
+                 log1p_(fabsf(S1[2] - S1[3]) / (S1[3] + 1e-3f)), oc.x, oc.y, dz, dn, S3[0], S3[1], S3[2], S3[3],
+                 1.0f - nv / 4.0f, log1p_(MV), log1p_(mu4), log1p_(VA)};
+  stu(a.F0, X, Y, make_uint4(pk(f[0], f[1]), pk(f[2], f[3]), pk(f[4], f[5]), pk(f[6], f[7])));
+  stu(a.F1, X, Y, make_uint4(pk(f[8], f[9]), pk(f[10], f[11]), pk(f[12], f[13]), pk(f[14], f[15])));
+  stu(a.F2, X, Y, make_uint4(pk(f[16], f[17]), pk(f[18], f[19]), pk(AC[0], AC[1]), pk(AC[2], AC[3])));
+  stu(a.F3, X, Y, make_uint4(pk(oc.x, oc.y), pk(Z, nv), 0u, 0u));
+}
+
+// ============================================================================ edge-aware a-trous
+// filt (atrousFn, hush.js:431-476) over an LDS tile (SW texels per row) holding the source (rgb, var) and
+// the decoded guide (normal, depth) of clamp-to-edge texels; taps outside the image are skipped.
+
+template <int R> struct Kern;
+template <> struct Kern<1> { static constexpr double h[3] = {0.25, 0.5, 0.25}; };
+template <> struct Kern<2> { static constexpr double h[5] = {1.0 / 16, 1.0 / 4, 3.0 / 8, 1.0 / 4, 1.0 / 16}; };
+
+// sqrt(d2) for the squared tap distances of a (2R+1)^2 kernel
+__host__ __device__ constexpr double tap_dist(int d2) {
+  return d2 == 1 ? 1.0 : d2 == 2 ? 1.4142135623730951 : d2 == 4 ? 2.0 : d2 == 5 ? 2.23606797749979
+       : d2 == 8 ? 2.8284271247461903 : 0.0;
+}
+
+template <int R, int STEP, bool PRE, int SW>
+__device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int tx, int ty, int qx, int qy, int w, int h,
+                                       float s, float cc) {
+  const int c = ty * SW + tx;
+  const float4 gp = sG[c], center = sA[c];
+  float gv;
+  if (PRE) {
+    gv = 0.0625f * (sA[c - SW - 1].w + sA[c - SW + 1].w + sA[c + SW - 1].w + sA[c + SW + 1].w)
+       + 0.125f * (sA[c - SW].w + sA[c - 1].w + sA[c + 1].w + sA[c + SW].w)
+       + 0.25f * center.w;
+  } else {
+    gv = center.w;
+  }
+  const float lc = lum(center.x, center.y, center.z);
+  const float sl = s / (sqrtf(mx(gv, 1e-12f)) + 1e-6f);
+  const float zl = sG[c - 1].w, zr = sG[c + 1].w, zu = sG[c - SW].w, zd = sG[c + SW].w;
+  const float gx = mn(zr > 0.0f ? fabsf(zr - gp.w) : 1e9f, zl > 0.0f ? fabsf(zl - gp.w) : 1e9f);
+  const float gy = mn(zd > 0.0f ? fabsf(zd - gp.w) : 1e9f, zu > 0.0f ? fabsf(zu - gp.w) : 1e9f);
+  const float zg = mn(mx(gx, gy), gp.w);
+  float iz[9];
+#pragma unroll
+  for (int d2 = 1; d2 <= 2 * R * R; d2++)
+    if (tap_dist(d2) > 0.0) iz[d2] = 1.0f / (zg * (float)(tap_dist(d2) * STEP) + 1e-3f);
+  const float cw = (float)(Kern<R>::h[R] * Kern<R>::h[R]) * cc;
+  float sw = cw;
+  float scx = center.x * cw, scy = center.y * cw, scz = center.z * cw;
+  float sv = center.w * cw * cw;
+#pragma unroll
+  for (int dy = -R; dy <= R; dy++)
+#pragma unroll
+    for (int dx = -R; dx <= R; dx++) {
+      if (dx == 0 && dy == 0) continue;
+      const int o = dy * STEP * SW + dx * STEP;
+      const float4 gq = sG[c + o];
+      if (inside(qx + dx * STEP, qy + dy * STEP, w, h) && gq.w >= 0.0f) {
+        const float4 cq = sA[c + o];
+        const float ww = (float)(Kern<R>::h[dx + R] * Kern<R>::h[dy + R])
+                       * pow128(mx(dot3(make_float3(gq.x, gq.y, gq.z), make_float3(gp.x, gp.y, gp.z)), 0.0f))
+                       * expf(-fabsf(gq.w - gp.w) * iz[dx * dx + dy * dy] - sl * fabsf(lum(cq.x, cq.y, cq.z) - lc));
+        sw += ww;
+        scx += ww * cq.x; scy += ww * cq.y; scz += ww * cq.z;
+        sv += ww * ww * cq.w;
+      }
+    }
+  return make_float4(scx / sw, scy / sw, scz / sw, sv / (sw * sw));
+}
+
+// decoded half-res guide from an F3 texel: octDec(oct normal), block depth (hush.js:337)
+__device__ __forceinline__ float4 guide_f3(uint4 g3) {
+  float2 oc = unpack2(g3.x);
+  float3 n = oct_dec(oc.x, oc.y);
+  return make_float4(n.x, n.y, n.z, unpack2(g3.y).x);
+}
+// decoded guide from an rgba16float (oct.x, oct.y, depth, .) texel (hush.js:497, 566-567)
+__device__ __forceinline__ float4 guide_tex(float4 g) {
+  float3 n = oct_dec(g.x, g.y);
+  return make_float4(n.x, n.y, n.z, g.z);
+}
+
+// ============================================================================ pass 2: infer + mid1
+// INFER (hush.js:312-368): MLP 20-h-h-12 on the block features -> par0/par1/par2, then a-trous 3x3 step 1
+// with prefilter of the block means, s = qf16(s1), c = qf16(c1) -> Bh[0].
+
+template <int HID>
+__device__ __forceinline__ void mlp_fp32(const float *__restrict__ w, int hid, const float *f, float *o) {
+  const int h = HID > 0 ? HID : hid;
+  const float *W1 = w, *B1 = W1 + h * 20, *W2 = B1 + h, *B2 = W2 + h * h, *W3 = B2 + h, *B3 = W3 + 12 * h;
+  float h1[HID > 0 ? HID : 256], h2[HID > 0 ? HID : 256];
+#pragma unroll
+  for (int j = 0; j < h; j++) {
+    float acc = B1[j];
+#pragma unroll
+    for (int i = 0; i < 20; i++) acc = __builtin_fmaf(f[i], W1[j * 20 + i], acc);
+    h1[j] = mx(acc, 0.0f);
+  }
+#pragma unroll
+  for (int j = 0; j < h; j++) {
+    float acc = B2[j];
+#pragma unroll
+    for (int i = 0; i < h; i++) acc = __builtin_fmaf(h1[i], W2[j * h + i], acc);
+    h2[j] = mx(acc, 0.0f);
+  }
+#pragma unroll
+  for (int j = 0; j < 12; j++) {
+    float acc = B3[j];
+#pragma unroll
+    for (int i = 0; i < h; i++) acc = __builtin_fmaf(h2[i], W3[j * h + i], acc);
+    o[j] = acc;
+  }
+}
+
+// shader-f16 MLP of the reference's f16 option (hush.js:690, 699-702): binary16 inputs, weights,
+// products and sums, each rounded to nearest even
+__device__ __forceinline__ void mlp_f16(const float *__restrict__ w, int h, const float *f, float *o) {
+  const float *W1 = w, *B1 = W1 + h * 20, *W2 = B1 + h, *B2 = W2 + h * h, *W3 = B2 + h, *B3 = W3 + 12 * h;
+  _Float16 fh[20], h1[256], h2[256];
+  for (int i = 0; i < 20; i++) fh[i] = (_Float16)f[i];
+  for (int j = 0; j < h; j++) {
+    _Float16 acc = (_Float16)B1[j];
+    for (int i = 0; i < 20; i++) { _Float16 p = fh[i] * (_Float16)W1[j * 20 + i]; acc = acc + p; }
+    h1[j] = acc > (_Float16)0 ? acc : (_Float16)0;
+  }
+  for (int j = 0; j < h; j++) {
+    _Float16 acc = (_Float16)B2[j];
+    for (int i = 0; i < h; i++) { _Float16 p = h1[i] * (_Float16)W2[j * h + i]; acc = acc + p; }
+    h2[j] = acc > (_Float16)0 ? acc : (_Float16)0;
+  }
+  for (int j = 0; j < 12; j++) {
+    _Float16 acc = (_Float16)B3[j];
+    for (int i = 0; i < h; i++) { _Float16 p = h2[i] * (_Float16)W3[j * h + i]; acc = acc + p; }
+    o[j] = (float)acc;
+  }
+}
+
+// activations (hush.js:357-366) and the four half-res outputs of one texel
+__device__ __forceinline__ void infer_outputs(const InferArgs &a, const float4 *sA, const float4 *sG, int tx, int ty,
+                                              int X, int Y, const float *o) {
+  constexpr int SW = TW + 2;
+  const float alpha = 1.0f / 32.0f + (31.0f / 32.0f) * sigmoid(o[0]);
+  const float gamma = 0.5f + 7.5f * sigmoid(o[1]);
+  const float beta = sigmoid(o[2]);
+  const float kappa = sigmoid(o[3]);
+  float s[4], c[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    s[k] = 0.25f * expf(mn(mx(o[4 + k], -5.0f), 5.0f));
+    c[k] = expf(mn(mx(o[8 + k], -4.0f), 4.0f));
+  }
+  st(a.out0, X, Y, make_float4(alpha, gamma, beta, kappa));
+  st(a.out1, X, Y, make_float4(s[0], s[1], s[2], s[3]));
+  st(a.out2, X, Y, make_float4(c[0], c[1], c[2], c[3]));
+  st(a.outMid, X, Y, filt<1, 1, true, SW>(sA, sG, tx, ty, X, Y, a.W2, a.H2, qf16(s[1]), qf16(c[1])));
+}
+
+__device__ __forceinline__ void unpack_features(uint4 f0, uint4 f1, uint4 f2, float *f) {
+  uint32_t u[10] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x, f2.y};
+#pragma unroll
+  for (int i = 0; i < 10; i++) {
+    float2 t = unpack2(u[i]);
+    f[2 * i] = t.x; f[2 * i + 1] = t.y;
+  }
+}
+
+template <int HID, int MODE>   // MODE 0: fp32, 1: shader-f16 arithmetic
+__global__ __launch_bounds__(NT) void k_infer(InferArgs a) {
+  constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
+  __shared__ float4 sA[SN], sG[SN];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  for (int i = tid; i < SN; i += NT) {
+    int qx = clampi(x0 - AP + i % SW, 0, a.W2 - 1), qy = clampi(y0 - AP + i / SW, 0, a.H2 - 1);
+    uint4 g2 = ldu(a.F2, qx, qy), g3 = ldu(a.F3, qx, qy);
+    float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
+    sA[i] = make_float4(a01.x, a01.y, a23.x, a23.y);
+    sG[i] = guide_f3(g3);
+  }
+  __syncthreads();
+  const int lx = tid & 31, ly = tid >> 5;
+  const int X = x0 + lx, Y = y0 + ly;
+  if (!inside(X, Y, a.W2, a.H2)) return;
+  const int tx = lx + AP, ty = ly + AP;
+  if (sG[ty * SW + tx].w < 0.0f) {
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    st(a.out0, X, Y, z); st(a.out1, X, Y, z); st(a.out2, X, Y, z); st(a.outMid, X, Y, z);
+    return;
+  }
+  float f[20];
+  unpack_features(ldu(a.F0, X, Y), ldu(a.F1, X, Y), ldu(a.F2, X, Y), f);
+  float o[12];
+  if (MODE == 0) mlp_fp32<HID>(a.w, a.hidden, f, o);
+  else mlp_f16(a.w, a.hidden, f, o);
+  infer_outputs(a, sA, sG, tx, ty, X, Y, o);
+}
+
+// ============================================================================ passes 3, 4, 6: a-trous
+// atrousShader (hush.js:482-525): mid2 (R1 step 2) and mid3 (R1 step 4) at half res with the F3 guide
+// and learned s/c from par1/par2; coarse2 (R2 step 2, prefilter) at quarter res with the CG guide.
+
+template <int R, int STEP, bool PRE, bool GUIDE_F3, int COMP>
+__global__ __launch_bounds__(NT) void k_atrous(AtrousArgs a) {
+  constexpr int AP = R * STEP > 1 ? R * STEP : 1;
+  constexpr int SW = TW + 2 * AP, SH = TH + 2 * AP, SN = SW * SH;
+  __shared__ float4 sA[SN], sG[SN];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  for (int i = tid; i < SN; i += NT) {
+    int qx = clampi(x0 - AP + i % SW, 0, a.w - 1), qy = clampi(y0 - AP + i / SW, 0, a.h - 1);
+    sA[i] = ld(a.inA, qx, qy);
+    sG[i] = GUIDE_F3 ? guide_f3(ldu(a.guideF, qx, qy)) : guide_tex(ld(a.guideT, qx, qy));
+  }
+  __syncthreads();
+  const int lx = tid & 31, ly = tid >> 5;
+  const int X = x0 + lx, Y = y0 + ly;
+  if (!inside(X, Y, a.w, a.h)) return;
+  const int tx = lx + AP, ty = ly + AP;
+  if (sG[ty * SW + tx].w < 0.0f) { st(a.out, X, Y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
+  float s, c;
+  if (COMP < 0) { s = a.s; c = 1.0f; }
+  else {
+    float4 p1 = ld(a.par1, X, Y), p2 = ld(a.par2, X, Y);
+    s = COMP == 0 ? p1.x : COMP == 1 ? p1.y : COMP == 2 ? p1.z : p1.w;
+    c = COMP == 0 ? p2.x : COMP == 1 ? p2.y : COMP == 2 ? p2.z : p2.w;
+  }
+  st(a.out, X, Y, filt<R, STEP, PRE, SW>(sA, sG, tx, ty, X, Y, a.w, a.h, s, c));
+}
+
+// ============================================================================ pass 5: coarse1
+// COARSE1 (hush.js:372-422): count-weighted 2x2 half -> quarter down() of the block means and guides into
+// the tile, the coarse guide CG written once per texel, then a-trous 5x5 step 1 with prefilter -> C[1].
+
+__device__ __forceinline__ void down(const TexU &F2, const TexU &F3, int QX, int QY, int W2, int H2, float4 &col, float4 &g) {
+  float N = 0.0f, vr = 0.0f, z = 0.0f, clx = 0.0f, cly = 0.0f, clz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    int hx = QX * 2 + (j & 1), hy = QY * 2 + (j >> 1);
+    if (hx >= W2 || hy >= H2) continue;
+    uint4 g3 = ldu(F3, hx, hy);
+    float2 oz = unpack2(g3.y);
+    if (oz.y <= 0.0f) continue;
+    uint4 g2 = ldu(F2, hx, hy);
+    float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
+    float w = oz.y;
+    N += w;
+    clx += w * a01.x; cly += w * a01.y; clz += w * a23.x;
+    vr += w * a23.y;
+    float2 oc = unpack2(g3.x);
+    float3 d = oct_dec(oc.x, oc.y);
+    nx += w * d.x; ny += w * d.y; nz += w * d.z;
+    z += w * oz.x;
+  }
+  if (N == 0.0f) { col = make_float4(0.0f, 0.0f, 0.0f, 0.0f); g = make_float4(0.0f, 0.0f, -1.0f, 0.0f); return; }
+  float ln = mx(sqrtf(nx * nx + ny * ny + nz * nz), 1e-6f);
+  float2 o = oct_enc(make_float3(nx / ln, ny / ln, nz / ln));
+  col = make_float4(qf16(clx / N), qf16(cly / N), qf16(clz / N), qf16(vr / N / N));
+  g = make_float4(qf16(o.x), qf16(o.y), qf16(z / N), 0.0f);
+}
+
+__global__ __launch_bounds__(NT) void k_coarse1(Coarse1Args a) {
+  constexpr int AP = 2, SW = TW + 2 * AP, SH = TH + 2 * AP, SN = SW * SH;
+  __shared__ float4 sA[SN], sG[SN];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  for (int i = tid; i < SN; i += NT) {
+    int lqx = x0 - AP + i % SW, lqy = y0 - AP + i / SW;
+    int qx = clampi(lqx, 0, a.W4 - 1), qy = clampi(lqy, 0, a.H4 - 1);
+    float4 col, g;
+    down(a.F2, a.F3, qx, qy, a.W2, a.H2, col, g);
+    sA[i] = col;
+    sG[i] = guide_tex(g);
+    if (lqx == qx && lqy == qy && lqx >= x0 && lqy >= y0 && lqx < x0 + TW && lqy < y0 + TH) st(a.outG, qx, qy, g);
+  }
+  __syncthreads();
+  const int lx = tid & 31, ly = tid >> 5;
+  const int X = x0 + lx, Y = y0 + ly;
+  if (!inside(X, Y, a.W4, a.H4)) return;
+  const int tx = lx + AP, ty = ly + AP;
+  if (sG[ty * SW + tx].w < 0.0f) { st(a.outC, X, Y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
+  st(a.outC, X, Y, filt<2, 1, true, SW>(sA, sG, tx, ty, X, Y, a.W4, a.H4, a.s, 1.0f));
+}
+
+// ============================================================================ pass 7: fine + final
+// FINE_FINAL (hush.js:530-622): a-trous 3x3 step 1 with prefilter of A0 (guide key[cur]) with s0/c0 ->
+// histColor[cur]; joint-bilateral upsample of the mid (Bh[0]) and coarse (C[0]) levels; recombine with
+// kappa and beta; remodulate -> X.
+
+__device__ __forceinline__ float jbw(float frx, float fry, int j, float3 nq, float zq, float3 n, float z) {
+  float wb = ((j & 1) == 1 ? frx : 1.0f - frx) * ((j >> 1) == 1 ? fry : 1.0f - fry);
+  float dn = mx(dot3(nq, n), 0.0f);
+  float dn2 = dn * dn;
+  float dn4 = dn2 * dn2;
+  return wb * expf(-fabsf(zq - z) / (0.05f * z + 1e-3f)) * dn4 * dn4;
+}
+
+__global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
+  constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
+  __shared__ float4 sA[SN], sG[SN];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  for (int i = tid; i < SN; i += NT) {
+    int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
+    sA[i] = ld(a.A0, qx, qy);
+    sG[i] = guide_tex(ld(a.key, qx, qy));
+  }
+  __syncthreads();
+  const int lx = tid & 31, ly = tid >> 5;
+  const int px = x0 + lx, py = y0 + ly;
+  if (!inside(px, py, a.W, a.H)) return;
+  const int tx = lx + AP, ty = ly + AP;
+  const float4 alb = ld(a.mat, px, py), emi = ld(a.emi, px, py);
+  const float4 gp = sG[ty * SW + tx];
+  if (gp.w < 0.0f) {
+    st(a.outHist, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    st(a.outX, px, py, make_float4(emi.x, emi.y, emi.z, 0.0f));
+    return;
+  }
+  const int hx = px >> 1, hy = py >> 1;
+  const float4 p1 = ld(a.par1, hx, hy), p2 = ld(a.par2, hx, hy), pr = ld(a.par0, hx, hy);
+  const float4 A1 = filt<1, 1, true, SW>(sA, sG, tx, ty, px, py, a.W, a.H, p1.x, p2.x);
+  st(a.outHist, px, py, A1);
+  const float fx = qf16(A1.x), fy = qf16(A1.y), fz = qf16(A1.z);
+  const float3 n = make_float3(gp.x, gp.y, gp.z);
+  const float z = gp.w;
+  // half-res mid level (hush.js:585-602)
+  const float hpx = ((float)px + 0.5f) / 2.0f - 0.5f, hpy = ((float)py + 0.5f) / 2.0f - 0.5f;
+  const float hbx = floorf(hpx), hby = floorf(hpy);
+  float uBx = 0.0f, uBy = 0.0f, uBz = 0.0f, uAx = 0.0f, uAy = 0.0f, uAz = 0.0f, swh = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    int qx = (int)hbx + (j & 1), qy = (int)hby + (j >> 1);
+    if (qx < 0 || qy < 0 || qx >= a.W2 || qy >= a.H2) continue;
+    uint4 g3 = ldu(a.F3, qx, qy);
+    float2 oz = unpack2(g3.y);
+    if (oz.x < 0.0f) continue;
+    float2 oc = unpack2(g3.x);
+    float w = jbw(hpx - hbx, hpy - hby, j, oct_dec(oc.x, oc.y), oz.x, n, z);
+    uint4 g2 = ldu(a.F2, qx, qy);
+    float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
+    swh += w;
+    float4 m = ld(a.mid, qx, qy);
+    uBx += w * m.x; uBy += w * m.y; uBz += w * m.z;
+    uAx += w * a01.x; uAy += w * a01.y; uAz += w * a23.x;
+  }
+  float ix = fx, iy = fy, iz = fz;
+  if (swh > 1e-4f) {
+    ix = uBx / swh + pr.w * (fx - uAx / swh);
+    iy = uBy / swh + pr.w * (fy - uAy / swh);
+    iz = uBz / swh + pr.w * (fz - uAz / swh);
+  }
+  // quarter-res coarse level (hush.js:604-619)
+  const float cpx = ((float)px + 0.5f) / 4.0f - 0.5f, cpy = ((float)py + 0.5f) / 4.0f - 0.5f;
+  const float cbx = floorf(cpx), cby = floorf(cpy);
+  float uCx = 0.0f, uCy = 0.0f, uCz = 0.0f, swq = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    int qx = (int)cbx + (j & 1), qy = (int)cby + (j >> 1);
+    if (qx < 0 || qy < 0 || qx >= a.W4 || qy >= a.H4) continue;
+    float4 g = ld(a.cguide, qx, qy);
+    if (g.z < 0.0f) continue;
+    float w = jbw(cpx - cbx, cpy - cby, j, oct_dec(g.x, g.y), g.z, n, z);
+    swq += w;
+    float4 cc = ld(a.coarse, qx, qy);
+    uCx += w * cc.x; uCy += w * cc.y; uCz += w * cc.z;
+  }
+  float upx = ix, upy = iy, upz = iz;
+  if (swq > 1e-4f) { upx = uCx / swq; upy = uCy / swq; upz = uCz / swq; }
+  ix = ix + (upx - ix) * pr.z;
+  iy = iy + (upy - iy) * pr.z;
+  iz = iz + (upz - iz) * pr.z;
+  st(a.outX, px, py, make_float4(ix * alb.x + emi.x, iy * alb.y + emi.y, iz * alb.z + emi.z, 0.0f));
+}
+
+// ============================================================================ pass 8: TAA
+// TAA (hush.js:625-678): YCoCg variance clip (gamma 1.25) of a manually bilinear history, blend 0.1 in
+// Karis tone-mapped space.
+
+__device__ __forceinline__ float3 tm(float3 c) { float d = 1.0f + lum(c.x, c.y, c.z); return make_float3(c.x / d, c.y / d, c.z / d); }
+__device__ __forceinline__ float3 itm(float3 c) {
+  float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f);
+  return make_float3(c.x / d, c.y / d, c.z / d);
+}
+__device__ __forceinline__ float3 to_ycocg(float3 c) {
+  return make_float3(c.x * 0.25f + c.y * 0.5f + c.z * 0.25f, c.x * 0.5f - c.z * 0.5f, -c.x * 0.25f + c.y * 0.5f - c.z * 0.25f);
+}
+__device__ __forceinline__ float3 from_ycocg(float3 c) { return make_float3(c.x + c.y - c.z, c.x + c.z, c.x - c.y - c.z); }
+
+__global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
+  constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
+  __shared__ float4 sX[SN];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  for (int i = tid; i < SN; i += NT) {
+    int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
+    float4 x = ld(a.X, qx, qy);
+    float3 y = to_ycocg(tm(make_float3(x.x, x.y, x.z)));
+    sX[i] = make_float4(y.x, y.y, y.z, 0.0f);
+  }
+  __syncthreads();
+  const int lx = tid & 31, ly = tid >> 5;
+  const int px = x0 + lx, py = y0 + ly;
+  if (px >= a.W || py >= a.H) return;
+  const float4 xc = ld(a.X, px, py);
+  const float3 cur = tm(make_float3(xc.x, xc.y, xc.z));
+  if (a.reset) {
+    float3 r = itm(cur);
+    st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
+    return;
+  }
+  const int c0 = (ly + AP) * SW + lx + AP;
+  float m1x = 0.0f, m1y = 0.0f, m1z = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 9; j++) {
+    float4 s = sX[c0 + (j / 3 - 1) * SW + (j % 3 - 1)];
+    m1x += s.x; m1y += s.y; m1z += s.z;
+  }
+  m1x /= 9.0f; m1y /= 9.0f; m1z /= 9.0f;
+  float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 9; j++) {
+    float4 s = sX[c0 + (j / 3 - 1) * SW + (j % 3 - 1)];
+    float dx = s.x - m1x, dy = s.y - m1y, dz = s.z - m1z;
+    vx += dx * dx; vy += dy * dy; vz += dz * dz;
+  }
+  const float sdx = sqrtf(mx(vx / 9.0f, 1e-12f)), sdy = sqrtf(mx(vy / 9.0f, 1e-12f)), sdz = sqrtf(mx(vz / 9.0f, 1e-12f));
+  const float4 mv = ld(a.mv, px, py);
+  const float szx = (float)a.W, szy = (float)a.H;
+  const float ux = ((float)px + 0.5f + mv.x) / szx, uy = ((float)py + 0.5f + mv.y) / szy;
+  float3 res = cur;
+  if (ux >= 0.0f && uy >= 0.0f && ux <= 1.0f && uy <= 1.0f) {
+    float ppx = ux * szx - 0.5f, ppy = uy * szy - 0.5f;
+    float bx = floorf(ppx), by = floorf(ppy);
+    float fx = ppx - bx, fy = ppy - by;
+    float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int ox = j & 1, oy = j >> 1;
+      float wb = (ox == 1 ? fx : 1.0f - fx) * (oy == 1 ? fy : 1.0f - fy);
+      float4 t = ld(a.prev, clampi((int)bx + ox, 0, a.W - 1), clampi((int)by + oy, 0, a.H - 1));
+      hx += wb * t.x; hy += wb * t.y; hz += wb * t.z;
+    }
+    float3 hh = to_ycocg(tm(make_float3(hx, hy, hz)));
+    float kx = 1.25f * sdx, ky = 1.25f * sdy, kz = 1.25f * sdz;
+    hh.x = mn(mx(hh.x, m1x - kx), m1x + kx);
+    hh.y = mn(mx(hh.y, m1y - ky), m1y + ky);
+    hh.z = mn(mx(hh.z, m1z - kz), m1z + kz);
+    float3 hr = from_ycocg(hh);
+    res = make_float3(hr.x + (cur.x - hr.x) * 0.1f, hr.y + (cur.y - hr.y) * 0.1f, hr.z + (cur.z - hr.z) * 0.1f);
+  }
+  float3 r = itm(res);
+  st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
+}
+
+// ============================================================================ launchers
+
+static dim3 grid_for(int w, int h) { return dim3((w + TW - 1) / TW, (h + TH - 1) / TH); }
+
+hipError_t launch_temporal(const TemporalArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_temporal, grid_for(a.W, a.H), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_infer(const InferArgs &a, int mode, hipStream_t s) {
+  dim3 g = grid_for(a.W2, a.H2);
+  if (mode == MLP_FP32) {
+    if (a.hidden == 16) hipLaunchKernelGGL((k_infer<16, 0>), g, dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_infer<0, 0>), g, dim3(NT), 0, s, a);
+  } else if (mode == MLP_F16) {
+    hipLaunchKernelGGL((k_infer<0, 1>), g, dim3(NT), 0, s, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mid2(const AtrousArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL((k_atrous<1, 2, false, true, 2>), grid_for(a.w, a.h), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_mid3(const AtrousArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL((k_atrous<1, 4, false, true, 3>), grid_for(a.w, a.h), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_coarse1(const Coarse1Args &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_coarse1, grid_for(a.W4, a.H4), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_coarse2(const AtrousArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL((k_atrous<2, 2, true, false, -1>), grid_for(a.w, a.h), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_fine_final(const FineArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_fine_final, grid_for(a.W, a.H), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_taa(const TaaArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_taa, grid_for(a.W, a.H), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace hush

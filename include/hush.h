@@ -1,0 +1,139 @@
+/*
+ * hush.h — C ABI of the MI355X (gfx950) Hush denoiser.
+ *
+ * Replaces the WebGPU `Hush` class of kanaworksai-hue/hush (src/denoise/hush.js:725-875, typed in
+ * src/index.d.ts:36-60).  Device buffers stand in for GPUTextures: every surface is a row-major array of
+ * rgba16float texels (four IEEE binary16 values, 8 bytes) in HBM with a byte pitch.  The five inputs have
+ * the G-buffer layout of src/index.d.ts:2-18:
+ *   radiance     rgb = noisy 1 spp radiance without the primary hit's emission
+ *   normal_depth xyz = world normal, w = linear view depth (< 0 = background / sky)
+ *   motion       xy = previous - current pixel position (pixels), z = object id (< 2048), w = previous depth
+ *   material     rgb = albedo (1 for glass / lights / sky), a = roughness
+ *   emission     rgb = emission of the primary hit
+ * The output is rgba16float linear HDR (TAA-resolved), alpha 1.
+ *
+ * Errors: every call returns HUSH_OK (0) or a negative code; hush_last_error() gives the message, with
+ * the reference's texts (hush.js:735, 743, 765, 816, 818).
+ * Threading: one context per stream; calls on one context must be serialised by the caller.  Kernel
+ * arguments are captured at launch, so unlike the reference (hush.js:811-812) a caller may change inputs
+ * or exposure before the previous frame has executed.  The output of frame N is overwritten by frame N+2
+ * (two alternating buffers, hush.js:796-798).
+ * Streams: `stream` is a hipStream_t (NULL = the default stream) of the context's device.
+ */
+#ifndef HUSH_H
+#define HUSH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hush_ctx hush_t;
+
+enum hush_status {
+  HUSH_OK = 0,
+  HUSH_ERR_ARGUMENT = -1,     /* null pointer or invalid enum */
+  HUSH_ERR_WEIGHTS = -2,      /* weight count fits no MLP width (hush.js:743) */
+  HUSH_ERR_SIZE = -3,         /* non-positive size (hush.js:765) */
+  HUSH_ERR_INPUT = -4,        /* missing input or size mismatch (hush.js:816, 818) */
+  HUSH_ERR_DEVICE = -5,       /* HIP runtime error (allocation, launch) */
+  HUSH_ERR_UNSUPPORTED = -6   /* precision / width combination not built */
+};
+
+/* MLP arithmetic (the reference's `f16` option is HUSH_MLP_F16; the MFMA forms have no counterpart) */
+enum hush_precision {
+  HUSH_MLP_FP32 = 0,          /* plain FP32 MLP: the drop-in (hush.js:683-704 with f16 = false) */
+  HUSH_MLP_F16 = 1,           /* binary16 arithmetic, like shader-f16 (hush.js:690, 699-702) */
+  HUSH_MLP_FP16_MFMA = 2,     /* fp16 operands on MFMA, fp32 accumulation */
+  HUSH_MLP_BF16_MFMA = 3,     /* bf16 operands on MFMA, fp32 accumulation */
+  HUSH_MLP_FP8_MFMA = 4       /* fp8 E4M3 operands with per-layer scales on MFMA, fp32 accumulation */
+};
+
+enum hush_format { HUSH_RGBA16F = 0, HUSH_RGBA32U = 1, HUSH_R8U = 2 };
+
+/* a device surface: `ptr` points at texel (0, 0), rows are `pitch` bytes apart */
+typedef struct {
+  void *ptr;
+  size_t pitch;
+  int32_t width, height;
+  int32_t format;             /* enum hush_format; inputs and output are HUSH_RGBA16F */
+} hush_surface;
+
+/* the per-frame G-buffer (HushInputs, src/index.d.ts:11-17) */
+typedef struct {
+  hush_surface radiance, normal_depth, motion, material, emission;
+} hush_inputs;
+
+/* hiddenOf (hush.js:22-25): the MLP width h with h*h + 34h + 12 == count, h in [1, 256]; 0 if none */
+int32_t hush_hidden_of(size_t count);
+
+/* new Hush(device, {width, height, exposure, weights, f16}) (hush.js:731-761).  `weights` holds `count`
+   binary16 values in the layout of hush-weights.bin (W1 b1 W2 b2 W3 b3, row-major (out, in),
+   hush.js:20-21).  Allocates the history and level surfaces (hush.js:780-793) on `device`.
+   On failure *out is NULL and hush_last_error(NULL) holds the message. */
+int32_t hush_create(int32_t device, int32_t width, int32_t height, float exposure, const uint16_t *weights,
+                    size_t count, int32_t precision, hush_t **out);
+
+/* destroy() (hush.js:803-807) */
+void hush_destroy(hush_t *h);
+
+/* resize(width, height) (hush.js:764-794): no-op if unchanged, else reallocates and resets the history */
+int32_t hush_resize(hush_t *h, int32_t width, int32_t height);
+
+/* reset() (hush.js:801): the next frame starts from scratch */
+int32_t hush_reset(hush_t *h);
+
+/* the `exposure` field (index.d.ts:42-43): read at the next frame */
+int32_t hush_set_exposure(hush_t *h, float exposure);
+float hush_get_exposure(const hush_t *h);
+
+/* width / height getters (index.d.ts:38-39), the MLP width and the frame counter */
+int32_t hush_get_size(const hush_t *h, int32_t *width, int32_t *height);
+int32_t hush_hidden(const hush_t *h);
+int64_t hush_frame(const hush_t *h);
+
+/* MLP precision of later frames (enum hush_precision) */
+int32_t hush_set_precision(hush_t *h, int32_t precision);
+int32_t hush_get_precision(const hush_t *h);
+
+/* denoise(encoder, inputs) (hush.js:813-822): validates the five inputs, enqueues the 8 passes on
+   `stream` and returns the output surface in *out (may be NULL). */
+int32_t hush_denoise(hush_t *h, const hush_inputs *in, void *stream, hush_surface *out);
+
+/* encode(encoder, inputs) (hush.js:835-874): denoise() without the input checks */
+int32_t hush_encode(hush_t *h, const hush_inputs *in, void *stream, hush_surface *out);
+
+/* the `output` getter (hush.js:798): the last frame's output */
+int32_t hush_output(const hush_t *h, hush_surface *out);
+
+/* last error message of `h`, or of the calling thread's last failed hush_create when h is NULL */
+const char *hush_last_error(const hush_t *h);
+
+/* Per-pass timing, the counterpart of the optional `timer` argument (hush.js:833-856, gpu.js:25-77).
+   When enabled, every frame records hipEvents around its 8 passes.  hush_pass_times waits for the last
+   frame and writes 8 durations in milliseconds in dispatch order; labels[i] (optional) receive the
+   reference's labels: temporal, infer+mid, infer+mid, infer+mid, coarse, coarse, fine+final, taa. */
+int32_t hush_set_timing(hush_t *h, int32_t enabled);
+int32_t hush_pass_times(hush_t *h, float *ms, const char **labels);
+
+/* ---- parity and test hooks (no counterpart in the reference) ---- */
+
+/* internal surfaces by id (hush.js:783-793): 0 A0, 1 X, 2 Bh0, 3 Bh1, 4 mom0, 5 mom1, 6 key0, 7 key1,
+   8 histColor0, 9 histColor1, 10 taa0, 11 taa1, 12 par0_0, 13 par0_1, 14 par1, 15 par2, 16..19 F0..F3,
+   20 C0, 21 C1, 22 CG, 23 reprojection mask (HUSH_R8U; written only while enabled) */
+int32_t hush_debug_surface(hush_t *h, int32_t id, hush_surface *out);
+/* set the frame counter and the reset flag (teacher-forced single passes) */
+int32_t hush_debug_set_state(hush_t *h, int64_t frame, int32_t reset);
+/* run one pass (0 temporal .. 7 taa, dispatch order) at the current state without advancing */
+int32_t hush_debug_run_pass(hush_t *h, int32_t pass, const hush_inputs *in, void *stream);
+/* enable the per-pixel reprojection mask: bit i (0..3) = bilinear tap i valid, bit 4 = 3x3 fallback
+   used, bit 5 = history used (has), bit 7 = sky */
+int32_t hush_set_debug_mask(hush_t *h, int32_t enabled);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HUSH_H */

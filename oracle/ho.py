@@ -1,0 +1,188 @@
+"""ctypes wrapper of the CPU oracle (oracle/libhush_oracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the smoke check and the CPU-baseline leg of bench.py import this module, and only as the
+checker; the denoiser itself never calls it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libhush_oracle.so")
+
+SURF = {"A0": 0, "X": 1, "Bh0": 2, "Bh1": 3, "mom0": 4, "mom1": 5, "key0": 6, "key1": 7, "hist0": 8, "hist1": 9,
+        "taa0": 10, "taa1": 11, "par0_0": 12, "par0_1": 13, "par1": 14, "par2": 15, "F": 16, "C0": 17, "C1": 18,
+        "CG": 19}
+MODES = {"fp32": ("ho32_", ctypes.c_float, np.float32), "fp64q": ("ho64q_", ctypes.c_double, np.float64),
+         "fp64": ("ho64_", ctypes.c_double, np.float64)}
+_L = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB):
+            build()
+        _L = ctypes.CDLL(LIB)
+    return _L
+
+
+class Oracle:
+    """One FP32-mirror (or FP64) restatement of the Hush class state machine."""
+
+    def __init__(self, width, height, weights, exposure=1.0, f16=False, mode="fp32"):
+        L = load()
+        self.mode = mode
+        pre, self._ct, self._np = MODES[mode]
+        self._f = {}
+        for name in ("create", "destroy", "resize", "reset", "set_exposure", "set_state", "frame", "hidden",
+                     "surface", "mask", "features_in", "set_inputs", "run_pass", "denoise", "output_id", "mlp",
+                     "hidden_of", "f2h_rtz", "h2f", "qf16", "oct_enc", "oct_dec", "check"):
+            self._f[name] = getattr(L, pre + name)
+        F, ct = self._f, self._ct
+        F["create"].restype = ctypes.c_void_p
+        F["create"].argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_long, ct, ctypes.c_int]
+        F["destroy"].argtypes = [ctypes.c_void_p]
+        F["resize"].argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        F["reset"].argtypes = [ctypes.c_void_p]
+        F["set_exposure"].argtypes = [ctypes.c_void_p, ctypes.c_double]
+        F["set_state"].argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        F["frame"].argtypes = [ctypes.c_void_p]
+        F["surface"].restype = ctypes.c_void_p
+        F["surface"].argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+        F["mask"].restype = ctypes.c_void_p
+        F["mask"].argtypes = [ctypes.c_void_p]
+        F["features_in"].restype = ctypes.c_void_p
+        F["features_in"].argtypes = [ctypes.c_void_p]
+        F["set_inputs"].argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 5
+        F["run_pass"].argtypes = [ctypes.c_void_p, ctypes.c_int]
+        F["denoise"].argtypes = [ctypes.c_void_p]
+        F["output_id"].argtypes = [ctypes.c_void_p]
+        F["mlp"].argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        F["hidden_of"].argtypes = [ctypes.c_long]
+        F["f2h_rtz"].restype = ctypes.c_uint16
+        F["f2h_rtz"].argtypes = [ct]
+        F["h2f"].restype = ct
+        F["h2f"].argtypes = [ctypes.c_uint16]
+        F["qf16"].restype = ct
+        F["qf16"].argtypes = [ct]
+        F["oct_enc"].argtypes = [ct, ct, ct, ctypes.c_void_p]
+        F["oct_dec"].argtypes = [ct, ct, ctypes.c_void_p]
+        F["check"].argtypes = [ctypes.c_void_p, ctypes.c_void_p, ct, ct]
+        w = np.ascontiguousarray(weights, dtype=np.uint16)
+        self._w = w
+        self._c = F["create"](int(width), int(height), w.ctypes.data, int(w.size), exposure, int(bool(f16)))
+        if not self._c:
+            raise ValueError("oracle: bad weights or size")
+        self.width, self.height = int(width), int(height)
+        self._inputs = None
+
+    def close(self):
+        if self._c:
+            self._f["destroy"](self._c)
+            self._c = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state machine
+    def resize(self, w, h):
+        r = self._f["resize"](self._c, int(w), int(h))
+        if r != 0:
+            raise ValueError(f"Hush: bad size {w}x{h}")
+        self.width, self.height = int(w), int(h)
+
+    def reset(self):
+        self._f["reset"](self._c)
+
+    def set_exposure(self, e):
+        self._f["set_exposure"](self._c, float(e))
+
+    def set_state(self, frame, reset):
+        self._f["set_state"](self._c, int(frame), int(bool(reset)))
+
+    @property
+    def frame(self):
+        return int(self._f["frame"](self._c))
+
+    def set_inputs(self, g: dict):
+        arrs = [np.ascontiguousarray(g[k], dtype=np.uint16) for k in ("radiance", "normalDepth", "motion", "material", "emission")]
+        for a in arrs:
+            assert a.shape == (self.height, self.width, 4), a.shape
+        self._inputs = arrs
+        self._f["set_inputs"](self._c, *[a.ctypes.data for a in arrs])
+
+    def run_pass(self, p: int):
+        self._f["run_pass"](self._c, int(p))
+
+    def denoise(self, g: dict | None = None) -> np.ndarray:
+        if g is not None:
+            self.set_inputs(g)
+        sid = self._f["denoise"](self._c)
+        return self.surface(sid).copy()
+
+    def output(self) -> np.ndarray:
+        return self.surface(self._f["output_id"](self._c)).copy()
+
+    # ---- surfaces (views into the oracle's memory; write through them to teacher-force)
+    def surface(self, sid) -> np.ndarray:
+        sid = SURF[sid] if isinstance(sid, str) else int(sid)
+        w, h, ch = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        p = self._f["surface"](self._c, sid, ctypes.byref(w), ctypes.byref(h), ctypes.byref(ch))
+        buf = (self._ct * (w.value * h.value * ch.value)).from_address(p)
+        return np.ctypeslib.as_array(buf).reshape(h.value, w.value, ch.value)
+
+    def mask(self) -> np.ndarray:
+        p = self._f["mask"](self._c)
+        buf = (ctypes.c_uint8 * (self.width * self.height)).from_address(p)
+        return np.ctypeslib.as_array(buf).reshape(self.height, self.width)
+
+    def features_in(self) -> np.ndarray:
+        p = self._f["features_in"](self._c)
+        buf = (self._ct * (self.width * self.height * 20)).from_address(p)
+        return np.ctypeslib.as_array(buf).reshape(self.height, self.width, 20)
+
+    # ---- scalar helpers
+    def mlp(self, f: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(f, dtype=self._np)
+        o = np.zeros(12, self._np)
+        self._f["mlp"](self._c, f.ctypes.data, o.ctypes.data)
+        return o
+
+    def f2h_rtz(self, x) -> int:
+        return int(self._f["f2h_rtz"](x))
+
+    def h2f(self, h) -> float:
+        return float(self._f["h2f"](int(h)))
+
+    def qf16(self, x) -> float:
+        return float(self._f["qf16"](x))
+
+    def oct_enc(self, n):
+        o = np.zeros(2, self._np)
+        self._f["oct_enc"](float(n[0]), float(n[1]), float(n[2]), o.ctypes.data)
+        return o
+
+    def oct_dec(self, e):
+        o = np.zeros(3, self._np)
+        self._f["oct_dec"](float(e[0]), float(e[1]), o.ctypes.data)
+        return o
+
+    def check(self, k, n, zprev, obj) -> bool:
+        k = np.ascontiguousarray(k, dtype=self._np)
+        n = np.ascontiguousarray(n, dtype=self._np)
+        return bool(self._f["check"](k.ctypes.data, n.ctypes.data, zprev, obj))
+
+    def hidden_of(self, n) -> int:
+        return int(self._f["hidden_of"](int(n)))
