@@ -38,6 +38,20 @@ def algorithmic_bytes(w: int, h: int) -> dict:
             "taa": 32 * N}
 
 
+def measured_traffic(kernel: str, grid: int, precision: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    this workload (counters scaled by the copy-kernel calibration recorded beside them); None if absent."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{grid}_{precision}.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        d = json.load(fh)
+    for k, v in d["kernels"].items():
+        if k.startswith(kernel + "/"):
+            return int(v["traffic_bytes"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(frames_g, w, h, n_frames):
     """The oracle (FP32 mirror, C + OpenMP) timed on the host on the first `n_frames` frames."""
     from oracle.ho import Oracle
@@ -66,6 +80,9 @@ def main():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--cpu-frames", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run device-to-device copies of this size (the runtime's copy kernel calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the denoiser path runs whole frames on one device); "
@@ -129,9 +146,21 @@ def main():
                   "hbm_frac": round(ab[n] / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)} for n, m in zip(names, pass_ms)}
     tk = float(pass_ms[0])
     achieved = ab["temporal"] / (tk * 1e-3) / 1e9
+    traffic, traffic_src = (measured_traffic("k_temporal", "1080p_boxes", args.precision)
+                            if (W, H) == (1920, 1080) else (None, None))
     roofline = {"kernel": "temporal", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": ab["temporal"]}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": traffic_src, "algorithmic_bytes": ab["temporal"],
+                "algorithmic_bytes_rule": "80*N + 72*N2 (N = W*H, N2 = ceil(W/2)*ceil(H/2))"}
+
+    if args.copy_probe_mb:
+        nb = args.copy_probe_mb << 20
+        src, dst = hip.DeviceBuffer(nb), hip.DeviceBuffer(nb)
+        for _ in range(4):
+            hip.check(hip.lib().hipMemcpy(hip.ctypes.c_void_p(dst.ptr), hip.ctypes.c_void_p(src.ptr), nb,
+                                          hip.hipMemcpyDeviceToDevice), "copy probe")
+        hip.synchronize()
+        print(f"[bench] copy probe: 4 x {nb} bytes device-to-device", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

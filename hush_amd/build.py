@@ -63,9 +63,11 @@ def build_node():
         print("node addon: skipped (no node_api.h or source)")
         return None
     out = os.path.join(HERE, "node", "hush_napi.node")
-    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + inc, "-I" + os.path.join(ROOT, "include"),
-         "-DNAPI_VERSION=8", src, "-o", out + ".tmp", "-L" + HERE, "-lhush", "-Wl,-rpath,$ORIGIN/..",
-         "-Wl,--allow-shlib-undefined"])
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + inc, "-I" + os.path.join(ROOT, "include"),
+         "-I" + os.path.join(rocm, "include"), "-D__HIP_PLATFORM_AMD__", "-DNODE_GYP_MODULE_NAME=hush_napi",
+         src, "-o", out + ".tmp", "-L" + HERE, "-lhush", "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
+         "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + os.path.join(rocm, "lib"), "-Wl,--allow-shlib-undefined"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -1,0 +1,147 @@
+'use strict';
+// Hush denoiser for MI355X: the reference's JavaScript API (src/denoise/hush.js:725-875, src/index.d.ts:36-60)
+// over the N-API addon (hush_napi.node) and the HIP C ABI (include/hush.h).
+//
+// Device surfaces stand in for GPUTextures: rgba16float row-major buffers in HBM, created with
+// `device.createTexture(width, height)` and filled with `upload(Uint16Array)`.  Everything else keeps its
+// meaning: `new Hush(device, {width, height, exposure, weights, f16})`, `denoise(inputs, timer)`, `output`,
+// `resize`, `reset`, `exposure`, `destroy`, `hiddenOf`, `Hush.requiredLimits`, and the error messages.
+
+const fs = require('fs');
+const path = require('path');
+const native = require('./hush_napi.node');
+
+const N_FEAT = 20;
+const N_OUT = 12;
+const INPUTS = ['radiance', 'normalDepth', 'motion', 'material', 'emission'];
+const PRECISION = { fp32: 0, f16: 1, 'fp16-mfma': 2, 'bf16-mfma': 3, 'fp8-mfma': 4 };
+const DEFAULT_WEIGHTS_PATH = path.join(__dirname, '..', 'data', 'hush-weights.bin');
+
+// hiddenOf (hush.js:22-25): the MLP width h with h*h + (N_FEAT + N_OUT + 2) h + N_OUT == n, 0 if none
+function hiddenOf(n) {
+  for (let h = 1; h <= 256; h++) if (h * h + (N_FEAT + N_OUT + 2) * h + N_OUT === n) return h;
+  return 0;
+}
+
+// decodeWeights (hush.js:713-717): base64 string, typed array or ArrayBuffer -> Uint16Array
+function decodeWeights(w) {
+  if (typeof w === 'string') {
+    const b = Buffer.from(w, 'base64');
+    return new Uint16Array(b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength));
+  }
+  if (ArrayBuffer.isView(w)) return new Uint16Array(w.buffer.slice(w.byteOffset, w.byteOffset + w.byteLength));
+  return new Uint16Array(w);
+}
+
+function defaultWeights() {
+  const b = fs.readFileSync(DEFAULT_WEIGHTS_PATH);
+  return new Uint16Array(b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength));
+}
+
+// An rgba16float device surface (the GPUTexture of the reference's API).
+class DeviceTexture {
+  constructor(device, width, height, surface) {
+    this.device = device;
+    this.width = width;
+    this.height = height;
+    this.format = 'rgba16float';
+    this._s = surface || native.alloc(device.ordinal, width, height);
+    this._owned = !surface;
+  }
+  get surface() { return this._s; }
+  // binary16 bit patterns, width * height * 4 values, row-major
+  upload(data) { native.upload(this._s, data); return this; }
+  download() { return native.download(this._s); }
+  destroy() {
+    if (this._owned && this._s) native.free(this._s);
+    this._s = null;
+  }
+}
+
+// The device: a HIP ordinal with the limits object the reference checks (gpu.js:3-22 initDevice).
+function initDevice(ordinal = 0) {
+  const n = native.deviceCount();
+  if (!(ordinal >= 0 && ordinal < n)) throw new Error(`HIP device ${ordinal} not available (${n} devices)`);
+  const device = {
+    ordinal,
+    limits: { maxStorageTexturesPerShaderStage: 8 },
+    features: new Set(['shader-f16']),
+    createTexture(width, height) { return new DeviceTexture(device, width, height); },
+  };
+  return device;
+}
+
+const REQUIRED_LIMITS = Object.freeze({ maxStorageTexturesPerShaderStage: 8 });
+
+class Hush {
+  // options: { width, height, exposure = 1, weights = built-in, f16 = false, precision = 'fp32' }
+  constructor(device, options = {}) {
+    const { width, height, exposure = 1, weights, f16 = false } = options;
+    for (const [k, v] of Object.entries(REQUIRED_LIMITS)) {
+      if (!(device.limits && device.limits[k] >= v)) {
+        throw new Error(`Hush needs the device limit ${k} >= ${v} (this device: ${device.limits && device.limits[k]}); create the device with requiredLimits: Hush.requiredLimits`);
+      }
+    }
+    this.device = device;
+    const u16 = weights === undefined ? defaultWeights() : decodeWeights(weights);
+    this.hidden = hiddenOf(u16.length);
+    if (!this.hidden) throw new Error(`hush weights: ${u16.length} f16 values fit no MLP size`);
+    if (!(width > 0 && height > 0)) throw new Error(`Hush: bad size ${width}x${height}`);
+    const precision = options.precision !== undefined ? options.precision : (f16 ? 'f16' : 'fp32');
+    this.f16 = precision === 'f16';
+    this._h = native.create(device.ordinal, width, height, exposure, u16, PRECISION[precision]);
+    this._exposure = exposure;
+    this._timed = false;
+  }
+
+  get width() { return native.info(this._h).width; }
+  get height() { return native.info(this._h).height; }
+  get frame() { return native.info(this._h).frame; }
+
+  // Display exposure the network sees (index.d.ts:42-43); read at the next frame.
+  get exposure() { return this._exposure; }
+  set exposure(e) { this._exposure = e; native.setExposure(this._h, e); }
+
+  // Reallocates for a new input size (no-op if unchanged) and resets the history (hush.js:764-794).
+  resize(width, height) {
+    if (!(width > 0 && height > 0)) throw new Error(`Hush: bad size ${width}x${height}`);
+    native.resize(this._h, width, height);
+  }
+
+  // Output of the last denoise() (hush.js:798): two surfaces alternate between frames.
+  get output() { return new DeviceTexture(this.device, this.width, this.height, native.output(this._h)); }
+
+  // Drops the history (camera cut, scene switch) (hush.js:801).
+  reset() { native.reset(this._h); }
+
+  destroy() {
+    if (this._h) native.destroy(this._h);
+    this._h = null;
+  }
+
+  // One frame (hush.js:813-822). inputs = { radiance, normalDepth, motion, material, emission }: DeviceTextures
+  // of the denoiser's size (extra fields are ignored). timer (optional): an object whose `record(passes)` receives
+  // [{label, ms}] for the 8 passes of this frame, in the reference's label set.
+  denoise(inputs, timer, stream) {
+    for (const k of INPUTS) {
+      const t = inputs[k];
+      if (!t) throw new Error(`Hush.denoise: missing input '${k}'`);
+      if (t.width !== this.width || t.height !== this.height) {
+        throw new Error(`Hush.denoise: '${k}' is ${t.width}x${t.height}, the denoiser ${this.width}x${this.height} (call resize)`);
+      }
+    }
+    return this.encode(inputs, timer, stream);
+  }
+
+  // denoise() without the input checks (hush.js:835-874).
+  encode(inputs, timer, stream) {
+    const surf = {};
+    for (const k of INPUTS) surf[k] = inputs[k] && (inputs[k].surface || inputs[k]);
+    const out = native.denoise(this._h, surf, stream || 0n, true, !!timer);
+    if (timer && typeof timer.record === 'function') timer.record(native.passTimes(this._h));
+    return new DeviceTexture(this.device, out.width, out.height, out);
+  }
+}
+Hush.requiredLimits = REQUIRED_LIMITS;
+
+module.exports = { Hush, hiddenOf, initDevice, DeviceTexture, decodeWeights, defaultWeights, N_FEAT, N_OUT };
