@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--frames", type=int, default=64)
     ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--math", default=os.environ.get("HUSH_BENCH_MATH", "ieee"), choices=["ieee", "fast"],
+                    help="math policy: ieee (the oracle's operation-exact mirror) or fast")
     ap.add_argument("--cpu-frames", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -104,7 +106,7 @@ def main():
     hip.synchronize()
     print(f"[bench] inputs resident in HBM ({F * 5 * W * H * 8 / 1e9:.2f} GB)", file=sys.stderr, flush=True)
 
-    hs = hush_amd.Hush(0, {"width": W, "height": H, "precision": args.precision})
+    hs = hush_amd.Hush(0, {"width": W, "height": H, "precision": args.precision, "ieee_math": args.math == "ieee"})
     stream = hip.Stream()
 
     def run(n, start=0):
@@ -172,7 +174,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_frame, 5), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{W}x{H} boxes scene, {F}-frame sequence, {args.precision} MLP, built-in weights",
-                   "width": W, "height": H, "frames": F, "mlp": args.precision},
+                   "width": W, "height": H, "frames": F, "mlp": args.precision, "math": args.math},
         "frame_GB/s_algorithmic": round(total_bytes / (ms_frame * 1e-3) / 1e9, 1),
         "frame_hbm_frac": round(total_bytes / (ms_frame * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "passes": passes, "roofline": roofline, "cpu_baseline": cpu,

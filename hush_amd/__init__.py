@@ -26,7 +26,8 @@ PASS_LABELS = ("temporal", "infer+mid", "infer+mid", "infer+mid", "coarse", "coa
 PRECISIONS = {"fp32": 0, "f16": 1, "fp16-mfma": 2, "bf16-mfma": 3, "fp8-mfma": 4}
 SURFACES = {"A0": 0, "X": 1, "Bh0": 2, "Bh1": 3, "mom0": 4, "mom1": 5, "key0": 6, "key1": 7, "hist0": 8, "hist1": 9,
             "taa0": 10, "taa1": 11, "par0_0": 12, "par0_1": 13, "par1": 14, "par2": 15, "F0": 16, "F1": 17,
-            "F2": 18, "F3": 19, "C0": 20, "C1": 21, "CG": 22, "mask": 23}
+            "F2": 18, "F3": 19, "C0": 20, "C1": 21, "CG": 22, "mask": 23,
+            "KD0": 24, "KD1": 25, "GH": 26, "CGD": 27}
 
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "hush-weights.bin")
 
@@ -114,6 +115,7 @@ class Hush:
         exposure = float(opts.get("exposure", 1.0))
         weights = opts.get("weights")
         precision = opts.get("precision", "f16" if opts.get("f16") else "fp32")
+        ieee = bool(opts.get("ieee_math", False))
         limits = getattr(device, "limits", None)
         if limits is not None:
             for k, v in self.requiredLimits.items():
@@ -133,6 +135,8 @@ class Hush:
             raise HushError(self._L.hush_last_error(None).decode(), r)
         self._h = p.value
         self.device = ordinal
+        if ieee:
+            self.ieee_math = True
         self._timer_on = False
 
     # ---- helpers
@@ -190,6 +194,16 @@ class Hush:
     def precision(self, p) -> None:
         self._check(self._L.hush_set_precision(self._h, PRECISIONS[p] if isinstance(p, str) else int(p)))
 
+    @property
+    def ieee_math(self) -> bool:
+        """True: strict IEEE arithmetic everywhere (the oracle's operation-exact mirror); False (default):
+        hardware reciprocal / sqrt / exp2 / log2 for continuous values, IEEE for every threshold decision."""
+        return bool(self._L.hush_get_ieee_math(self._h))
+
+    @ieee_math.setter
+    def ieee_math(self, on: bool) -> None:
+        self._check(self._L.hush_set_ieee_math(self._h, int(bool(on))))
+
     def denoise(self, inputs, timer: HushTimer | None = None, stream: int | None = None) -> Surface:
         """Runs one frame (hush.js:813-822) and returns the output surface (rgba16float, linear HDR)."""
         return self._run(self._L.hush_denoise, inputs, timer, stream)
@@ -245,6 +259,8 @@ class Hush:
             return Surface(s.width, s.height, np.uint32, 4, ptr=s.ptr, pitch=s.pitch, owner=self)
         if s.format == 2:
             return Surface(s.width, s.height, np.uint8, 1, ptr=s.ptr, pitch=s.pitch, owner=self)
+        if s.format == 3:
+            return Surface(s.width, s.height, np.float32, 4, ptr=s.ptr, pitch=s.pitch, owner=self)
         return self._wrap(s)
 
     def set_state(self, frame: int, reset: bool) -> None:

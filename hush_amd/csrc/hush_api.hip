@@ -18,7 +18,8 @@ thread_local char g_err[512];
 
 enum {
   S_A0, S_X, S_BH0, S_BH1, S_MOM0, S_MOM1, S_KEY0, S_KEY1, S_HIST0, S_HIST1, S_TAA0, S_TAA1,
-  S_PAR0_0, S_PAR0_1, S_PAR1, S_PAR2, S_F0, S_F1, S_F2, S_F3, S_C0, S_C1, S_CG, S_MASK, S_COUNT
+  S_PAR0_0, S_PAR0_1, S_PAR1, S_PAR2, S_F0, S_F1, S_F2, S_F3, S_C0, S_C1, S_CG, S_MASK,
+  S_KD0, S_KD1, S_GH, S_CGD, S_COUNT
 };
 
 const char *kPassLabel[8] = {"temporal", "infer+mid", "infer+mid", "infer+mid", "coarse", "coarse", "fine+final", "taa"};
@@ -52,9 +53,11 @@ struct hush_ctx {
   void *arena = nullptr;
   Tex tex[S_COUNT] = {};
   TexU F[4] = {};
+  TexF kd[2] = {}, gh = {}, cgd = {};   // decoded guides (key of each frame, half-res, quarter-res)
   uint8_t *mask = nullptr;
   int mask_pitch = 0;
   bool mask_on = false;
+  bool ieee = false;               // strict IEEE math policy (the oracle's operation-exact mirror)
   bool timing = false;
   bool ev_ready = false;
   hipEvent_t ev[9] = {};
@@ -87,6 +90,9 @@ static void free_surfaces(hush_t *h) {
   h->arena = nullptr;
   std::memset(h->tex, 0, sizeof(h->tex));
   std::memset(h->F, 0, sizeof(h->F));
+  std::memset(h->kd, 0, sizeof(h->kd));
+  h->gh = TexF{};
+  h->cgd = TexF{};
   h->mask = nullptr;
 }
 
@@ -100,6 +106,9 @@ static int alloc_surfaces(hush_t *h, int W, int H) {
   for (int id : {S_F0, S_F1, S_F2, S_F3}) specs.push_back({id, W2, H2, 16});
   for (int id : {S_C0, S_C1, S_CG}) specs.push_back({id, W4, H4, 8});
   specs.push_back({S_MASK, W, H, 1});
+  for (int id : {S_KD0, S_KD1}) specs.push_back({id, W, H, 16});
+  specs.push_back({S_GH, W2, H2, 16});
+  specs.push_back({S_CGD, W4, H4, 16});
   size_t total = 0;
   std::vector<size_t> offs;
   for (auto &s : specs) {
@@ -119,6 +128,9 @@ static int alloc_surfaces(hush_t *h, int W, int H) {
     int pitch = round_up(s.w, 64);
     if (s.id >= S_F0 && s.id <= S_F3) h->F[s.id - S_F0] = TexU{(uint4 *)p, pitch};
     else if (s.id == S_MASK) { h->mask = (uint8_t *)p; h->mask_pitch = pitch; }
+    else if (s.id == S_KD0 || s.id == S_KD1) h->kd[s.id - S_KD0] = TexF{(float4 *)p, pitch};
+    else if (s.id == S_GH) h->gh = TexF{(float4 *)p, pitch};
+    else if (s.id == S_CGD) h->cgd = TexF{(float4 *)p, pitch};
     else h->tex[s.id] = Tex{(uint2 *)p, pitch};
   }
   h->W = W; h->H = H; h->W2 = W2; h->H2 = H2; h->W4 = W4; h->H4 = H4;
@@ -223,6 +235,13 @@ extern "C" int32_t hush_set_precision(hush_t *h, int32_t p) {
 }
 extern "C" int32_t hush_get_precision(const hush_t *h) { return h ? h->precision : -1; }
 
+extern "C" int32_t hush_set_ieee_math(hush_t *h, int32_t on) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_ieee_math: null context");
+  h->ieee = on != 0;
+  return HUSH_OK;
+}
+extern "C" int32_t hush_get_ieee_math(const hush_t *h) { return h ? (h->ieee ? 1 : 0) : -1; }
+
 extern "C" const char *hush_last_error(const hush_t *h) { return h ? h->err : g_err; }
 
 static Tex in_tex(const hush_surface &s) { return Tex{(uint2 *)s.ptr, (int)(s.pitch / 8)}; }
@@ -232,6 +251,11 @@ static hush_surface surf_of(const hush_t *h, int id) {
   if (id >= S_F0 && id <= S_F3) {
     s.ptr = h->F[id - S_F0].p; s.pitch = (size_t)h->F[id - S_F0].pitch * 16;
     s.width = h->W2; s.height = h->H2; s.format = HUSH_RGBA32U;
+  } else if (id >= S_KD0 && id <= S_CGD) {
+    const TexF &t = id == S_GH ? h->gh : id == S_CGD ? h->cgd : h->kd[id - S_KD0];
+    s.ptr = t.p; s.pitch = (size_t)t.pitch * 16; s.format = HUSH_RGBA32F;
+    s.width = id == S_GH ? h->W2 : id == S_CGD ? h->W4 : h->W;
+    s.height = id == S_GH ? h->H2 : id == S_CGD ? h->H4 : h->H;
   } else if (id == S_MASK) {
     s.ptr = h->mask; s.pitch = (size_t)h->mask_pitch; s.width = h->W; s.height = h->H; s.format = HUSH_R8U;
   } else {
@@ -252,52 +276,52 @@ static hipError_t run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream
     case 0: {
       TemporalArgs a;
       a.rad = in_tex(in->radiance); a.nd = in_tex(in->normal_depth); a.mv = in_tex(in->motion); a.mat = in_tex(in->material);
-      a.pkey = t[S_KEY0 + prev]; a.pcol = t[S_HIST0 + prev]; a.pmom = t[S_MOM0 + prev]; a.ppar0 = t[S_PAR0_0 + prev];
-      a.outA = t[S_A0]; a.outMom = t[S_MOM0 + cur]; a.outKey = t[S_KEY0 + cur];
-      a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.F3 = h->F[3];
+      a.pkd = h->kd[prev]; a.pcol = t[S_HIST0 + prev]; a.pmom = t[S_MOM0 + prev]; a.ppar0 = t[S_PAR0_0 + prev];
+      a.outA = t[S_A0]; a.outMom = t[S_MOM0 + cur]; a.outKey = t[S_KEY0 + cur]; a.outKD = h->kd[cur];
+      a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.F3 = h->F[3]; a.outGH = h->gh;
       a.mask = h->mask_on ? h->mask : nullptr; a.mask_pitch = h->mask_pitch;
       a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.reset = h->reset ? 1 : 0; a.E = h->exposure;
-      return launch_temporal(a, s);
+      return launch_temporal(a, !h->ieee, s);
     }
     case 1: {
       InferArgs a;
-      a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.F3 = h->F[3];
+      a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.GH = h->gh;
       a.out0 = t[S_PAR0_0 + cur]; a.out1 = t[S_PAR1]; a.out2 = t[S_PAR2]; a.outMid = t[S_BH0];
       a.W2 = h->W2; a.H2 = h->H2; a.hidden = h->hidden; a.w = h->d_w; a.wq = h->d_wq; a.wscale = h->d_wscale;
-      return launch_infer(a, h->precision, s);
+      return launch_infer(a, h->precision, !h->ieee, s);
     }
     case 2:
     case 3: {
       AtrousArgs a{};
-      a.inA = t[pass == 2 ? S_BH0 : S_BH1]; a.guideF = h->F[3]; a.out = t[pass == 2 ? S_BH1 : S_BH0];
+      a.inA = t[pass == 2 ? S_BH0 : S_BH1]; a.guide = h->gh; a.out = t[pass == 2 ? S_BH1 : S_BH0];
       a.par1 = t[S_PAR1]; a.par2 = t[S_PAR2]; a.w = h->W2; a.h = h->H2; a.s = 0.0f;
-      return pass == 2 ? launch_mid2(a, s) : launch_mid3(a, s);
+      return pass == 2 ? launch_mid2(a, !h->ieee, s) : launch_mid3(a, !h->ieee, s);
     }
     case 4: {
       Coarse1Args a;
-      a.F2 = h->F[2]; a.F3 = h->F[3]; a.outC = t[S_C1]; a.outG = t[S_CG];
+      a.F2 = h->F[2]; a.GH = h->gh; a.outC = t[S_C1]; a.outG = t[S_CG]; a.outGD = h->cgd;
       a.W4 = h->W4; a.H4 = h->H4; a.W2 = h->W2; a.H2 = h->H2; a.s = 0.25f;   // COARSE_S (hush.js:26)
-      return launch_coarse1(a, s);
+      return launch_coarse1(a, !h->ieee, s);
     }
     case 5: {
       AtrousArgs a{};
-      a.inA = t[S_C1]; a.guideT = t[S_CG]; a.out = t[S_C0]; a.w = h->W4; a.h = h->H4; a.s = 0.25f;
-      return launch_coarse2(a, s);
+      a.inA = t[S_C1]; a.guide = h->cgd; a.out = t[S_C0]; a.w = h->W4; a.h = h->H4; a.s = 0.25f;
+      return launch_coarse2(a, !h->ieee, s);
     }
     case 6: {
       FineArgs a;
-      a.A0 = t[S_A0]; a.key = t[S_KEY0 + cur]; a.par0 = t[S_PAR0_0 + cur]; a.par1 = t[S_PAR1]; a.par2 = t[S_PAR2];
-      a.F2 = h->F[2]; a.F3 = h->F[3]; a.mid = t[S_BH0]; a.coarse = t[S_C0]; a.cguide = t[S_CG];
+      a.A0 = t[S_A0]; a.kd = h->kd[cur]; a.par0 = t[S_PAR0_0 + cur]; a.par1 = t[S_PAR1]; a.par2 = t[S_PAR2];
+      a.F2 = h->F[2]; a.GH = h->gh; a.mid = t[S_BH0]; a.coarse = t[S_C0]; a.cgd = h->cgd;
       a.mat = in_tex(in->material); a.emi = in_tex(in->emission);
       a.outHist = t[S_HIST0 + cur]; a.outX = t[S_X];
       a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.W4 = h->W4; a.H4 = h->H4;
-      return launch_fine_final(a, s);
+      return launch_fine_final(a, !h->ieee, s);
     }
     case 7: {
       TaaArgs a;
       a.X = t[S_X]; a.mv = in_tex(in->motion); a.prev = t[S_TAA0 + prev]; a.out = t[S_TAA0 + cur];
       a.W = h->W; a.H = h->H; a.reset = h->reset ? 1 : 0;
-      return launch_taa(a, s);
+      return launch_taa(a, !h->ieee, s);
     }
     default:
       return hipErrorInvalidValue;

@@ -24,6 +24,13 @@ struct TexU {           // rgba32uint surface
   uint4 *p;
   int pitch;
 };
+// Decoded guide surface, 16 B per texel: xyz = octDec of the stored binary16 octahedral normal (fp32,
+// computed once by the producing pass with IEEE arithmetic), w = a binary16 pair (depth and id / block
+// count) or an fp32 depth.  Consumers read it instead of decoding the same normal again.
+struct TexF {
+  float4 *p;
+  int pitch;
+};
 
 __device__ __forceinline__ float2 unpack2(uint32_t u) {
   half2v h = __builtin_bit_cast(half2v, u);
@@ -52,6 +59,11 @@ __device__ __forceinline__ float4 ld(const Tex &t, int x, int y) { return unpack
 __device__ __forceinline__ void st(const Tex &t, int x, int y, float4 v) { t.p[(size_t)y * t.pitch + x] = pack4(v); }
 __device__ __forceinline__ uint4 ldu(const TexU &t, int x, int y) { return t.p[(size_t)y * t.pitch + x]; }
 __device__ __forceinline__ void stu(const TexU &t, int x, int y, uint4 v) { t.p[(size_t)y * t.pitch + x] = v; }
+
+__device__ __forceinline__ float4 ldf(const TexF &t, int x, int y) { return t.p[(size_t)y * t.pitch + x]; }
+__device__ __forceinline__ void stf(const TexF &t, int x, int y, float4 v) { t.p[(size_t)y * t.pitch + x] = v; }
+// low / high binary16 of a packed pair kept in a float4 lane
+__device__ __forceinline__ float2 pair_of(float w) { return unpack2(__float_as_uint(w)); }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 __device__ __forceinline__ bool inside(int x, int y, int w, int h) { return x >= 0 && y >= 0 && x < w && y < h; }
@@ -82,13 +94,74 @@ __device__ __forceinline__ float3 oct_dec(float ex, float ey) {
   return make_float3(vx / l, vy / l, vz / l);
 }
 
-__device__ __forceinline__ float log1p_(float x) { return logf(1.0f + x); }   // hush.js:47
+// Tile origin of this workgroup in a 1-D grid of ceil(w/TW) * ceil(h/TH) workgroups.  The dispatcher hands
+// workgroup n to XCD n % 8, and each of the 8 XCDs has its own L2; remapping n gives every XCD one
+// contiguous band of tiles in raster order, so neighbouring tiles (which share apron texels) meet in the
+// same L2 instead of fetching the apron again from HBM.
+template <int TW, int TH>
+__device__ __forceinline__ int2 tile_origin(int w, int h) {
+  const int gx = (w + TW - 1) / TW, gy = (h + TH - 1) / TH;
+  const int T = gx * gy, n = blockIdx.x;
+  const int q = T >> 3, r = T & 7, x = n & 7, i = n >> 3;
+  const int L = x * q + (x < r ? x : r) + i;
+  return make_int2((L % gx) * TW, (L / gx) * TH);
+}
+
+// Math policies.  IEEE: correctly rounded division and square root, ocml expf / logf -- the operation-exact
+// mirror of the oracle.  FAST: hardware reciprocal, square root, exp2 and log2 (about 1-2 ulp in fp32),
+// used only where the result is a continuous value bound for a binary16 store; every decision the
+// reference takes on a threshold (validity tests, sky tests, the history key) is computed with IEEE in
+// both policies, so masks and ids stay bit-exact.
+template <bool FAST> struct Math;
+template <> struct Math<false> {
+  static __device__ __forceinline__ float div(float a, float b) { return a / b; }
+  static __device__ __forceinline__ float rcp(float b) { return 1.0f / b; }
+  static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
+  static __device__ __forceinline__ float exp(float x) { return expf(x); }
+  static __device__ __forceinline__ float log(float x) { return logf(x); }
+};
+template <> struct Math<true> {
+  static __device__ __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+  // e^x = 2^t * 2^lo with t = fl(x log2 e) and lo its rounding error (x log2 e carried to ~48 bits)
+  static __device__ __forceinline__ float exp(float x) {
+    const float L2E = 1.44269502e+00f, L2E_LO = 1.92596299e-08f;
+    const float t = x * L2E;
+    const float lo = __builtin_fmaf(x, L2E, -t) + x * L2E_LO;
+    const float e = __builtin_amdgcn_exp2f(t);
+    return e == __builtin_inff() ? e : __builtin_fmaf(e, lo * 0.693147182f, e);   // no inf * lo on overflow
+  }
+  static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.693147182f; }
+};
+
+template <bool FAST> __device__ __forceinline__ float log1p_(float x) { return Math<FAST>::log(1.0f + x); }  // hush.js:47
 
 __device__ __forceinline__ float pow128(float x) {                            // hush.js:70-73
   float x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8, x32 = x16 * x16, x64 = x32 * x32;
   return x64 * x64;
 }
 
-__device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + expf(-x)); }
+template <bool FAST> __device__ __forceinline__ float sigmoid(float x) { return Math<FAST>::rcp(1.0f + Math<FAST>::exp(-x)); }
+
+// octDec with the policy's arithmetic (continuous uses only; validity tests use oct_dec)
+template <bool FAST> __device__ __forceinline__ float3 oct_dec_m(float ex, float ey) {
+  if (!FAST) return oct_dec(ex, ey);
+  float vx = ex, vy = ey, vz = 1.0f - fabsf(ex) - fabsf(ey);
+  float t = mx(-vz, 0.0f);
+  vx += vx >= 0.0f ? -t : t;
+  vy += vy >= 0.0f ? -t : t;
+  float r = __builtin_amdgcn_rcpf(mx(__builtin_amdgcn_sqrtf(vx * vx + vy * vy + vz * vz), 1e-20f));
+  return make_float3(vx * r, vy * r, vz * r);
+}
+// octEnc with the policy's arithmetic (continuous uses only; the history key uses oct_enc)
+template <bool FAST> __device__ __forceinline__ float2 oct_enc_m(float3 n) {
+  if (!FAST) return oct_enc(n);
+  float r = __builtin_amdgcn_rcpf(mx(fabsf(n.x) + fabsf(n.y) + fabsf(n.z), 1e-20f));
+  float vx = n.x * r, vy = n.y * r, vz = n.z * r;
+  float sx = vx >= 0.0f ? 1.0f : -1.0f, sy = vy >= 0.0f ? 1.0f : -1.0f;
+  if (vz < 0.0f) return make_float2((1.0f - fabsf(vy)) * sx, (1.0f - fabsf(vx)) * sy);
+  return make_float2(vx, vy);
+}
 
 }  // namespace hush

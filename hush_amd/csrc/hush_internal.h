@@ -11,9 +11,12 @@ enum MlpMode { MLP_FP32 = 0, MLP_F16 = 1, MLP_FP16_MFMA = 2, MLP_BF16_MFMA = 3, 
 
 struct TemporalArgs {
   Tex rad, nd, mv, mat;              // this frame's G-buffer
-  Tex pkey, pcol, pmom, ppar0;       // history: key, histColor, mom, par0 of the previous frame
+  TexF pkd;                          // history: decoded key of the previous frame (normal, (depth, id))
+  Tex pcol, pmom, ppar0;             // history: histColor, mom, par0 of the previous frame
   Tex outA, outMom, outKey;
+  TexF outKD;                        // decoded key of this frame
   TexU F0, F1, F2, F3;
+  TexF outGH;                        // decoded half-res guide (normal, packed (Z, count))
   uint8_t *mask;                     // optional per-pixel reprojection bits (debug export)
   int mask_pitch;
   int W, H, W2, H2, reset;
@@ -21,7 +24,8 @@ struct TemporalArgs {
 };
 
 struct InferArgs {
-  TexU F0, F1, F2, F3;
+  TexU F0, F1, F2;
+  TexF GH;
   Tex out0, out1, out2, outMid;      // par0[cur], par1, par2, Bh[0]
   int W2, H2, hidden;
   const float *w;                    // MLP weights widened to fp32 (hush.js:744)
@@ -30,24 +34,29 @@ struct InferArgs {
 };
 
 struct AtrousArgs {
-  Tex inA, guideT;
-  TexU guideF;
+  Tex inA;
+  TexF guide;                        // GH (w = packed (Z, count)) or CGD (w = depth)
   Tex out, par1, par2;
   int w, h;
   float s;
 };
 
 struct Coarse1Args {
-  TexU F2, F3;
+  TexU F2;
+  TexF GH;
   Tex outC, outG;
+  TexF outGD;                        // decoded coarse guide
   int W4, H4, W2, H2;
   float s;
 };
 
 struct FineArgs {
-  Tex A0, key, par0, par1, par2;
-  TexU F2, F3;
-  Tex mid, coarse, cguide, mat, emi;
+  Tex A0;
+  TexF kd;                           // decoded key of this frame (the fine level's guide)
+  Tex par0, par1, par2;
+  TexU F2;
+  TexF GH, cgd;
+  Tex mid, coarse, mat, emi;
   Tex outHist, outX;
   int W, H, W2, H2, W4, H4;
 };
@@ -57,13 +66,13 @@ struct TaaArgs {
   int W, H, reset;
 };
 
-hipError_t launch_temporal(const TemporalArgs &a, hipStream_t s);
-hipError_t launch_infer(const InferArgs &a, int mode, hipStream_t s);
-hipError_t launch_mid2(const AtrousArgs &a, hipStream_t s);
-hipError_t launch_mid3(const AtrousArgs &a, hipStream_t s);
-hipError_t launch_coarse1(const Coarse1Args &a, hipStream_t s);
-hipError_t launch_coarse2(const AtrousArgs &a, hipStream_t s);
-hipError_t launch_fine_final(const FineArgs &a, hipStream_t s);
-hipError_t launch_taa(const TaaArgs &a, hipStream_t s);
+hipError_t launch_temporal(const TemporalArgs &a, bool fast, hipStream_t s);
+hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s);
+hipError_t launch_mid2(const AtrousArgs &a, bool fast, hipStream_t s);
+hipError_t launch_mid3(const AtrousArgs &a, bool fast, hipStream_t s);
+hipError_t launch_coarse1(const Coarse1Args &a, bool fast, hipStream_t s);
+hipError_t launch_coarse2(const AtrousArgs &a, bool fast, hipStream_t s);
+hipError_t launch_fine_final(const FineArgs &a, bool fast, hipStream_t s);
+hipError_t launch_taa(const TaaArgs &a, bool fast, hipStream_t s);
 
 }  // namespace hush

@@ -18,62 +18,86 @@ constexpr int TW = 32, TH = 8, NT = 256;
 // TEMPORAL (hush.js:79-308): demodulate, 3x3 stats, reproject (bilinear + 3x3 fallback), clamp and blend,
 // store A0 / mom / key; then per 2x2 block the 20 MLP features and the half-res guide (F0..F3).
 
-__device__ __forceinline__ bool check(float4 k, float3 n, float zprev, float obj) {  // hush.js:103-105
-  if (!(k.z >= 0.0f)) return false;
-  if (!(k.w == obj)) return false;
+// check (hush.js:103-105) on a decoded key texel: xyz = octDec(k.xy) (exact), w = binary16 (k.z, k.w)
+__device__ __forceinline__ bool check(float4 kd, float3 n, float zprev, float obj) {
+  const float2 zi = pair_of(kd.w);
+  if (!(zi.x >= 0.0f)) return false;
+  if (!(zi.y == obj)) return false;
   float tol = 0.05f * zprev;
   tol = tol + 0.01f;
-  if (!(fabsf(k.z - zprev) <= tol)) return false;
-  return dot3(oct_dec(k.x, k.y), n) >= 0.9f;
+  if (!(fabsf(zi.x - zprev) <= tol)) return false;
+  return dot3(make_float3(kd.x, kd.y, kd.z), n) >= 0.9f;
 }
 
 // per-pixel feature inputs kept in LDS for the block phase (structure-of-arrays, conflict-free)
 enum { P0X, P0Y, P0Z, P0W, P1X, P1Y, P1Z, P1W, P2Z, P2W, NSX, NSY, NSZ, P3X, P3Y, P3Z, P3W, P4X, P4Y, P4Z, P4W, NP };
 
+template <bool FAST>
 __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
+  using F = Math<FAST>;
   constexpr int SW = TW + 2, SH = TH + 2, SN = SW * SH;
   __shared__ float4 sI[SN];        // demodulated illumination, valid flag
   __shared__ float sL[SN];         // lum(illum)
   __shared__ float sQ[SN];         // E * qf16(lum(illum))
   __shared__ float sP[NP][NT];     // feature inputs P0..P4 (+ decoded normal) per pixel
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const int2 org = tile_origin<TW, TH>(a.W, a.H);
+  const int x0 = org.x, y0 = org.y;
   const float E = a.E;
-  for (int i = tid; i < SN; i += NT) {
-    int qx = x0 - 1 + i % SW, qy = y0 - 1 + i / SW;
+  const int lx = tid & 31, ly = tid >> 5;
+  const int px = x0 + lx, py = y0 + ly;
+  const bool in_img = inside(px, py, a.W, a.H);
+
+  // this pixel's G-buffer (also its own staged texel)
+  float4 nd = make_float4(0.0f, 0.0f, 0.0f, -1.0f), mv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mr = mv, rd = mv;
+  if (in_img) { nd = ld(a.nd, px, py); mv = ld(a.mv, px, py); mr = ld(a.mat, px, py); rd = ld(a.rad, px, py); }
+  const bool use_hist = in_img && !a.reset && nd.w >= 0.0f;
+
+  // tile staging (hush.js:287-294): each thread its own pixel, then 84 threads the 1-texel apron
+  {
+    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (in_img && nd.w >= 0.0f)
+      e = make_float4(F::div(rd.x, mx(mr.x, 1e-3f)), F::div(rd.y, mx(mr.y, 1e-3f)), F::div(rd.z, mx(mr.z, 1e-3f)), 1.0f);
+    const int c = (ly + 1) * SW + lx + 1;
+    const float l = lum(e.x, e.y, e.z);
+    sI[c] = e; sL[c] = l; sQ[c] = E * qf16(l);
+  }
+  if (tid < 2 * SW + 2 * TH) {
+    int tx, ty;
+    if (tid < SW) { tx = tid; ty = 0; }
+    else if (tid < 2 * SW) { tx = tid - SW; ty = SH - 1; }
+    else if (tid < 2 * SW + TH) { tx = 0; ty = tid - 2 * SW + 1; }
+    else { tx = SW - 1; ty = tid - 2 * SW - TH + 1; }
+    const int qx = x0 - 1 + tx, qy = y0 - 1 + ty;
     float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (inside(qx, qy, a.W, a.H)) {
-      float4 nd = ld(a.nd, qx, qy);
-      if (nd.w >= 0.0f) {
+      float4 qnd = ld(a.nd, qx, qy);
+      if (qnd.w >= 0.0f) {
         float4 r = ld(a.rad, qx, qy), m = ld(a.mat, qx, qy);
-        e = make_float4(r.x / mx(m.x, 1e-3f), r.y / mx(m.y, 1e-3f), r.z / mx(m.z, 1e-3f), 1.0f);
+        e = make_float4(F::div(r.x, mx(m.x, 1e-3f)), F::div(r.y, mx(m.y, 1e-3f)), F::div(r.z, mx(m.z, 1e-3f)), 1.0f);
       }
     }
-    sI[i] = e;
-    float l = lum(e.x, e.y, e.z);
-    sL[i] = l;
-    sQ[i] = E * qf16(l);
+    const int c = ty * SW + tx;
+    const float l = lum(e.x, e.y, e.z);
+    sI[c] = e; sL[c] = l; sQ[c] = E * qf16(l);
   }
   __syncthreads();
 
-  const int lx = tid & 31, ly = tid >> 5;
-  const int px = x0 + lx, py = y0 + ly;
-  float P[NP];
-#pragma unroll
-  for (int k = 0; k < NP; k++) P[k] = 0.0f;
-  P[P2Z] = -1.0f;
-  if (inside(px, py, a.W, a.H)) {
-    float4 nd = ld(a.nd, px, py);
-    float4 mv = ld(a.mv, px, py);
+  // feature inputs go straight to LDS (sky / outside: zeros with depth -1, hush.js:112, 297)
+  float *P = &sP[0][tid];
+  bool p_written = false;
+  if (in_img) {
     uint32_t mbits = 0;
     if (nd.w < 0.0f) {                                   // sky (hush.js:115-119)
       st(a.outA, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
       st(a.outMom, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-      st(a.outKey, px, py, make_float4(0.0f, 0.0f, -1.0f, mv.z));
+      const uint32_t zid = pack_rtz(-1.0f, mv.z);
+      a.outKey.p[(size_t)py * a.outKey.pitch + px] = make_uint2(0u, zid);
+      const float3 d0 = oct_dec(0.0f, 0.0f);
+      stf(a.outKD, px, py, make_float4(d0.x, d0.y, d0.z, __uint_as_float(zid)));
       mbits = 0x80;
     } else {
       const float3 n = make_float3(nd.x, nd.y, nd.z);
-      const float4 mr = ld(a.mat, px, py);
       const int c0 = (ly + 1) * SW + lx + 1;
       const float4 ce = sI[c0];
       const float l = sL[c0];
@@ -86,7 +110,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         cnt += t.w;
       }
       cnt = mx(cnt, 1.0f);
-      const float mux = s1x / cnt, muy = s1y / cnt, muz = s1z / cnt;
+      const float mux = F::div(s1x, cnt), muy = F::div(s1y, cnt), muz = F::div(s1z, cnt);
       const float slm = lum(mux, muy, muz);
       float svx = 0.0f, svy = 0.0f, svz = 0.0f, svar = 0.0f;
 #pragma unroll
@@ -98,29 +122,30 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         svx += t.w * dx * dx; svy += t.w * dy * dy; svz += t.w * dz * dz;
         svar += t.w * dl * dl;
       }
-      const float sdx = sqrtf(mx(svx / cnt, 1e-12f)), sdy = sqrtf(mx(svy / cnt, 1e-12f)), sdz = sqrtf(mx(svz / cnt, 1e-12f));
-      svar /= cnt;
+      const float sdx = F::sqrt(mx(F::div(svx, cnt), 1e-12f)), sdy = F::sqrt(mx(F::div(svy, cnt), 1e-12f)),
+                  sdz = F::sqrt(mx(F::div(svz, cnt), 1e-12f));
+      svar = F::div(svar, cnt);
       // reprojection (hush.js:144-187)
       bool has = false;
       float hcx = 0.0f, hcy = 0.0f, hcz = 0.0f;
       float4 hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       float v = 0.0f, alpha = 1.0f, gamma = 1.0f;
-      if (!a.reset) {
+      if (use_hist) {
         const int W = a.W, H = a.H;
-        float ppx = (float)px + mv.x, ppy = (float)py + mv.y;
+        const float ppx = (float)px + mv.x, ppy = (float)py + mv.y;
         float bx = floorf(ppx), by = floorf(ppy);
         float fx = ppx - bx, fy = ppy - by;
         int bix = (int)bx, biy = (int)by;
         float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
         int cx = (int)floorf(ppx + 0.5f), cy = (int)floorf(ppy + 0.5f);
-        // all 12 bilinear loads and the par0 load issued before the tests
+        // the 12 bilinear loads (decoded key, colour, moments) and par0, issued before any test
         float4 k4[4], c4[4], m4[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          int qx = clampi(bix + (i & 1), 0, W - 1), qy = clampi(biy + (i >> 1), 0, H - 1);
-          k4[i] = ld(a.pkey, qx, qy); c4[i] = ld(a.pcol, qx, qy); m4[i] = ld(a.pmom, qx, qy);
+          const int qx = clampi(bix + (i & 1), 0, W - 1), qy = clampi(biy + (i >> 1), 0, H - 1);
+          k4[i] = ldf(a.pkd, qx, qy); c4[i] = ld(a.pcol, qx, qy); m4[i] = ld(a.pmom, qx, qy);
         }
-        float4 pq = ld(a.ppar0, clampi(cx, 0, W - 1) >> 1, clampi(cy, 0, H - 1) >> 1);
+        const float4 pq = ld(a.ppar0, clampi(cx, 0, W - 1) >> 1, clampi(cy, 0, H - 1) >> 1);
         float sumW = 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -132,8 +157,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
           }
         }
         if (sumW >= 0.01f) {
-          hcx /= sumW; hcy /= sumW; hcz /= sumW;
-          hm.x /= sumW; hm.y /= sumW; hm.z /= sumW; hm.w /= sumW;
+          hcx = F::div(hcx, sumW); hcy = F::div(hcy, sumW); hcz = F::div(hcz, sumW);
+          hm.x = F::div(hm.x, sumW); hm.y = F::div(hm.y, sumW); hm.z = F::div(hm.z, sumW); hm.w = F::div(hm.w, sumW);
           v = sumW; has = true;
         } else {
           float n3 = 0.0f;
@@ -141,7 +166,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
           hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
           for (int j = 0; j < 9; j++) {
             int qx = cx + j % 3 - 1, qy = cy + j / 3 - 1;
-            if (inside(qx, qy, W, H) && check(ld(a.pkey, qx, qy), n, mv.w, mv.z)) {
+            if (inside(qx, qy, W, H) && check(ldf(a.pkd, qx, qy), n, mv.w, mv.z)) {
               n3 += 1.0f;
               float4 cc = ld(a.pcol, qx, qy), mm = ld(a.pmom, qx, qy);
               hcx += cc.x; hcy += cc.y; hcz += cc.z;
@@ -149,8 +174,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
             }
           }
           if (n3 > 0.0f) {
-            hcx /= n3; hcy /= n3; hcz /= n3;
-            hm.x /= n3; hm.y /= n3; hm.z /= n3; hm.w /= n3;
+            hcx = F::div(hcx, n3); hcy = F::div(hcy, n3); hcz = F::div(hcz, n3);
+            hm.x = F::div(hm.x, n3); hm.y = F::div(hm.y, n3); hm.z = F::div(hm.z, n3); hm.w = F::div(hm.w, n3);
             v = 0.1f; has = true;
             mbits |= 0x10;
           }
@@ -166,13 +191,13 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         float gx = gamma * sdx, gy = gamma * sdy, gz = gamma * sdz;
         float hlx = mn(mx(hcx, mux - gx), mux + gx), hly = mn(mx(hcy, muy - gy), muy + gy), hlz = mn(mx(hcz, muz - gz), muz + gz);
         len1 = mn(hm.w + 1.0f, 32.0f);
-        float aa = mx(alpha, 1.0f / len1);
+        float aa = mx(alpha, F::rcp(len1));
         cr = hlx + (ce.x - hlx) * aa; cg = hly + (ce.y - hly) * aa; cb = hlz + (ce.z - hlz) * aa;
         nmx = hm.x + (l - hm.x) * aa;
         nmy = hm.y + (l * l - hm.y) * aa;
-        nfast = hm.z + (l - hm.z) * mx(1.0f / 6.0f, 1.0f / len1);
+        nfast = hm.z + (l - hm.z) * mx(1.0f / 6.0f, F::rcp(len1));
         float lh = lum(hcx, hcy, hcz);
-        lag = fabsf(slm - lh) / (sqrtf(mx(svar / cnt, 1e-12f)) + 0.02f * lh + 1e-3f);
+        lag = F::div(fabsf(slm - lh), F::sqrt(mx(F::div(svar, cnt), 1e-12f)) + 0.02f * lh + 1e-3f);
       } else {
         v = 0.0f;
       }
@@ -181,24 +206,32 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       float2 oc = oct_enc(n);
       st(a.outA, px, py, A);
       st(a.outMom, px, py, make_float4(nmx, nmy, nfast, len1));
-      st(a.outKey, px, py, make_float4(oc.x, oc.y, nd.w, mv.z));
+      // the key as stored, and its exact decode for the next frame's check and this frame's fine level
+      const uint32_t okp = pack_rtz(oc.x, oc.y), zid = pack_rtz(nd.w, mv.z);
+      a.outKey.p[(size_t)py * a.outKey.pitch + px] = make_uint2(okp, zid);
+      const float2 ok = unpack2(okp);
+      const float3 dec = oct_dec(ok.x, ok.y);
+      stf(a.outKD, px, py, make_float4(dec.x, dec.y, dec.z, __uint_as_float(zid)));
       // feature inputs from the stored values (hush.js:212-222)
       float aqx = qf16(A.x), aqy = qf16(A.y), aqz = qf16(A.z), aqw = qf16(A.w);
       float m1 = E * qf16(nmx);
       float m2 = E * E * qf16(nmy);
-      P[P0X] = E * lum(aqx, aqy, aqz); P[P0Y] = E * qf16(l); P[P0Z] = mx(m2 - m1 * m1, 0.0f); P[P0W] = len1;
-      P[P1X] = qf16(v); P[P1Y] = qf16(lag); P[P1Z] = E * qf16(nfast); P[P1W] = m1;
+      P[P0X * NT] = E * lum(aqx, aqy, aqz); P[P0Y * NT] = E * qf16(l); P[P0Z * NT] = mx(m2 - m1 * m1, 0.0f); P[P0W * NT] = len1;
+      P[P1X * NT] = qf16(v); P[P1Y * NT] = qf16(lag); P[P1Z * NT] = E * qf16(nfast); P[P1W * NT] = m1;
       float ocx = qf16(oc.x), ocy = qf16(oc.y);
-      P[P2Z] = qf16(nd.w); P[P2W] = qf16(sqrtf(mv.x * mv.x + mv.y * mv.y));
-      float3 ns = oct_dec(ocx, ocy);
-      P[NSX] = ns.x; P[NSY] = ns.y; P[NSZ] = ns.z;
-      P[P3X] = mr.x; P[P3Y] = mr.y; P[P3Z] = mr.z; P[P3W] = mr.w;
-      P[P4X] = aqx; P[P4Y] = aqy; P[P4Z] = aqz; P[P4W] = aqw;
+      P[P2Z * NT] = qf16(nd.w); P[P2W * NT] = qf16(F::sqrt(mv.x * mv.x + mv.y * mv.y));
+      float3 ns = (ocx == ok.x && ocy == ok.y) ? dec : oct_dec_m<FAST>(ocx, ocy);
+      P[NSX * NT] = ns.x; P[NSY * NT] = ns.y; P[NSZ * NT] = ns.z;
+      P[P3X * NT] = mr.x; P[P3Y * NT] = mr.y; P[P3Z * NT] = mr.z; P[P3W * NT] = mr.w;
+      P[P4X * NT] = aqx; P[P4Y * NT] = aqy; P[P4Z * NT] = aqz; P[P4W * NT] = aqw;
+      p_written = true;
     }
     if (a.mask) a.mask[(size_t)py * a.mask_pitch + px] = (uint8_t)mbits;
   }
+  if (!p_written) {
 #pragma unroll
-  for (int k = 0; k < NP; k++) sP[k][tid] = P[k];
+    for (int k = 0; k < NP; k++) P[k * NT] = k == P2Z ? -1.0f : 0.0f;
+  }
   __syncthreads();
 
   // block phase (hush.js:226-282, 301-306): wave 0 computes the 16x4 half-res texels of this tile
@@ -231,10 +264,13 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     stu(a.F0, X, Y, make_uint4(0, 0, 0, 0));
     stu(a.F1, X, Y, make_uint4(0, 0, 0, 0));
     stu(a.F2, X, Y, make_uint4(0, 0, 0, 0));
-    stu(a.F3, X, Y, make_uint4(pk(0.0f, 0.0f), pk(-1.0f, 0.0f), 0u, 0u));
+    const uint4 g3 = make_uint4(pk(0.0f, 0.0f), pk(-1.0f, 0.0f), 0u, 0u);
+    stu(a.F3, X, Y, g3);
+    const float3 d0 = oct_dec(0.0f, 0.0f);
+    stf(a.outGH, X, Y, make_float4(d0.x, d0.y, d0.z, __uint_as_float(g3.y)));
     return;
   }
-  const float k = 1.0f / nv;
+  const float k = F::rcp(nv);
 #pragma unroll
   for (int c = 0; c < 4; c++) { S0[c] *= k; S1[c] *= k; S3[c] *= k; AC[c] *= k; }
   Nx *= k; Ny *= k; Nz *= k; MV *= k; Z *= k;
@@ -253,7 +289,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     s1 += w * E * qf16(sL[idx]);
     n4 += w;
   }
-  const float mu4 = s1 / mx(n4, 1.0f);
+  const float mu4 = F::div(s1, mx(n4, 1.0f));
   float var4 = 0.0f;
 #pragma unroll
   for (int j = 0; j < 16; j++) {
@@ -261,9 +297,9 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     float d = sQ[idx] - mu4;
     var4 += sI[idx].w * d * d;
   }
-  var4 /= mx(n4, 1.0f);
-  const float ln = mx(sqrtf(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
-  const float3 Nn = make_float3(Nx / ln, Ny / ln, Nz / ln);
+  var4 = F::div(var4, mx(n4, 1.0f));
+  const float ln = mx(F::sqrt(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
+  const float3 Nn = make_float3(F::div(Nx, ln), F::div(Ny, ln), F::div(Nz, ln));
   float zmax = -1e30f, zmin = 1e30f, mind = 1e30f;
 #pragma unroll
   for (int j = 0; j < 4; j++)
@@ -271,16 +307,20 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       zmax = mx(zmax, zs[j]); zmin = mn(zmin, zs[j]);
       mind = mn(mind, dot3(make_float3(nsx[j], nsy[j], nsz[j]), Nn));
     }
-  const float dz = mn(mx((zmax - zmin) / mx(zmin, 1e-6f), 0.0f), 1.0f);
+  const float dz = mn(mx(F::div(zmax - zmin, mx(zmin, 1e-6f)), 0.0f), 1.0f);
   const float dn = mn(mx(1.0f - mind, 0.0f), 2.0f);
-  const float2 oc = oct_enc(Nn);
-  float f[20] = {log1p_(Lacc), log1p_(S0[1]), log1p_(S0[2]), log1p_(var4), S0[3] / 32.0f, S1[0], log1p_(S1[1]),
-                 log1p_(fabsf(S1[2] - S1[3]) / (S1[3] + 1e-3f)), oc.x, oc.y, dz, dn, S3[0], S3[1], S3[2], S3[3],
-                 1.0f - nv / 4.0f, log1p_(MV), log1p_(mu4), log1p_(VA)};
+  const float2 oc = oct_enc_m<FAST>(Nn);
+  float f[20] = {log1p_<FAST>(Lacc), log1p_<FAST>(S0[1]), log1p_<FAST>(S0[2]), log1p_<FAST>(var4), S0[3] / 32.0f, S1[0],
+                 log1p_<FAST>(S1[1]), log1p_<FAST>(F::div(fabsf(S1[2] - S1[3]), S1[3] + 1e-3f)), oc.x, oc.y, dz, dn, S3[0],
+                 S3[1], S3[2], S3[3], 1.0f - nv / 4.0f, log1p_<FAST>(MV), log1p_<FAST>(mu4), log1p_<FAST>(VA)};
   stu(a.F0, X, Y, make_uint4(pk(f[0], f[1]), pk(f[2], f[3]), pk(f[4], f[5]), pk(f[6], f[7])));
   stu(a.F1, X, Y, make_uint4(pk(f[8], f[9]), pk(f[10], f[11]), pk(f[12], f[13]), pk(f[14], f[15])));
   stu(a.F2, X, Y, make_uint4(pk(f[16], f[17]), pk(f[18], f[19]), pk(AC[0], AC[1]), pk(AC[2], AC[3])));
-  stu(a.F3, X, Y, make_uint4(pk(oc.x, oc.y), pk(Z, nv), 0u, 0u));
+  const uint4 g3 = make_uint4(pk(oc.x, oc.y), pk(Z, nv), 0u, 0u);
+  stu(a.F3, X, Y, g3);
+  const float2 go = unpack2(g3.x);
+  const float3 gd = oct_dec(go.x, go.y);
+  stf(a.outGH, X, Y, make_float4(gd.x, gd.y, gd.z, __uint_as_float(g3.y)));
 }
 
 // ============================================================================ edge-aware a-trous
@@ -297,9 +337,10 @@ __host__ __device__ constexpr double tap_dist(int d2) {
        : d2 == 8 ? 2.8284271247461903 : 0.0;
 }
 
-template <int R, int STEP, bool PRE, int SW>
+template <int R, int STEP, bool PRE, int SW, bool FAST>
 __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int tx, int ty, int qx, int qy, int w, int h,
                                        float s, float cc) {
+  using F = Math<FAST>;
   const int c = ty * SW + tx;
   const float4 gp = sG[c], center = sA[c];
   float gv;
@@ -311,7 +352,7 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
     gv = center.w;
   }
   const float lc = lum(center.x, center.y, center.z);
-  const float sl = s / (sqrtf(mx(gv, 1e-12f)) + 1e-6f);
+  const float sl = F::div(s, F::sqrt(mx(gv, 1e-12f)) + 1e-6f);
   const float zl = sG[c - 1].w, zr = sG[c + 1].w, zu = sG[c - SW].w, zd = sG[c + SW].w;
   const float gx = mn(zr > 0.0f ? fabsf(zr - gp.w) : 1e9f, zl > 0.0f ? fabsf(zl - gp.w) : 1e9f);
   const float gy = mn(zd > 0.0f ? fabsf(zd - gp.w) : 1e9f, zu > 0.0f ? fabsf(zu - gp.w) : 1e9f);
@@ -319,7 +360,7 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
   float iz[9];
 #pragma unroll
   for (int d2 = 1; d2 <= 2 * R * R; d2++)
-    if (tap_dist(d2) > 0.0) iz[d2] = 1.0f / (zg * (float)(tap_dist(d2) * STEP) + 1e-3f);
+    if (tap_dist(d2) > 0.0) iz[d2] = F::rcp(zg * (float)(tap_dist(d2) * STEP) + 1e-3f);
   const float cw = (float)(Kern<R>::h[R] * Kern<R>::h[R]) * cc;
   float sw = cw;
   float scx = center.x * cw, scy = center.y * cw, scz = center.z * cw;
@@ -335,24 +376,22 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
         const float4 cq = sA[c + o];
         const float ww = (float)(Kern<R>::h[dx + R] * Kern<R>::h[dy + R])
                        * pow128(mx(dot3(make_float3(gq.x, gq.y, gq.z), make_float3(gp.x, gp.y, gp.z)), 0.0f))
-                       * expf(-fabsf(gq.w - gp.w) * iz[dx * dx + dy * dy] - sl * fabsf(lum(cq.x, cq.y, cq.z) - lc));
+                       * F::exp(-fabsf(gq.w - gp.w) * iz[dx * dx + dy * dy] - sl * fabsf(lum(cq.x, cq.y, cq.z) - lc));
         sw += ww;
         scx += ww * cq.x; scy += ww * cq.y; scz += ww * cq.z;
         sv += ww * ww * cq.w;
       }
     }
-  return make_float4(scx / sw, scy / sw, scz / sw, sv / (sw * sw));
+  return make_float4(F::div(scx, sw), F::div(scy, sw), F::div(scz, sw), F::div(sv, sw * sw));
 }
 
 // decoded half-res guide from an F3 texel: octDec(oct normal), block depth (hush.js:337)
-__device__ __forceinline__ float4 guide_f3(uint4 g3) {
-  float2 oc = unpack2(g3.x);
-  float3 n = oct_dec(oc.x, oc.y);
-  return make_float4(n.x, n.y, n.z, unpack2(g3.y).x);
-}
+// LDS guide (normal, depth) from a decoded half-res guide texel (w = binary16 (Z, count))
+__device__ __forceinline__ float4 guide_gh(float4 g) { return make_float4(g.x, g.y, g.z, pair_of(g.w).x); }
 // decoded guide from an rgba16float (oct.x, oct.y, depth, .) texel (hush.js:497, 566-567)
+template <bool FAST>
 __device__ __forceinline__ float4 guide_tex(float4 g) {
-  float3 n = oct_dec(g.x, g.y);
+  float3 n = oct_dec_m<FAST>(g.x, g.y);
   return make_float4(n.x, n.y, n.z, g.z);
 }
 
@@ -412,23 +451,25 @@ __device__ __forceinline__ void mlp_f16(const float *__restrict__ w, int h, cons
 }
 
 // activations (hush.js:357-366) and the four half-res outputs of one texel
+template <bool FAST>
 __device__ __forceinline__ void infer_outputs(const InferArgs &a, const float4 *sA, const float4 *sG, int tx, int ty,
                                               int X, int Y, const float *o) {
+  using F = Math<FAST>;
   constexpr int SW = TW + 2;
-  const float alpha = 1.0f / 32.0f + (31.0f / 32.0f) * sigmoid(o[0]);
-  const float gamma = 0.5f + 7.5f * sigmoid(o[1]);
-  const float beta = sigmoid(o[2]);
-  const float kappa = sigmoid(o[3]);
+  const float alpha = 1.0f / 32.0f + (31.0f / 32.0f) * sigmoid<FAST>(o[0]);
+  const float gamma = 0.5f + 7.5f * sigmoid<FAST>(o[1]);
+  const float beta = sigmoid<FAST>(o[2]);
+  const float kappa = sigmoid<FAST>(o[3]);
   float s[4], c[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    s[k] = 0.25f * expf(mn(mx(o[4 + k], -5.0f), 5.0f));
-    c[k] = expf(mn(mx(o[8 + k], -4.0f), 4.0f));
+    s[k] = 0.25f * F::exp(mn(mx(o[4 + k], -5.0f), 5.0f));
+    c[k] = F::exp(mn(mx(o[8 + k], -4.0f), 4.0f));
   }
   st(a.out0, X, Y, make_float4(alpha, gamma, beta, kappa));
   st(a.out1, X, Y, make_float4(s[0], s[1], s[2], s[3]));
   st(a.out2, X, Y, make_float4(c[0], c[1], c[2], c[3]));
-  st(a.outMid, X, Y, filt<1, 1, true, SW>(sA, sG, tx, ty, X, Y, a.W2, a.H2, qf16(s[1]), qf16(c[1])));
+  st(a.outMid, X, Y, filt<1, 1, true, SW, FAST>(sA, sG, tx, ty, X, Y, a.W2, a.H2, qf16(s[1]), qf16(c[1])));
 }
 
 __device__ __forceinline__ void unpack_features(uint4 f0, uint4 f1, uint4 f2, float *f) {
@@ -440,18 +481,19 @@ __device__ __forceinline__ void unpack_features(uint4 f0, uint4 f1, uint4 f2, fl
   }
 }
 
-template <int HID, int MODE>   // MODE 0: fp32, 1: shader-f16 arithmetic
+template <int HID, int MODE, bool FAST>   // MODE 0: fp32, 1: shader-f16 arithmetic
 __global__ __launch_bounds__(NT) void k_infer(InferArgs a) {
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
   __shared__ float4 sA[SN], sG[SN];
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const int2 org = tile_origin<TW, TH>(a.W2, a.H2);
+  const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.W2 - 1), qy = clampi(y0 - AP + i / SW, 0, a.H2 - 1);
-    uint4 g2 = ldu(a.F2, qx, qy), g3 = ldu(a.F3, qx, qy);
+    uint4 g2 = ldu(a.F2, qx, qy);
     float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
     sA[i] = make_float4(a01.x, a01.y, a23.x, a23.y);
-    sG[i] = guide_f3(g3);
+    sG[i] = guide_gh(ldf(a.GH, qx, qy));
   }
   __syncthreads();
   const int lx = tid & 31, ly = tid >> 5;
@@ -468,24 +510,27 @@ __global__ __launch_bounds__(NT) void k_infer(InferArgs a) {
   float o[12];
   if (MODE == 0) mlp_fp32<HID>(a.w, a.hidden, f, o);
   else mlp_f16(a.w, a.hidden, f, o);
-  infer_outputs(a, sA, sG, tx, ty, X, Y, o);
+  infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
 }
 
 // ============================================================================ passes 3, 4, 6: a-trous
 // atrousShader (hush.js:482-525): mid2 (R1 step 2) and mid3 (R1 step 4) at half res with the F3 guide
 // and learned s/c from par1/par2; coarse2 (R2 step 2, prefilter) at quarter res with the CG guide.
 
-template <int R, int STEP, bool PRE, bool GUIDE_F3, int COMP>
+// GUIDE_PACKED: guide w = binary16 (Z, count) (GH); else w = depth (CGD)
+template <int R, int STEP, bool PRE, bool GUIDE_PACKED, int COMP, bool FAST>
 __global__ __launch_bounds__(NT) void k_atrous(AtrousArgs a) {
   constexpr int AP = R * STEP > 1 ? R * STEP : 1;
   constexpr int SW = TW + 2 * AP, SH = TH + 2 * AP, SN = SW * SH;
   __shared__ float4 sA[SN], sG[SN];
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const int2 org = tile_origin<TW, TH>(a.w, a.h);
+  const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.w - 1), qy = clampi(y0 - AP + i / SW, 0, a.h - 1);
     sA[i] = ld(a.inA, qx, qy);
-    sG[i] = GUIDE_F3 ? guide_f3(ldu(a.guideF, qx, qy)) : guide_tex(ld(a.guideT, qx, qy));
+    const float4 g = ldf(a.guide, qx, qy);
+    sG[i] = GUIDE_PACKED ? guide_gh(g) : g;
   }
   __syncthreads();
   const int lx = tid & 31, ly = tid >> 5;
@@ -500,21 +545,22 @@ __global__ __launch_bounds__(NT) void k_atrous(AtrousArgs a) {
     s = COMP == 0 ? p1.x : COMP == 1 ? p1.y : COMP == 2 ? p1.z : p1.w;
     c = COMP == 0 ? p2.x : COMP == 1 ? p2.y : COMP == 2 ? p2.z : p2.w;
   }
-  st(a.out, X, Y, filt<R, STEP, PRE, SW>(sA, sG, tx, ty, X, Y, a.w, a.h, s, c));
+  st(a.out, X, Y, filt<R, STEP, PRE, SW, FAST>(sA, sG, tx, ty, X, Y, a.w, a.h, s, c));
 }
 
 // ============================================================================ pass 5: coarse1
 // COARSE1 (hush.js:372-422): count-weighted 2x2 half -> quarter down() of the block means and guides into
 // the tile, the coarse guide CG written once per texel, then a-trous 5x5 step 1 with prefilter -> C[1].
 
-__device__ __forceinline__ void down(const TexU &F2, const TexU &F3, int QX, int QY, int W2, int H2, float4 &col, float4 &g) {
+template <bool FAST>
+__device__ __forceinline__ void down(const TexU &F2, const TexF &GH, int QX, int QY, int W2, int H2, float4 &col, float4 &g) {
   float N = 0.0f, vr = 0.0f, z = 0.0f, clx = 0.0f, cly = 0.0f, clz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     int hx = QX * 2 + (j & 1), hy = QY * 2 + (j >> 1);
     if (hx >= W2 || hy >= H2) continue;
-    uint4 g3 = ldu(F3, hx, hy);
-    float2 oz = unpack2(g3.y);
+    const float4 gh = ldf(GH, hx, hy);
+    float2 oz = pair_of(gh.w);
     if (oz.y <= 0.0f) continue;
     uint4 g2 = ldu(F2, hx, hy);
     float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
@@ -522,31 +568,38 @@ __device__ __forceinline__ void down(const TexU &F2, const TexU &F3, int QX, int
     N += w;
     clx += w * a01.x; cly += w * a01.y; clz += w * a23.x;
     vr += w * a23.y;
-    float2 oc = unpack2(g3.x);
-    float3 d = oct_dec(oc.x, oc.y);
-    nx += w * d.x; ny += w * d.y; nz += w * d.z;
+    nx += w * gh.x; ny += w * gh.y; nz += w * gh.z;
     z += w * oz.x;
   }
   if (N == 0.0f) { col = make_float4(0.0f, 0.0f, 0.0f, 0.0f); g = make_float4(0.0f, 0.0f, -1.0f, 0.0f); return; }
-  float ln = mx(sqrtf(nx * nx + ny * ny + nz * nz), 1e-6f);
-  float2 o = oct_enc(make_float3(nx / ln, ny / ln, nz / ln));
-  col = make_float4(qf16(clx / N), qf16(cly / N), qf16(clz / N), qf16(vr / N / N));
-  g = make_float4(qf16(o.x), qf16(o.y), qf16(z / N), 0.0f);
+  using F = Math<FAST>;
+  float ln = mx(F::sqrt(nx * nx + ny * ny + nz * nz), 1e-6f);
+  float2 o = oct_enc_m<FAST>(make_float3(F::div(nx, ln), F::div(ny, ln), F::div(nz, ln)));
+  col = make_float4(qf16(F::div(clx, N)), qf16(F::div(cly, N)), qf16(F::div(clz, N)), qf16(F::div(F::div(vr, N), N)));
+  g = make_float4(qf16(o.x), qf16(o.y), qf16(F::div(z, N)), 0.0f);
 }
 
+template <bool FAST>
 __global__ __launch_bounds__(NT) void k_coarse1(Coarse1Args a) {
   constexpr int AP = 2, SW = TW + 2 * AP, SH = TH + 2 * AP, SN = SW * SH;
   __shared__ float4 sA[SN], sG[SN];
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const int2 org = tile_origin<TW, TH>(a.W4, a.H4);
+  const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int lqx = x0 - AP + i % SW, lqy = y0 - AP + i / SW;
     int qx = clampi(lqx, 0, a.W4 - 1), qy = clampi(lqy, 0, a.H4 - 1);
     float4 col, g;
-    down(a.F2, a.F3, qx, qy, a.W2, a.H2, col, g);
+    down<FAST>(a.F2, a.GH, qx, qy, a.W2, a.H2, col, g);
     sA[i] = col;
-    sG[i] = guide_tex(g);
-    if (lqx == qx && lqy == qy && lqx >= x0 && lqy >= y0 && lqx < x0 + TW && lqy < y0 + TH) st(a.outG, qx, qy, g);
+    sG[i] = guide_tex<FAST>(g);
+    if (lqx == qx && lqy == qy && lqx >= x0 && lqy >= y0 && lqx < x0 + TW && lqy < y0 + TH) {
+      const uint2 gp = pack4(g);
+      a.outG.p[(size_t)qy * a.outG.pitch + qx] = gp;
+      const float4 gs = unpack4(gp);
+      const float3 d = oct_dec(gs.x, gs.y);
+      stf(a.outGD, qx, qy, make_float4(d.x, d.y, d.z, gs.z));
+    }
   }
   __syncthreads();
   const int lx = tid & 31, ly = tid >> 5;
@@ -554,7 +607,7 @@ __global__ __launch_bounds__(NT) void k_coarse1(Coarse1Args a) {
   if (!inside(X, Y, a.W4, a.H4)) return;
   const int tx = lx + AP, ty = ly + AP;
   if (sG[ty * SW + tx].w < 0.0f) { st(a.outC, X, Y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
-  st(a.outC, X, Y, filt<2, 1, true, SW>(sA, sG, tx, ty, X, Y, a.W4, a.H4, a.s, 1.0f));
+  st(a.outC, X, Y, filt<2, 1, true, SW, FAST>(sA, sG, tx, ty, X, Y, a.W4, a.H4, a.s, 1.0f));
 }
 
 // ============================================================================ pass 7: fine + final
@@ -562,23 +615,28 @@ __global__ __launch_bounds__(NT) void k_coarse1(Coarse1Args a) {
 // histColor[cur]; joint-bilateral upsample of the mid (Bh[0]) and coarse (C[0]) levels; recombine with
 // kappa and beta; remodulate -> X.
 
+template <bool FAST>
 __device__ __forceinline__ float jbw(float frx, float fry, int j, float3 nq, float zq, float3 n, float z) {
   float wb = ((j & 1) == 1 ? frx : 1.0f - frx) * ((j >> 1) == 1 ? fry : 1.0f - fry);
   float dn = mx(dot3(nq, n), 0.0f);
   float dn2 = dn * dn;
   float dn4 = dn2 * dn2;
-  return wb * expf(-fabsf(zq - z) / (0.05f * z + 1e-3f)) * dn4 * dn4;
+  return wb * Math<FAST>::exp(Math<FAST>::div(-fabsf(zq - z), 0.05f * z + 1e-3f)) * dn4 * dn4;
 }
 
+template <bool FAST>
 __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
+  using F = Math<FAST>;
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
   __shared__ float4 sA[SN], sG[SN];
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const int2 org = tile_origin<TW, TH>(a.W, a.H);
+  const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
     sA[i] = ld(a.A0, qx, qy);
-    sG[i] = guide_tex(ld(a.key, qx, qy));
+    const float4 kd = ldf(a.kd, qx, qy);
+    sG[i] = make_float4(kd.x, kd.y, kd.z, pair_of(kd.w).x);
   }
   __syncthreads();
   const int lx = tid & 31, ly = tid >> 5;
@@ -594,7 +652,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   }
   const int hx = px >> 1, hy = py >> 1;
   const float4 p1 = ld(a.par1, hx, hy), p2 = ld(a.par2, hx, hy), pr = ld(a.par0, hx, hy);
-  const float4 A1 = filt<1, 1, true, SW>(sA, sG, tx, ty, px, py, a.W, a.H, p1.x, p2.x);
+  const float4 A1 = filt<1, 1, true, SW, FAST>(sA, sG, tx, ty, px, py, a.W, a.H, p1.x, p2.x);
   st(a.outHist, px, py, A1);
   const float fx = qf16(A1.x), fy = qf16(A1.y), fz = qf16(A1.z);
   const float3 n = make_float3(gp.x, gp.y, gp.z);
@@ -607,11 +665,10 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   for (int j = 0; j < 4; j++) {
     int qx = (int)hbx + (j & 1), qy = (int)hby + (j >> 1);
     if (qx < 0 || qy < 0 || qx >= a.W2 || qy >= a.H2) continue;
-    uint4 g3 = ldu(a.F3, qx, qy);
-    float2 oz = unpack2(g3.y);
+    const float4 gh = ldf(a.GH, qx, qy);
+    const float2 oz = pair_of(gh.w);
     if (oz.x < 0.0f) continue;
-    float2 oc = unpack2(g3.x);
-    float w = jbw(hpx - hbx, hpy - hby, j, oct_dec(oc.x, oc.y), oz.x, n, z);
+    float w = jbw<FAST>(hpx - hbx, hpy - hby, j, make_float3(gh.x, gh.y, gh.z), oz.x, n, z);
     uint4 g2 = ldu(a.F2, qx, qy);
     float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
     swh += w;
@@ -621,9 +678,9 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   }
   float ix = fx, iy = fy, iz = fz;
   if (swh > 1e-4f) {
-    ix = uBx / swh + pr.w * (fx - uAx / swh);
-    iy = uBy / swh + pr.w * (fy - uAy / swh);
-    iz = uBz / swh + pr.w * (fz - uAz / swh);
+    ix = F::div(uBx, swh) + pr.w * (fx - F::div(uAx, swh));
+    iy = F::div(uBy, swh) + pr.w * (fy - F::div(uAy, swh));
+    iz = F::div(uBz, swh) + pr.w * (fz - F::div(uAz, swh));
   }
   // quarter-res coarse level (hush.js:604-619)
   const float cpx = ((float)px + 0.5f) / 4.0f - 0.5f, cpy = ((float)py + 0.5f) / 4.0f - 0.5f;
@@ -633,15 +690,15 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   for (int j = 0; j < 4; j++) {
     int qx = (int)cbx + (j & 1), qy = (int)cby + (j >> 1);
     if (qx < 0 || qy < 0 || qx >= a.W4 || qy >= a.H4) continue;
-    float4 g = ld(a.cguide, qx, qy);
-    if (g.z < 0.0f) continue;
-    float w = jbw(cpx - cbx, cpy - cby, j, oct_dec(g.x, g.y), g.z, n, z);
+    const float4 g = ldf(a.cgd, qx, qy);
+    if (g.w < 0.0f) continue;
+    float w = jbw<FAST>(cpx - cbx, cpy - cby, j, make_float3(g.x, g.y, g.z), g.w, n, z);
     swq += w;
     float4 cc = ld(a.coarse, qx, qy);
     uCx += w * cc.x; uCy += w * cc.y; uCz += w * cc.z;
   }
   float upx = ix, upy = iy, upz = iz;
-  if (swq > 1e-4f) { upx = uCx / swq; upy = uCy / swq; upz = uCz / swq; }
+  if (swq > 1e-4f) { upx = F::div(uCx, swq); upy = F::div(uCy, swq); upz = F::div(uCz, swq); }
   ix = ix + (upx - ix) * pr.z;
   iy = iy + (upy - iy) * pr.z;
   iz = iz + (upz - iz) * pr.z;
@@ -652,25 +709,31 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
 // TAA (hush.js:625-678): YCoCg variance clip (gamma 1.25) of a manually bilinear history, blend 0.1 in
 // Karis tone-mapped space.
 
-__device__ __forceinline__ float3 tm(float3 c) { float d = 1.0f + lum(c.x, c.y, c.z); return make_float3(c.x / d, c.y / d, c.z / d); }
-__device__ __forceinline__ float3 itm(float3 c) {
+template <bool FAST> __device__ __forceinline__ float3 tm(float3 c) {
+  float d = 1.0f + lum(c.x, c.y, c.z);
+  return make_float3(Math<FAST>::div(c.x, d), Math<FAST>::div(c.y, d), Math<FAST>::div(c.z, d));
+}
+template <bool FAST> __device__ __forceinline__ float3 itm(float3 c) {
   float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f);
-  return make_float3(c.x / d, c.y / d, c.z / d);
+  return make_float3(Math<FAST>::div(c.x, d), Math<FAST>::div(c.y, d), Math<FAST>::div(c.z, d));
 }
 __device__ __forceinline__ float3 to_ycocg(float3 c) {
   return make_float3(c.x * 0.25f + c.y * 0.5f + c.z * 0.25f, c.x * 0.5f - c.z * 0.5f, -c.x * 0.25f + c.y * 0.5f - c.z * 0.25f);
 }
 __device__ __forceinline__ float3 from_ycocg(float3 c) { return make_float3(c.x + c.y - c.z, c.x + c.z, c.x - c.y - c.z); }
 
+template <bool FAST>
 __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
+  using F = Math<FAST>;
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
   __shared__ float4 sX[SN];
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const int2 org = tile_origin<TW, TH>(a.W, a.H);
+  const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
     float4 x = ld(a.X, qx, qy);
-    float3 y = to_ycocg(tm(make_float3(x.x, x.y, x.z)));
+    float3 y = to_ycocg(tm<FAST>(make_float3(x.x, x.y, x.z)));
     sX[i] = make_float4(y.x, y.y, y.z, 0.0f);
   }
   __syncthreads();
@@ -678,9 +741,9 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
   const int px = x0 + lx, py = y0 + ly;
   if (px >= a.W || py >= a.H) return;
   const float4 xc = ld(a.X, px, py);
-  const float3 cur = tm(make_float3(xc.x, xc.y, xc.z));
+  const float3 cur = tm<FAST>(make_float3(xc.x, xc.y, xc.z));
   if (a.reset) {
-    float3 r = itm(cur);
+    float3 r = itm<FAST>(cur);
     st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
     return;
   }
@@ -691,7 +754,7 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
     float4 s = sX[c0 + (j / 3 - 1) * SW + (j % 3 - 1)];
     m1x += s.x; m1y += s.y; m1z += s.z;
   }
-  m1x /= 9.0f; m1y /= 9.0f; m1z /= 9.0f;
+  m1x = F::div(m1x, 9.0f); m1y = F::div(m1y, 9.0f); m1z = F::div(m1z, 9.0f);
   float vx = 0.0f, vy = 0.0f, vz = 0.0f;
 #pragma unroll
   for (int j = 0; j < 9; j++) {
@@ -699,7 +762,8 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
     float dx = s.x - m1x, dy = s.y - m1y, dz = s.z - m1z;
     vx += dx * dx; vy += dy * dy; vz += dz * dz;
   }
-  const float sdx = sqrtf(mx(vx / 9.0f, 1e-12f)), sdy = sqrtf(mx(vy / 9.0f, 1e-12f)), sdz = sqrtf(mx(vz / 9.0f, 1e-12f));
+  const float sdx = F::sqrt(mx(F::div(vx, 9.0f), 1e-12f)), sdy = F::sqrt(mx(F::div(vy, 9.0f), 1e-12f)),
+              sdz = F::sqrt(mx(F::div(vz, 9.0f), 1e-12f));
   const float4 mv = ld(a.mv, px, py);
   const float szx = (float)a.W, szy = (float)a.H;
   const float ux = ((float)px + 0.5f + mv.x) / szx, uy = ((float)py + 0.5f + mv.y) / szy;
@@ -716,7 +780,7 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
       float4 t = ld(a.prev, clampi((int)bx + ox, 0, a.W - 1), clampi((int)by + oy, 0, a.H - 1));
       hx += wb * t.x; hy += wb * t.y; hz += wb * t.z;
     }
-    float3 hh = to_ycocg(tm(make_float3(hx, hy, hz)));
+    float3 hh = to_ycocg(tm<FAST>(make_float3(hx, hy, hz)));
     float kx = 1.25f * sdx, ky = 1.25f * sdy, kz = 1.25f * sdz;
     hh.x = mn(mx(hh.x, m1x - kx), m1x + kx);
     hh.y = mn(mx(hh.y, m1y - ky), m1y + ky);
@@ -724,54 +788,72 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
     float3 hr = from_ycocg(hh);
     res = make_float3(hr.x + (cur.x - hr.x) * 0.1f, hr.y + (cur.y - hr.y) * 0.1f, hr.z + (cur.z - hr.z) * 0.1f);
   }
-  float3 r = itm(res);
+  float3 r = itm<FAST>(res);
   st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
 }
 
 // ============================================================================ launchers
 
-static dim3 grid_for(int w, int h) { return dim3((w + TW - 1) / TW, (h + TH - 1) / TH); }
+// 1-D grids: tile_origin() maps the workgroup index to a tile
+static dim3 grid_for(int w, int h) { return dim3(((w + TW - 1) / TW) * ((h + TH - 1) / TH)); }
 
-hipError_t launch_temporal(const TemporalArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_temporal, grid_for(a.W, a.H), dim3(NT), 0, s, a);
+#define HUSH_LAUNCH(KERNEL, GRID, ARGS)                                                  \
+  do {                                                                                   \
+    if (fast) hipLaunchKernelGGL(KERNEL(true), GRID, dim3(NT), 0, s, ARGS);             \
+    else hipLaunchKernelGGL(KERNEL(false), GRID, dim3(NT), 0, s, ARGS);                 \
+  } while (0)
+
+#define K_TEMPORAL(F) k_temporal<F>
+#define K_INFER16(F) k_infer<16, 0, F>
+#define K_INFER(F) k_infer<0, 0, F>
+#define K_INFER_F16(F) k_infer<0, 1, F>
+#define K_MID2(F) k_atrous<1, 2, false, true, 2, F>
+#define K_MID3(F) k_atrous<1, 4, false, true, 3, F>
+#define K_COARSE2(F) k_atrous<2, 2, true, false, -1, F>
+#define K_COARSE1(F) k_coarse1<F>
+#define K_FINE(F) k_fine_final<F>
+#define K_TAA(F) k_taa<F>
+
+hipError_t launch_temporal(const TemporalArgs &a, bool fast, hipStream_t s) {
+  HUSH_LAUNCH(K_TEMPORAL, grid_for(a.W, a.H), a);
   return hipGetLastError();
 }
 
-hipError_t launch_infer(const InferArgs &a, int mode, hipStream_t s) {
+hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s) {
   dim3 g = grid_for(a.W2, a.H2);
   if (mode == MLP_FP32) {
-    if (a.hidden == 16) hipLaunchKernelGGL((k_infer<16, 0>), g, dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL((k_infer<0, 0>), g, dim3(NT), 0, s, a);
+    if (a.hidden == 16) HUSH_LAUNCH(K_INFER16, g, a);
+    else HUSH_LAUNCH(K_INFER, g, a);
   } else if (mode == MLP_F16) {
-    hipLaunchKernelGGL((k_infer<0, 1>), g, dim3(NT), 0, s, a);
+    HUSH_LAUNCH(K_INFER_F16, g, a);
   } else {
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_mid2(const AtrousArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL((k_atrous<1, 2, false, true, 2>), grid_for(a.w, a.h), dim3(NT), 0, s, a);
+hipError_t launch_mid2(const AtrousArgs &a, bool fast, hipStream_t s) {
+  HUSH_LAUNCH(K_MID2, grid_for(a.w, a.h), a);
   return hipGetLastError();
 }
-hipError_t launch_mid3(const AtrousArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL((k_atrous<1, 4, false, true, 3>), grid_for(a.w, a.h), dim3(NT), 0, s, a);
+hipError_t launch_mid3(const AtrousArgs &a, bool fast, hipStream_t s) {
+  HUSH_LAUNCH(K_MID3, grid_for(a.w, a.h), a);
   return hipGetLastError();
 }
-hipError_t launch_coarse1(const Coarse1Args &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_coarse1, grid_for(a.W4, a.H4), dim3(NT), 0, s, a);
+hipError_t launch_coarse1(const Coarse1Args &a, bool fast, hipStream_t s) {
+  HUSH_LAUNCH(K_COARSE1, grid_for(a.W4, a.H4), a);
   return hipGetLastError();
 }
-hipError_t launch_coarse2(const AtrousArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL((k_atrous<2, 2, true, false, -1>), grid_for(a.w, a.h), dim3(NT), 0, s, a);
+hipError_t launch_coarse2(const AtrousArgs &a, bool fast, hipStream_t s) {
+  HUSH_LAUNCH(K_COARSE2, grid_for(a.w, a.h), a);
   return hipGetLastError();
 }
-hipError_t launch_fine_final(const FineArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_fine_final, grid_for(a.W, a.H), dim3(NT), 0, s, a);
+hipError_t launch_fine_final(const FineArgs &a, bool fast, hipStream_t s) {
+  HUSH_LAUNCH(K_FINE, grid_for(a.W, a.H), a);
   return hipGetLastError();
 }
-hipError_t launch_taa(const TaaArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_taa, grid_for(a.W, a.H), dim3(NT), 0, s, a);
+hipError_t launch_taa(const TaaArgs &a, bool fast, hipStream_t s) {
+  HUSH_LAUNCH(K_TAA, grid_for(a.W, a.H), a);
   return hipGetLastError();
 }
 

@@ -51,7 +51,7 @@ enum hush_precision {
   HUSH_MLP_FP8_MFMA = 4       /* fp8 E4M3 operands with per-layer scales on MFMA, fp32 accumulation */
 };
 
-enum hush_format { HUSH_RGBA16F = 0, HUSH_RGBA32U = 1, HUSH_R8U = 2 };
+enum hush_format { HUSH_RGBA16F = 0, HUSH_RGBA32U = 1, HUSH_R8U = 2, HUSH_RGBA32F = 3 };
 
 /* a device surface: `ptr` points at texel (0, 0), rows are `pitch` bytes apart */
 typedef struct {
@@ -98,6 +98,13 @@ int64_t hush_frame(const hush_t *h);
 int32_t hush_set_precision(hush_t *h, int32_t precision);
 int32_t hush_get_precision(const hush_t *h);
 
+/* Math policy of later frames (no counterpart in the reference, whose arithmetic is the browser's):
+   0 (default) = FAST: hardware reciprocal / square root / exp2 / log2 for continuous values;
+   1 = IEEE: correctly rounded division and square root, accurate exp / log throughout.
+   Both evaluate every threshold decision (reprojection validity, sky, history key) with IEEE arithmetic. */
+int32_t hush_set_ieee_math(hush_t *h, int32_t on);
+int32_t hush_get_ieee_math(const hush_t *h);
+
 /* denoise(encoder, inputs) (hush.js:813-822): validates the five inputs, enqueues the 8 passes on
    `stream` and returns the output surface in *out (may be NULL). */
 int32_t hush_denoise(hush_t *h, const hush_inputs *in, void *stream, hush_surface *out);
@@ -122,7 +129,10 @@ int32_t hush_pass_times(hush_t *h, float *ms, const char **labels);
 
 /* internal surfaces by id (hush.js:783-793): 0 A0, 1 X, 2 Bh0, 3 Bh1, 4 mom0, 5 mom1, 6 key0, 7 key1,
    8 histColor0, 9 histColor1, 10 taa0, 11 taa1, 12 par0_0, 13 par0_1, 14 par1, 15 par2, 16..19 F0..F3,
-   20 C0, 21 C1, 22 CG, 23 reprojection mask (HUSH_R8U; written only while enabled) */
+   20 C0, 21 C1, 22 CG, 23 reprojection mask (HUSH_R8U; written only while enabled); decoded guides
+   (HUSH_RGBA32F, no counterpart in the reference: xyz = octDec of the stored normal): 24, 25 key of
+   frame parity 0 / 1 (w = binary16 depth, id), 26 half-res guide (w = binary16 Z, count), 27 coarse
+   guide (w = depth) */
 int32_t hush_debug_surface(hush_t *h, int32_t id, hush_surface *out);
 /* set the frame counter and the reset flag (teacher-forced single passes) */
 int32_t hush_debug_set_state(hush_t *h, int64_t frame, int32_t reset);

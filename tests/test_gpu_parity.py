@@ -3,10 +3,12 @@
 Teacher forcing: before each pass the oracle receives the GPU's full state, so each kernel is checked in
 isolation; the end-to-end test then runs whole sequences independently on both sides.
 
-Tolerances (FP32 path):
-  * reprojection validity masks (tap bits, fallback, has, sky): bit-exact;
-  * stored outputs: at most 1 binary16 ULP per pass under teacher forcing (RTZ boundary flips from the
-    1-ULP differences of expf / logf between the GPU's ocml and glibc), and >= 99.9% bit-exact;
+Both math policies are checked.  Tolerances (FP32 path):
+  * reprojection validity masks (tap bits, fallback, has, sky): bit-exact under either policy;
+  * IEEE policy, teacher forced: at most 1 binary16 ULP per stored value (RTZ boundary flips from the
+    1-ULP differences of expf / logf between the GPU's ocml and glibc) and >= 99.9% bit-exact;
+  * FAST policy, teacher forced: at most 2 binary16 ULP, >= 99% bit-exact, relative error <= 1e-3 for
+    >= 99.9% of values (hardware rcp / sqrt / exp2 / log2 are 1-2 fp32 ulp);
   * end-to-end HDR output: |gpu - oracle| <= 1e-3 * max(|oracle|, 1e-3) for >= 99.9% of values, the rest
     within 1e-2 relative (history feedback compounds single-ULP flips over frames).
 """
@@ -33,11 +35,11 @@ def _oracle_out(o, name):
     return o.surface(name)
 
 
-def teacher_forced(kind, w, h, frames, seed, exposure=1.0):
+def teacher_forced(kind, w, h, frames, seed, exposure=1.0, ieee=True):
     import hush_amd
     gen = _gen(kind, w, h, seed)
     wts = hush_amd.default_weights()
-    hs = hush_amd.Hush(0, {"width": w, "height": h, "exposure": exposure})
+    hs = hush_amd.Hush(0, {"width": w, "height": h, "exposure": exposure, "ieee_math": ieee})
     hs.set_debug_mask(True)
     o = Oracle(w, h, wts, exposure=exposure)
     stats = []
@@ -80,23 +82,28 @@ def _report(tag, stats):
         print(tag, s)
 
 
+@pytest.mark.parametrize("ieee", [True, False], ids=["ieee", "fast"])
 @pytest.mark.parametrize("kind,w,h,frames,seed", [("spheres", 640, 360, 4, 1), ("boxes", 160, 96, 4, 2),
                                                   ("train", 63, 37, 3, 3)])
-def test_teacher_forced_passes(kind, w, h, frames, seed):
-    stats = teacher_forced(kind, w, h, frames, seed)
-    _report(f"tf-{kind}-{w}x{h}", stats)
+def test_teacher_forced_passes(kind, w, h, frames, seed, ieee):
+    stats = teacher_forced(kind, w, h, frames, seed, ieee=ieee)
+    _report(f"tf-{kind}-{w}x{h}-{'ieee' if ieee else 'fast'}", stats)
     masks = [s for s in stats if s["surface"] == "mask"]
     assert all(s["exact"] == 1.0 for s in masks), masks
     vals = [s for s in stats if s["surface"] != "mask"]
-    bad = [s for s in vals if s["max_ulp"] > 1.0 or s["exact"] < 0.999]
+    if ieee:
+        bad = [s for s in vals if s["max_ulp"] > 1.0 or s["exact"] < 0.999]
+    else:
+        bad = [s for s in vals if s["max_ulp"] > 2.0 or s["exact"] < 0.99 or s["frac_rel_gt_1e-3"] > 1e-3]
     assert not bad, bad
 
 
-def test_end_to_end_sequence():
+@pytest.mark.parametrize("ieee", [True, False], ids=["ieee", "fast"])
+def test_end_to_end_sequence(ieee):
     import hush_amd
     w, h, frames = 640, 360, 8
     gen = _gen("spheres", w, h, 7)
-    hs = hush_amd.Hush(0, {"width": w, "height": h})
+    hs = hush_amd.Hush(0, {"width": w, "height": h, "ieee_math": ieee})
     hs.set_debug_mask(True)
     o = Oracle(w, h, hush_amd.default_weights())
     surfs = None
@@ -110,9 +117,51 @@ def test_end_to_end_sequence():
         st["frame"] = f
         st["mask_exact"] = float(np.mean(hs.surface("mask").download()[..., 0] == o.mask()))
         stats.append(st)
-    _report("e2e-spheres-640x360", stats)
+    _report(f"e2e-spheres-640x360-{'ieee' if ieee else 'fast'}", stats)
     hs.destroy()
     for st in stats:
         assert st["mask_exact"] == 1.0, st
         assert st["frac_rel_gt_1e-3"] <= 1e-3, st
         assert st["max_rel"] <= 1e-2, st
+
+
+def _oct_dec_np(ex, ey):
+    """octDec (hush.js:40-46) in numpy float32: IEEE-exact, same operation order as the kernels' oct_dec"""
+    ex = ex.astype(np.float32)
+    ey = ey.astype(np.float32)
+    vx, vy = ex.copy(), ey.copy()
+    vz = (np.float32(1) - np.abs(ex)) - np.abs(ey)
+    t = np.maximum(-vz, np.float32(0))
+    vx = vx + np.where(vx >= 0, -t, t)
+    vy = vy + np.where(vy >= 0, -t, t)
+    l = np.maximum(np.sqrt((vx * vx + vy * vy) + vz * vz), np.float32(1e-20))
+    return np.stack([vx / l, vy / l, vz / l], axis=-1)
+
+
+@pytest.mark.parametrize("ieee", [True, False], ids=["ieee", "fast"])
+def test_decoded_guides_match_reference_surfaces(ieee):
+    """The decoded-guide surfaces the kernels read (history key, half-res and coarse guides) hold exactly the
+    IEEE octDec of the reference's stored surfaces (key, F3, CG) and the same binary16 depth / id / count."""
+    import hush_amd
+    w, h = 200, 120
+    gen = _gen("boxes", w, h, 5)
+    hs = hush_amd.Hush(0, {"width": w, "height": h, "ieee_math": ieee})
+    surfs = None
+    for f in range(3):
+        surfs = upload_frame(gen.render(f), surfs)
+        hs.denoise(surfs)
+        cur = f & 1
+        key = hs.surface(f"key{cur}").download()
+        kd = hs.surface(f"KD{cur}").download()
+        assert np.array_equal(kd[..., :3], _oct_dec_np(key[..., 0], key[..., 1]))
+        assert np.array_equal(kd[..., 3:].view(np.uint32)[..., 0], key[..., 2:].copy().view(np.uint32)[..., 0])
+        f3 = hs.surface("F3").download()
+        gh = hs.surface("GH").download()
+        o = f3[..., :1].copy().view(np.float16)
+        assert np.array_equal(gh[..., :3], _oct_dec_np(o[..., 0], o[..., 1]))
+        assert np.array_equal(gh[..., 3].view(np.uint32), f3[..., 1])
+        cg = hs.surface("CG").download()
+        cgd = hs.surface("CGD").download()
+        assert np.array_equal(cgd[..., :3], _oct_dec_np(cg[..., 0], cg[..., 1]))
+        assert np.array_equal(cgd[..., 3], cg[..., 2].astype(np.float32))
+    hs.destroy()
