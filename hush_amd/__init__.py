@@ -115,7 +115,7 @@ class Hush:
         exposure = float(opts.get("exposure", 1.0))
         weights = opts.get("weights")
         precision = opts.get("precision", "f16" if opts.get("f16") else "fp32")
-        ieee = bool(opts.get("ieee_math", False))
+        ieee = bool(opts.get("ieee_math", True))
         limits = getattr(device, "limits", None)
         if limits is not None:
             for k, v in self.requiredLimits.items():
@@ -135,8 +135,8 @@ class Hush:
             raise HushError(self._L.hush_last_error(None).decode(), r)
         self._h = p.value
         self.device = ordinal
-        if ieee:
-            self.ieee_math = True
+        if not ieee:
+            self.ieee_math = False
         self._timer_on = False
 
     # ---- helpers
@@ -196,8 +196,8 @@ class Hush:
 
     @property
     def ieee_math(self) -> bool:
-        """True: strict IEEE arithmetic everywhere (the oracle's operation-exact mirror); False (default):
-        hardware reciprocal / sqrt / exp2 / log2 for continuous values, IEEE for every threshold decision."""
+        """True (default): IEEE arithmetic everywhere, the oracle's operation-exact mirror.  False: hardware
+        reciprocal / sqrt / exp2 / log2 in the spatial filters; IEEE for every threshold decision."""
         return bool(self._L.hush_get_ieee_math(self._h))
 
     @ieee_math.setter

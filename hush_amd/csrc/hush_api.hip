@@ -57,7 +57,7 @@ struct hush_ctx {
   uint8_t *mask = nullptr;
   int mask_pitch = 0;
   bool mask_on = false;
-  bool ieee = false;               // strict IEEE math policy (the oracle's operation-exact mirror)
+  bool ieee = true;                // IEEE math policy (the oracle's operation-exact mirror); false = FAST
   bool timing = false;
   bool ev_ready = false;
   hipEvent_t ev[9] = {};

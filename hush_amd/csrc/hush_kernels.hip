@@ -32,8 +32,10 @@ __device__ __forceinline__ bool check(float4 kd, float3 n, float zprev, float ob
 // per-pixel feature inputs kept in LDS for the block phase (structure-of-arrays, conflict-free)
 enum { P0X, P0Y, P0Z, P0W, P1X, P1Y, P1Z, P1W, P2Z, P2W, NSX, NSY, NSZ, P3X, P3Y, P3Z, P3W, P4X, P4Y, P4Z, P4W, NP };
 
-template <bool FAST>
+// Temporal statistics (moments, variances, cancellation-prone differences such as m2 - m1^2) are
+// ill-conditioned, so this pass is evaluated with IEEE arithmetic under both math policies.
 __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
+  constexpr bool FAST = false;
   using F = Math<FAST>;
   constexpr int SW = TW + 2, SH = TH + 2, SN = SW * SH;
   __shared__ float4 sI[SN];        // demodulated illumination, valid flag
@@ -572,7 +574,7 @@ __device__ __forceinline__ void down(const TexU &F2, const TexF &GH, int QX, int
     z += w * oz.x;
   }
   if (N == 0.0f) { col = make_float4(0.0f, 0.0f, 0.0f, 0.0f); g = make_float4(0.0f, 0.0f, -1.0f, 0.0f); return; }
-  using F = Math<FAST>;
+  using F = Math<false>;   // stored level values: IEEE under both policies (they set the slope sl)
   float ln = mx(F::sqrt(nx * nx + ny * ny + nz * nz), 1e-6f);
   float2 o = oct_enc_m<FAST>(make_float3(F::div(nx, ln), F::div(ny, ln), F::div(nz, ln)));
   col = make_float4(qf16(F::div(clx, N)), qf16(F::div(cly, N)), qf16(F::div(clz, N)), qf16(F::div(F::div(vr, N), N)));
@@ -709,13 +711,15 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
 // TAA (hush.js:625-678): YCoCg variance clip (gamma 1.25) of a manually bilinear history, blend 0.1 in
 // Karis tone-mapped space.
 
-template <bool FAST> __device__ __forceinline__ float3 tm(float3 c) {
+// Karis tone map and its inverse, IEEE under both policies: itm(tm(x)) lands on x exactly for most texels
+// (the reset frame), where any negative error would drop a round-toward-zero store by a full binary16 ulp.
+__device__ __forceinline__ float3 tm(float3 c) {
   float d = 1.0f + lum(c.x, c.y, c.z);
-  return make_float3(Math<FAST>::div(c.x, d), Math<FAST>::div(c.y, d), Math<FAST>::div(c.z, d));
+  return make_float3(c.x / d, c.y / d, c.z / d);
 }
-template <bool FAST> __device__ __forceinline__ float3 itm(float3 c) {
+__device__ __forceinline__ float3 itm(float3 c) {
   float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f);
-  return make_float3(Math<FAST>::div(c.x, d), Math<FAST>::div(c.y, d), Math<FAST>::div(c.z, d));
+  return make_float3(c.x / d, c.y / d, c.z / d);
 }
 __device__ __forceinline__ float3 to_ycocg(float3 c) {
   return make_float3(c.x * 0.25f + c.y * 0.5f + c.z * 0.25f, c.x * 0.5f - c.z * 0.5f, -c.x * 0.25f + c.y * 0.5f - c.z * 0.25f);
@@ -733,7 +737,7 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
     float4 x = ld(a.X, qx, qy);
-    float3 y = to_ycocg(tm<FAST>(make_float3(x.x, x.y, x.z)));
+    float3 y = to_ycocg(tm(make_float3(x.x, x.y, x.z)));
     sX[i] = make_float4(y.x, y.y, y.z, 0.0f);
   }
   __syncthreads();
@@ -741,9 +745,9 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
   const int px = x0 + lx, py = y0 + ly;
   if (px >= a.W || py >= a.H) return;
   const float4 xc = ld(a.X, px, py);
-  const float3 cur = tm<FAST>(make_float3(xc.x, xc.y, xc.z));
+  const float3 cur = tm(make_float3(xc.x, xc.y, xc.z));
   if (a.reset) {
-    float3 r = itm<FAST>(cur);
+    float3 r = itm(cur);
     st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
     return;
   }
@@ -780,7 +784,7 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
       float4 t = ld(a.prev, clampi((int)bx + ox, 0, a.W - 1), clampi((int)by + oy, 0, a.H - 1));
       hx += wb * t.x; hy += wb * t.y; hz += wb * t.z;
     }
-    float3 hh = to_ycocg(tm<FAST>(make_float3(hx, hy, hz)));
+    float3 hh = to_ycocg(tm(make_float3(hx, hy, hz)));
     float kx = 1.25f * sdx, ky = 1.25f * sdy, kz = 1.25f * sdz;
     hh.x = mn(mx(hh.x, m1x - kx), m1x + kx);
     hh.y = mn(mx(hh.y, m1y - ky), m1y + ky);
@@ -788,7 +792,7 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
     float3 hr = from_ycocg(hh);
     res = make_float3(hr.x + (cur.x - hr.x) * 0.1f, hr.y + (cur.y - hr.y) * 0.1f, hr.z + (cur.z - hr.z) * 0.1f);
   }
-  float3 r = itm<FAST>(res);
+  float3 r = itm(res);
   st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
 }
 
@@ -803,7 +807,6 @@ static dim3 grid_for(int w, int h) { return dim3(((w + TW - 1) / TW) * ((h + TH 
     else hipLaunchKernelGGL(KERNEL(false), GRID, dim3(NT), 0, s, ARGS);                 \
   } while (0)
 
-#define K_TEMPORAL(F) k_temporal<F>
 #define K_INFER16(F) k_infer<16, 0, F>
 #define K_INFER(F) k_infer<0, 0, F>
 #define K_INFER_F16(F) k_infer<0, 1, F>
@@ -814,8 +817,8 @@ static dim3 grid_for(int w, int h) { return dim3(((w + TW - 1) / TW) * ((h + TH 
 #define K_FINE(F) k_fine_final<F>
 #define K_TAA(F) k_taa<F>
 
-hipError_t launch_temporal(const TemporalArgs &a, bool fast, hipStream_t s) {
-  HUSH_LAUNCH(K_TEMPORAL, grid_for(a.W, a.H), a);
+hipError_t launch_temporal(const TemporalArgs &a, bool, hipStream_t s) {
+  hipLaunchKernelGGL(k_temporal, grid_for(a.W, a.H), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 

@@ -99,8 +99,9 @@ int32_t hush_set_precision(hush_t *h, int32_t precision);
 int32_t hush_get_precision(const hush_t *h);
 
 /* Math policy of later frames (no counterpart in the reference, whose arithmetic is the browser's):
-   0 (default) = FAST: hardware reciprocal / square root / exp2 / log2 for continuous values;
-   1 = IEEE: correctly rounded division and square root, accurate exp / log throughout.
+   1 (default) = IEEE: correctly rounded division and square root, accurate exp / log throughout;
+   0 = FAST: hardware reciprocal / square root / exp2 / log2 in the spatial filters, MLP activations and
+   upsampling weights (about 1% of output values move by more than 1e-3 relative after a few frames).
    Both evaluate every threshold decision (reprojection validity, sky, history key) with IEEE arithmetic. */
 int32_t hush_set_ieee_math(hush_t *h, int32_t on);
 int32_t hush_get_ieee_math(const hush_t *h);
