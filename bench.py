@@ -52,6 +52,55 @@ def measured_traffic(kernel: str, grid: int, precision: str):
     return None, None
 
 
+def mlp_stage(hs, hip, W, H, reps=20):
+    """The MLP alone (hush_mlp_forward) over this frame's real half-res feature rows, per precision: mean
+    duration over `reps` launches with hipEvents, and achieved TFLOP/s at 1536 FLOP per row."""
+    F = np.concatenate([hs.surface(f"F{k}").download().view(np.float16) for k in range(3)], -1)
+    rows = np.ascontiguousarray(F.reshape(-1, 24)[:, :20])
+    n = rows.shape[0]
+    fin, fout = hip.DeviceBuffer(rows.nbytes), hip.DeviceBuffer(n * 48)
+    hip.check(hip.lib().hipMemcpy(hip.ctypes.c_void_p(fin.ptr), rows.ctypes.data_as(hip.ctypes.c_void_p), rows.nbytes,
+                                  hip.hipMemcpyHostToDevice), "features")
+    res = {}
+    e0, e1 = hip.Event(), hip.Event()
+    for p in ("fp32", "f16", "fp16-mfma", "bf16-mfma", "fp8-mfma"):
+        hs.mlp_forward(fin, fout, n, p)
+        hip.synchronize()
+        e0.record()
+        for _ in range(reps):
+            hs.mlp_forward(fin, fout, n, p)
+        e1.record()
+        e1.synchronize()
+        us = e0.elapsed_ms(e1) / reps * 1e3
+        res[p] = {"us": round(us, 2), "TFLOP/s": round(1536.0 * n / (us * 1e-6) / 1e12, 2)}
+    res["rows"] = n
+    res["fp16_mfma_speedup_vs_fp32"] = round(res["fp32"]["us"] / res["fp16-mfma"]["us"], 2)
+    return res
+
+
+def psnr_check(precisions, w=640, h=360, frames=8):
+    """PSNR (tone-mapped, peak 1) of each MLP precision's output against the FP64 oracle, mean over an
+    8-frame sequence of the same scene type at 640x360"""
+    from oracle.ho import Oracle
+    from tests.parity import psnr, upload_frame
+    import hush_amd
+    from hush_amd.scene import SceneGenerator
+    gen = SceneGenerator("boxes", w, h, seed=1)
+    gs = [gen.render(f) for f in range(frames)]
+    o = Oracle(w, h, hush_amd.default_weights(), mode="fp64")
+    refs = [o.denoise(g) for g in gs]
+    out = {}
+    for p in precisions:
+        hs = hush_amd.Hush(0, {"width": w, "height": h, "precision": p})
+        surfs, vals = None, []
+        for g, r in zip(gs, refs):
+            surfs = upload_frame(g, surfs)
+            vals.append(psnr(hs.denoise(surfs).download().astype(np.float32), r))
+        hs.destroy()
+        out[p] = round(float(np.mean(vals)), 3)
+    return out
+
+
 def cpu_baseline(frames_g, w, h, n_frames):
     """The oracle (FP32 mirror, C + OpenMP) timed on the host on the first `n_frames` frames."""
     from oracle.ho import Oracle
@@ -82,6 +131,7 @@ def main():
                     help="math policy: ieee (the oracle's operation-exact mirror) or fast")
     ap.add_argument("--cpu-frames", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the MLP-stage, FP16-MFMA frame and PSNR legs")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run device-to-device copies of this size (the runtime's copy kernel calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
@@ -155,6 +205,27 @@ def main():
                 "traffic_source": traffic_src, "algorithmic_bytes": ab["temporal"],
                 "algorithmic_bytes_rule": "80*N + 72*N2 (N = W*H, N2 = ceil(W/2)*ceil(H/2))"}
 
+    extras = {}
+    if not args.no_extras:
+        extras["mlp_stage_1080p"] = mlp_stage(hs, hip, W, H)
+        print(f"[bench] MLP stage: {extras['mlp_stage_1080p']}", file=sys.stderr, flush=True)
+        # the whole frame with the FP16-MFMA MLP
+        hm = hush_amd.Hush(0, {"width": W, "height": H, "precision": "fp16-mfma", "ieee_math": args.math == "ieee"})
+        for i in range(args.warmup):
+            hm.encode(dev[i % F], stream=stream.handle)
+        stream.synchronize()
+        e0.record(stream.handle)
+        for i in range(args.steps):
+            if i % F == 0:
+                hm.reset()
+            hm.encode(dev[i % F], stream=stream.handle)
+        e1.record(stream.handle)
+        e1.synchronize()
+        extras["fp16_mfma_ms_per_frame"] = round(e0.elapsed_ms(e1) / args.steps, 5)
+        hm.destroy()
+        extras["psnr_db_vs_fp64_oracle_640x360"] = psnr_check(["fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma"])
+        print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
+
     if args.copy_probe_mb:
         nb = args.copy_probe_mb << 20
         src, dst = hip.DeviceBuffer(nb), hip.DeviceBuffer(nb)
@@ -177,7 +248,7 @@ def main():
                    "width": W, "height": H, "frames": F, "mlp": args.precision, "math": args.math},
         "frame_GB/s_algorithmic": round(total_bytes / (ms_frame * 1e-3) / 1e9, 1),
         "frame_hbm_frac": round(total_bytes / (ms_frame * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "passes": passes, "roofline": roofline, "cpu_baseline": cpu,
+        "passes": passes, "roofline": roofline, "cpu_baseline": cpu, **extras,
     }
     print(json.dumps(line), flush=True)
     hs.destroy()

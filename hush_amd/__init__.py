@@ -250,6 +250,14 @@ class Hush:
         except Exception:
             pass
 
+    def mlp_forward(self, features, out, n: int, precision="fp32", stream: int | None = None) -> None:
+        """The MLP alone: `features` a device buffer of n x 20 binary16, `out` of n x 12 float32."""
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        fp = features.ptr if hasattr(features, "ptr") else int(features)
+        op = out.ptr if hasattr(out, "ptr") else int(out)
+        self._check(self._L.hush_mlp_forward(self._h, prec, ctypes.c_void_p(fp), ctypes.c_void_p(op), int(n),
+                                             ctypes.c_void_p(stream or 0)))
+
     # ---- parity hooks
     def surface(self, name) -> Surface:
         sid = SURFACES[name] if isinstance(name, str) else int(name)

@@ -77,7 +77,20 @@ static int hip_fail(hush_t *h, hipError_t e, const char *what) {
   return fail(h, HUSH_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
 }
 
-static bool precision_built(int p) { return p == HUSH_MLP_FP32 || p == HUSH_MLP_F16; }
+// the MFMA forms are built for the 16-wide hidden layers of the 20-16-16-12 network (the built-in size)
+static bool precision_built(int p, int hidden) {
+  return p == HUSH_MLP_FP32 || p == HUSH_MLP_F16 || ((p == HUSH_MLP_FP16_MFMA || p == HUSH_MLP_BF16_MFMA || p == HUSH_MLP_FP8_MFMA) && hidden == 16);
+}
+
+// smallest power of two s with max|w| / s <= 448 (the largest fp8 E4M3 value)
+static float fp8_scale(const float *w, int n) {
+  float m = 0.0f;
+  for (int i = 0; i < n; i++) m = std::fmax(m, std::fabs(w[i]));
+  if (!(m > 0.0f)) return 1.0f;
+  int e;
+  std::frexp(m / 448.0f, &e);
+  return std::ldexp(1.0f, e);
+}
 
 extern "C" int32_t hush_hidden_of(size_t n) {
   for (long h = 1; h <= 256; h++)
@@ -162,7 +175,8 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
   if (!hid) return fail(nullptr, HUSH_ERR_WEIGHTS, "hush weights: %zu f16 values fit no MLP size", count);
   if (!(width > 0 && height > 0)) return fail(nullptr, HUSH_ERR_SIZE, "Hush: bad size %dx%d", width, height);
   if (precision < HUSH_MLP_FP32 || precision > HUSH_MLP_FP8_MFMA) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_create: unknown precision %d", precision);
-  if (!precision_built(precision)) return fail(nullptr, HUSH_ERR_UNSUPPORTED, "hush_create: precision %d is not built", precision);
+  if (!precision_built(precision, hid))
+    return fail(nullptr, HUSH_ERR_UNSUPPORTED, "hush_create: precision %d needs hidden width 16 (this blob: %d)", precision, hid);
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return hip_fail(nullptr, e, "hipSetDevice");
   hush_t *h = new (std::nothrow) hush_ctx();
@@ -179,6 +193,18 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
     hip_fail(nullptr, e, "weights upload");
     hush_destroy(h);
     return HUSH_ERR_DEVICE;
+  }
+  if (hid == 16) {   // fragment-order weights of the MFMA forms
+    e = hipMalloc(&h->d_wq, sizeof(MfmaPack));
+    if (e == hipSuccess)
+      e = launch_pack_mlp(h->d_w, (MfmaPack *)h->d_wq, fp8_scale(wf.data(), 320), fp8_scale(wf.data() + 336, 256),
+                          fp8_scale(wf.data() + 608, 192), nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      hip_fail(nullptr, e, "MFMA weight packing");
+      hush_destroy(h);
+      return HUSH_ERR_DEVICE;
+    }
   }
   for (int i = 0; i < 9; i++) (void)hipEventCreate(&h->ev[i]);
   int r = hush_resize(h, width, height);
@@ -229,7 +255,8 @@ extern "C" int64_t hush_frame(const hush_t *h) { return h ? h->frame : -1; }
 extern "C" int32_t hush_set_precision(hush_t *h, int32_t p) {
   if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_precision: null context");
   if (p < HUSH_MLP_FP32 || p > HUSH_MLP_FP8_MFMA) return fail(h, HUSH_ERR_ARGUMENT, "hush_set_precision: unknown precision %d", p);
-  if (!precision_built(p)) return fail(h, HUSH_ERR_UNSUPPORTED, "hush_set_precision: precision %d is not built", p);
+  if (!precision_built(p, h->hidden))
+    return fail(h, HUSH_ERR_UNSUPPORTED, "hush_set_precision: precision %d needs hidden width 16 (this blob: %d)", p, h->hidden);
   h->precision = p;
   return HUSH_OK;
 }
@@ -373,6 +400,20 @@ extern "C" int32_t hush_denoise(hush_t *h, const hush_inputs *in, void *stream, 
 extern "C" int32_t hush_output(const hush_t *h, hush_surface *out) {
   if (!h || !out) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_output: null argument");
   *out = surf_of(h, S_TAA0 + (int)((h->frame + 1) & 1));
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_mlp_forward(hush_t *h, int32_t precision, const void *features, float *out, int64_t n, void *stream) {
+  if (!h || (n > 0 && (!features || !out))) return fail(h, HUSH_ERR_ARGUMENT, "hush_mlp_forward: null argument");
+  if (precision < HUSH_MLP_FP32 || precision > HUSH_MLP_FP8_MFMA) return fail(h, HUSH_ERR_ARGUMENT, "hush_mlp_forward: unknown precision %d", precision);
+  if (h->hidden != 16 || !precision_built(precision, h->hidden))
+    return fail(h, HUSH_ERR_UNSUPPORTED, "hush_mlp_forward: built for hidden width 16 (this blob: %d)", h->hidden);
+  if (n < 0 || n > (int64_t)1 << 30) return fail(h, HUSH_ERR_ARGUMENT, "hush_mlp_forward: bad row count %lld", (long long)n);
+  if (((uintptr_t)features & 3) != 0 || ((uintptr_t)out & 3) != 0) return fail(h, HUSH_ERR_ARGUMENT, "hush_mlp_forward: pointers must be 4-byte aligned");
+  if (n == 0) return HUSH_OK;
+  hipError_t e = hipSetDevice(h->device);
+  if (e == hipSuccess) e = launch_mlp(precision, (const uint32_t *)features, out, (int)n, h->d_w, h->d_wq, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(h, e, "hush_mlp_forward");
   return HUSH_OK;
 }
 

@@ -9,6 +9,15 @@ namespace hush {
 
 enum MlpMode { MLP_FP32 = 0, MLP_F16 = 1, MLP_FP16_MFMA = 2, MLP_BF16_MFMA = 3, MLP_FP8_MFMA = 4 };
 
+// weight fragments of the MFMA MLP forms in lane order (hidden width 16), packed on the device
+struct MfmaPack {
+  uint2 h[4][64];          // fp16: L1 k 0-15, L1 k 16-31, L2, L3 (4 x 16-bit per lane)
+  uint2 b[4][64];          // bf16, same order
+  uint2 q[3][64];          // fp8 E4M3: L1, L2, L3, k = 8(l/16) .. +7 (8 x 8-bit per lane)
+  float b1[16], b2[16], b3[16];
+  float sw[3];             // fp8 weight scale per layer (power of two)
+};
+
 struct TemporalArgs {
   Tex rad, nd, mv, mat;              // this frame's G-buffer
   TexF pkd;                          // history: decoded key of the previous frame (normal, (depth, id))
@@ -74,5 +83,7 @@ hipError_t launch_coarse1(const Coarse1Args &a, bool fast, hipStream_t s);
 hipError_t launch_coarse2(const AtrousArgs &a, bool fast, hipStream_t s);
 hipError_t launch_fine_final(const FineArgs &a, bool fast, hipStream_t s);
 hipError_t launch_taa(const TaaArgs &a, bool fast, hipStream_t s);
+hipError_t launch_pack_mlp(const float *w, MfmaPack *p, float s1, float s2, float s3, hipStream_t s);
+hipError_t launch_mlp(int mode, const uint32_t *feat, float *out, int n, const float *w, const void *wq, hipStream_t s);
 
 }  // namespace hush

@@ -515,6 +515,345 @@ __global__ __launch_bounds__(NT) void k_infer(InferArgs a) {
   infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
 }
 
+// ============================================================================ MFMA forms of the MLP
+// The MLP 20-16-16-12 of 64 texels per wave, in four groups of 16 texels.  Each layer is D = W . X^T on a
+// 16x16 MFMA tile: A = weight fragment (16 units x K), B = activations (K x 16 texels), D = 16 units x 16
+// texels with fp32 accumulation.  With the 16x16x16 f16 / bf16 shapes, lane l holds D[4(l/16)+i][l%16],
+// which is exactly the B fragment (rows k = 4(l/16)+i, column l%16) of the next layer: hidden activations
+// stay in registers.  The fp8 form uses the 16x16x32 shape (8 values per lane, k = 8(l/16)+i) and
+// regroups the hidden values with lane shuffles.  Weights are the hidden-width-16 blob only.
+
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t pack_h2(float a, float b) {       // binary16 pair, round to nearest even
+  half2v h = {(_Float16)a, (_Float16)b};
+  return __builtin_bit_cast(uint32_t, h);
+}
+__device__ __forceinline__ uint32_t bf16_rne(float x) {               // bfloat16 bits, round to nearest even
+  uint32_t u = __float_as_uint(x);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ uint32_t pack_bf2(float a, float b) { return bf16_rne(a) | (bf16_rne(b) << 16); }
+// four floats -> four fp8 (E4M3) bytes, saturated to +-448, round to nearest even
+__device__ __forceinline__ uint32_t pack_fp8x4(float a, float b, float c, float d) {
+  const float M = 448.0f;
+  int r = __builtin_amdgcn_cvt_pk_fp8_f32(fminf(fmaxf(a, -M), M), fminf(fmaxf(b, -M), M), 0, false);
+  r = __builtin_amdgcn_cvt_pk_fp8_f32(fminf(fmaxf(c, -M), M), fminf(fmaxf(d, -M), M), r, true);
+  return (uint32_t)r;
+}
+// smallest power of two s with max / s <= 448 (1 for max == 0)
+__device__ __forceinline__ float pow2_scale(float m) {
+  if (!(m > 0.0f)) return 1.0f;
+  int e;
+  frexpf(m / 448.0f, &e);
+  return ldexpf(1.0f, e);
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+
+// one lane per weight-fragment slot: converts the fp32-widened blob into the three fragment sets with the
+// conversion instructions the MFMAs read, so the formats agree by construction
+__global__ __launch_bounds__(64) void k_pack_mlp(const float *__restrict__ w, MfmaPack *p, float s1, float s2, float s3) {
+  const int l = threadIdx.x, m = l & 15, kb = l >> 4;
+  const float *W1 = w, *B1 = W1 + 320, *W2 = B1 + 16, *B2 = W2 + 256, *W3 = B2 + 16, *B3 = W3 + 192;
+  auto w1 = [&](int mm, int k) { return k < 20 ? W1[mm * 20 + k] : 0.0f; };
+  auto w2 = [&](int mm, int k) { return k < 16 ? W2[mm * 16 + k] : 0.0f; };
+  auto w3 = [&](int mm, int k) { return (mm < 12 && k < 16) ? W3[mm * 16 + k] : 0.0f; };
+  float v[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    v[0][i] = w1(m, 4 * kb + i); v[1][i] = w1(m, 16 + 4 * kb + i);
+    v[2][i] = w2(m, 4 * kb + i); v[3][i] = w3(m, 4 * kb + i);
+  }
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    p->h[b][l] = make_uint2(pack_h2(v[b][0], v[b][1]), pack_h2(v[b][2], v[b][3]));
+    p->b[b][l] = make_uint2(pack_bf2(v[b][0], v[b][1]), pack_bf2(v[b][2], v[b][3]));
+  }
+  const float r1 = 1.0f / s1, r2 = 1.0f / s2, r3 = 1.0f / s3;     // powers of two: exact
+  float q[3][8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    q[0][i] = w1(m, 8 * kb + i) * r1; q[1][i] = w2(m, 8 * kb + i) * r2; q[2][i] = w3(m, 8 * kb + i) * r3;
+  }
+#pragma unroll
+  for (int L = 0; L < 3; L++)
+    p->q[L][l] = make_uint2(pack_fp8x4(q[L][0], q[L][1], q[L][2], q[L][3]), pack_fp8x4(q[L][4], q[L][5], q[L][6], q[L][7]));
+  if (l < 16) { p->b1[l] = B1[l]; p->b2[l] = B2[l]; p->b3[l] = l < 12 ? B3[l] : 0.0f; }
+  if (l == 0) { p->sw[0] = s1; p->sw[1] = s2; p->sw[2] = s3; }
+}
+
+// bias + ReLU on a D fragment (units 4kb+i)
+__device__ __forceinline__ f32x4 bias_relu(f32x4 d, const float *b, int kb) {
+  f32x4 r;
+#pragma unroll
+  for (int i = 0; i < 4; i++) r[i] = mx(d[i] + b[4 * kb + i], 0.0f);
+  return r;
+}
+
+// The MLP of this wave's 64 texels.  u: this lane's features as 10 binary16 pairs (F0.xyzw, F1.xyzw,
+// F2.xy); o: the 12 outputs of this lane's texel.  sX (64 x 16 words) and sO (64 x 16 floats) are this
+// wave's LDS scratch; every lane of the workgroup must call this (it holds workgroup barriers).
+template <int MODE>
+__device__ void mlp_mfma(const MfmaPack *__restrict__ P, const uint32_t *u, float *o, uint32_t *sX, float *sO, int lane) {
+  const int kb = lane >> 4, tl = lane & 15;
+  if (MODE == MLP_FP16_MFMA) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) sX[lane * 16 + j] = j < 10 ? u[j] : 0u;
+  } else if (MODE == MLP_BF16_MFMA) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      float2 f = j < 10 ? unpack2(u[j]) : make_float2(0.0f, 0.0f);
+      sX[lane * 16 + j] = pack_bf2(f.x, f.y);
+    }
+  } else {                                             // fp8: keep the binary16 features, scale per group below
+#pragma unroll
+    for (int j = 0; j < 16; j++) sX[lane * 16 + j] = j < 10 ? u[j] : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const int t = 16 * g + tl;                        // texel (row of this wave) of this lane's B column
+    f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
+      const uint2 x0 = *(const uint2 *)&sX[t * 16 + 2 * kb];        // k = 4kb .. 4kb+3
+      const uint2 x1 = *(const uint2 *)&sX[t * 16 + 8 + 2 * kb];    // k = 16 + 4kb ..
+      const uint2 (*Wf)[64] = MODE == MLP_FP16_MFMA ? P->h : P->b;
+      if (MODE == MLP_FP16_MFMA) {
+        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, Wf[0][lane]), __builtin_bit_cast(f16x4, x0), d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, Wf[1][lane]), __builtin_bit_cast(f16x4, x1), d, 0, 0, 0);
+        f32x4 h = bias_relu(d, P->b1, kb);
+        uint2 hb = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, Wf[2][lane]), __builtin_bit_cast(f16x4, hb), d, 0, 0, 0);
+        h = bias_relu(d, P->b2, kb);
+        hb = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, Wf[3][lane]), __builtin_bit_cast(f16x4, hb), d, 0, 0, 0);
+      } else {
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, Wf[0][lane]), __builtin_bit_cast(s16x4, x0), d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, Wf[1][lane]), __builtin_bit_cast(s16x4, x1), d, 0, 0, 0);
+        f32x4 h = bias_relu(d, P->b1, kb);
+        uint2 hb = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, Wf[2][lane]), __builtin_bit_cast(s16x4, hb), d, 0, 0, 0);
+        h = bias_relu(d, P->b2, kb);
+        hb = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, Wf[3][lane]), __builtin_bit_cast(s16x4, hb), d, 0, 0, 0);
+      }
+    } else {
+      // fp8, 16x16x32: lane l holds k = 8(l/16) .. +7 of texel l%16; power-of-two scale per layer and group
+      float xv[8];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float2 f = unpack2(sX[t * 16 + 4 * kb + i]);
+        xv[2 * i] = f.x; xv[2 * i + 1] = f.y;
+      }
+      float am = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 8; i++) am = fmaxf(am, fabsf(xv[i]));
+      float sx = pow2_scale(wave_max(am)), rx = 1.0f / sx;
+      uint2 xb = make_uint2(pack_fp8x4(xv[0] * rx, xv[1] * rx, xv[2] * rx, xv[3] * rx),
+                            pack_fp8x4(xv[4] * rx, xv[5] * rx, xv[6] * rx, xv[7] * rx));
+      d = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[0][lane]),
+                                                     (long)__builtin_bit_cast(uint64_t, xb), d, 0, 0, 0);
+#pragma unroll
+      for (int L = 1; L < 3; L++) {
+        const float scale = sx * P->sw[L - 1];
+        f32x4 h;
+#pragma unroll
+        for (int i = 0; i < 4; i++) h[i] = mx(d[i] * scale + (L == 1 ? P->b1 : P->b2)[4 * kb + i], 0.0f);
+        // regroup: this lane (kb' = kb < 2) needs units 8kb' .. 8kb'+7 of texel tl, held by lanes
+        // tl + 32kb' (units 8kb' .. +3) and tl + 32kb' + 16 (units 8kb'+4 .. +7)
+        float hv[8];
+        const int src = tl + 32 * (kb & 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) { hv[i] = __shfl(h[i], src); hv[4 + i] = __shfl(h[i], src + 16); }
+        am = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) am = fmaxf(am, h[i]);
+        sx = pow2_scale(wave_max(am));
+        rx = 1.0f / sx;
+        const bool live = kb < 2;                     // k >= 16 are zero for the 16-wide layers
+#pragma unroll
+        for (int i = 0; i < 8; i++) hv[i] = live ? hv[i] * rx : 0.0f;
+        xb = make_uint2(pack_fp8x4(hv[0], hv[1], hv[2], hv[3]), pack_fp8x4(hv[4], hv[5], hv[6], hv[7]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[L][lane]),
+                                                       (long)__builtin_bit_cast(uint64_t, xb), d, 0, 0, 0);
+      }
+      const float scale = sx * P->sw[2];
+#pragma unroll
+      for (int i = 0; i < 4; i++) d[i] = d[i] * scale;
+    }
+    if (kb < 3) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) sO[t * 16 + 4 * kb + i] = d[i] + P->b3[4 * kb + i];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 12; j++) o[j] = sO[lane * 16 + j];
+}
+
+template <int MODE, bool FAST>
+__global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
+  constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
+  __shared__ float4 sA[SN], sG[SN];
+  __shared__ uint32_t sX[4][64 * 16];
+  __shared__ float sO[4][64 * 16];
+  const int tid = threadIdx.x;
+  const int2 org = tile_origin<TW, TH>(a.W2, a.H2);
+  const int x0 = org.x, y0 = org.y;
+  for (int i = tid; i < SN; i += NT) {
+    int qx = clampi(x0 - AP + i % SW, 0, a.W2 - 1), qy = clampi(y0 - AP + i / SW, 0, a.H2 - 1);
+    uint4 g2 = ldu(a.F2, qx, qy);
+    float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
+    sA[i] = make_float4(a01.x, a01.y, a23.x, a23.y);
+    sG[i] = guide_gh(ldf(a.GH, qx, qy));
+  }
+  const int lx = tid & 31, ly = tid >> 5;
+  const int X = x0 + lx, Y = y0 + ly;
+  const bool in_img = inside(X, Y, a.W2, a.H2);
+  uint32_t u[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (in_img) {
+    const uint4 f0 = ldu(a.F0, X, Y), f1 = ldu(a.F1, X, Y), f2 = ldu(a.F2, X, Y);
+    u[0] = f0.x; u[1] = f0.y; u[2] = f0.z; u[3] = f0.w; u[4] = f1.x; u[5] = f1.y; u[6] = f1.z; u[7] = f1.w;
+    u[8] = f2.x; u[9] = f2.y;
+  }
+  float o[12];
+  const int wv = tid >> 6;
+  mlp_mfma<MODE>((const MfmaPack *)a.wq, u, o, sX[wv], sO[wv], tid & 63);   // holds the tile barrier too
+  if (!in_img) return;
+  const int tx = lx + AP, ty = ly + AP;
+  if (sG[ty * SW + tx].w < 0.0f) {
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    st(a.out0, X, Y, z); st(a.out1, X, Y, z); st(a.out2, X, Y, z); st(a.outMid, X, Y, z);
+    return;
+  }
+  infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
+}
+
+// The MLP alone over n feature rows (20 binary16 each, row-major, 40 B) -> n x 12 fp32 outputs.  Used to
+// time the MLP stage in isolation and by training tools.  The MFMA forms read their B fragments and write
+// their D fragments straight from / to global memory (no LDS): lane l of group g serves row
+// 64w + 16g + l%16, k block l/16, and writes outputs 4(l/16) .. +3 of that row as one 16-byte store.
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_mlp(const uint32_t *__restrict__ feat, float *__restrict__ out, int n,
+                                             const float *__restrict__ w, const void *__restrict__ wq) {
+  const int tid = threadIdx.x;
+  if (MODE == MLP_FP32 || MODE == MLP_F16) {
+    const long r = (long)blockIdx.x * NT + tid;
+    if (r >= n) return;
+    float f[20], o[12];
+    const uint2 *row = (const uint2 *)(feat + r * 10);
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      const uint2 v = row[j];
+      const float2 a = unpack2(v.x), b = unpack2(v.y);
+      f[4 * j] = a.x; f[4 * j + 1] = a.y; f[4 * j + 2] = b.x; f[4 * j + 3] = b.y;
+    }
+    if (MODE == MLP_FP32) mlp_fp32<16>(w, 16, f, o);
+    else mlp_f16(w, 16, f, o);
+    float4 *dst = (float4 *)(out + r * 12);
+    dst[0] = make_float4(o[0], o[1], o[2], o[3]);
+    dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+    dst[2] = make_float4(o[8], o[9], o[10], o[11]);
+    return;
+  }
+  const MfmaPack *__restrict__ P = (const MfmaPack *)wq;
+  const int lane = tid & 63, kb = lane >> 4, tl = lane & 15;
+  const long base = ((long)blockIdx.x * NT + (tid & ~63));          // first row of this wave
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const long r = base + 16 * g + tl;
+    const bool live = r < n;
+    const uint32_t *row = feat + (live ? r : 0) * 10;
+    f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
+      uint2 x0 = live ? *(const uint2 *)(row + 2 * kb) : make_uint2(0, 0);                       // k = 4kb ..
+      uint2 x1 = (live && kb == 0) ? *(const uint2 *)(row + 8) : make_uint2(0, 0);              // k = 16 .. 19
+      if (MODE == MLP_FP16_MFMA) {
+        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, P->h[0][lane]), __builtin_bit_cast(f16x4, x0), d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, P->h[1][lane]), __builtin_bit_cast(f16x4, x1), d, 0, 0, 0);
+        f32x4 h = bias_relu(d, P->b1, kb);
+        uint2 hb = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, P->h[2][lane]), __builtin_bit_cast(f16x4, hb), d, 0, 0, 0);
+        h = bias_relu(d, P->b2, kb);
+        hb = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, P->h[3][lane]), __builtin_bit_cast(f16x4, hb), d, 0, 0, 0);
+      } else {
+        const float2 a0 = unpack2(x0.x), a1 = unpack2(x0.y), c0 = unpack2(x1.x), c1 = unpack2(x1.y);
+        x0 = make_uint2(pack_bf2(a0.x, a0.y), pack_bf2(a1.x, a1.y));
+        x1 = make_uint2(pack_bf2(c0.x, c0.y), pack_bf2(c1.x, c1.y));
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, P->b[0][lane]), __builtin_bit_cast(s16x4, x0), d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, P->b[1][lane]), __builtin_bit_cast(s16x4, x1), d, 0, 0, 0);
+        f32x4 h = bias_relu(d, P->b1, kb);
+        uint2 hb = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, P->b[2][lane]), __builtin_bit_cast(s16x4, hb), d, 0, 0, 0);
+        h = bias_relu(d, P->b2, kb);
+        hb = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, P->b[3][lane]), __builtin_bit_cast(s16x4, hb), d, 0, 0, 0);
+      }
+    } else {
+      // fp8, 16x16x32: lane holds k = 8kb .. 8kb+7 (words 4kb .. 4kb+3; k >= 20 zero)
+      float xv[8];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int wd = 4 * kb + i;
+        const float2 f = (live && wd < 10) ? unpack2(row[wd]) : make_float2(0.0f, 0.0f);
+        xv[2 * i] = f.x; xv[2 * i + 1] = f.y;
+      }
+      float am = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 8; i++) am = fmaxf(am, fabsf(xv[i]));
+      float sx = pow2_scale(wave_max(am)), rx = 1.0f / sx;
+      uint2 xb = make_uint2(pack_fp8x4(xv[0] * rx, xv[1] * rx, xv[2] * rx, xv[3] * rx),
+                            pack_fp8x4(xv[4] * rx, xv[5] * rx, xv[6] * rx, xv[7] * rx));
+      d = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[0][lane]),
+                                                     (long)__builtin_bit_cast(uint64_t, xb), d, 0, 0, 0);
+#pragma unroll
+      for (int L = 1; L < 3; L++) {
+        const float scale = sx * P->sw[L - 1];
+        f32x4 h;
+#pragma unroll
+        for (int i = 0; i < 4; i++) h[i] = mx(d[i] * scale + (L == 1 ? P->b1 : P->b2)[4 * kb + i], 0.0f);
+        float hv[8];
+        const int src = tl + 32 * (kb & 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) { hv[i] = __shfl(h[i], src); hv[4 + i] = __shfl(h[i], src + 16); }
+        am = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) am = fmaxf(am, h[i]);
+        sx = pow2_scale(wave_max(am));
+        rx = 1.0f / sx;
+#pragma unroll
+        for (int i = 0; i < 8; i++) hv[i] = kb < 2 ? hv[i] * rx : 0.0f;
+        xb = make_uint2(pack_fp8x4(hv[0], hv[1], hv[2], hv[3]), pack_fp8x4(hv[4], hv[5], hv[6], hv[7]));
+        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[L][lane]),
+                                                       (long)__builtin_bit_cast(uint64_t, xb), d, 0, 0, 0);
+      }
+      const float scale = sx * P->sw[2];
+#pragma unroll
+      for (int i = 0; i < 4; i++) d[i] = d[i] * scale;
+    }
+    if (live && kb < 3)
+      *(float4 *)(out + r * 12 + 4 * kb) = make_float4(d[0] + P->b3[4 * kb], d[1] + P->b3[4 * kb + 1],
+                                                       d[2] + P->b3[4 * kb + 2], d[3] + P->b3[4 * kb + 3]);
+  }
+}
+
 // ============================================================================ passes 3, 4, 6: a-trous
 // atrousShader (hush.js:482-525): mid2 (R1 step 2) and mid3 (R1 step 4) at half res with the F3 guide
 // and learned s/c from par1/par2; coarse2 (R2 step 2, prefilter) at quarter res with the CG guide.
@@ -810,6 +1149,9 @@ static dim3 grid_for(int w, int h) { return dim3(((w + TW - 1) / TW) * ((h + TH 
 #define K_INFER16(F) k_infer<16, 0, F>
 #define K_INFER(F) k_infer<0, 0, F>
 #define K_INFER_F16(F) k_infer<0, 1, F>
+#define K_INFER_H(F) k_infer_mfma<MLP_FP16_MFMA, F>
+#define K_INFER_B(F) k_infer_mfma<MLP_BF16_MFMA, F>
+#define K_INFER_Q(F) k_infer_mfma<MLP_FP8_MFMA, F>
 #define K_MID2(F) k_atrous<1, 2, false, true, 2, F>
 #define K_MID3(F) k_atrous<1, 4, false, true, 3, F>
 #define K_COARSE2(F) k_atrous<2, 2, true, false, -1, F>
@@ -829,8 +1171,34 @@ hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s) 
     else HUSH_LAUNCH(K_INFER, g, a);
   } else if (mode == MLP_F16) {
     HUSH_LAUNCH(K_INFER_F16, g, a);
+  } else if (a.hidden != 16 || !a.wq) {
+    return hipErrorInvalidValue;
+  } else if (mode == MLP_FP16_MFMA) {
+    HUSH_LAUNCH(K_INFER_H, g, a);
+  } else if (mode == MLP_BF16_MFMA) {
+    HUSH_LAUNCH(K_INFER_B, g, a);
+  } else if (mode == MLP_FP8_MFMA) {
+    HUSH_LAUNCH(K_INFER_Q, g, a);
   } else {
     return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_mlp(const float *w, MfmaPack *p, float s1, float s2, float s3, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, s, w, p, s1, s2, s3);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp(int mode, const uint32_t *feat, float *out, int n, const float *w, const void *wq, hipStream_t s) {
+  dim3 g((n + NT - 1) / NT);
+  switch (mode) {
+    case MLP_FP32: hipLaunchKernelGGL(k_mlp<MLP_FP32>, g, dim3(NT), 0, s, feat, out, n, w, wq); break;
+    case MLP_F16: hipLaunchKernelGGL(k_mlp<MLP_F16>, g, dim3(NT), 0, s, feat, out, n, w, wq); break;
+    case MLP_FP16_MFMA: hipLaunchKernelGGL(k_mlp<MLP_FP16_MFMA>, g, dim3(NT), 0, s, feat, out, n, w, wq); break;
+    case MLP_BF16_MFMA: hipLaunchKernelGGL(k_mlp<MLP_BF16_MFMA>, g, dim3(NT), 0, s, feat, out, n, w, wq); break;
+    case MLP_FP8_MFMA: hipLaunchKernelGGL(k_mlp<MLP_FP8_MFMA>, g, dim3(NT), 0, s, feat, out, n, w, wq); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

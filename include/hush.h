@@ -46,10 +46,14 @@ enum hush_status {
 enum hush_precision {
   HUSH_MLP_FP32 = 0,          /* plain FP32 MLP: the drop-in (hush.js:683-704 with f16 = false) */
   HUSH_MLP_F16 = 1,           /* binary16 arithmetic, like shader-f16 (hush.js:690, 699-702) */
-  HUSH_MLP_FP16_MFMA = 2,     /* fp16 operands on MFMA, fp32 accumulation */
+  HUSH_MLP_FP16_MFMA = 2,     /* fp16 operands on MFMA (v_mfma_f32_16x16x16_f16), fp32 accumulation */
   HUSH_MLP_BF16_MFMA = 3,     /* bf16 operands on MFMA, fp32 accumulation */
-  HUSH_MLP_FP8_MFMA = 4       /* fp8 E4M3 operands with per-layer scales on MFMA, fp32 accumulation */
+  HUSH_MLP_FP8_MFMA = 4       /* fp8 E4M3 operands on MFMA (16x16x32), fp32 accumulation; power-of-two
+                                 scale per layer for the weights and per layer and 16-texel tile for the
+                                 activations */
 };
+/* The MFMA forms need hidden width 16 (the built-in 20-16-16-12 network); other widths report
+   HUSH_ERR_UNSUPPORTED for them. */
 
 enum hush_format { HUSH_RGBA16F = 0, HUSH_RGBA32U = 1, HUSH_R8U = 2, HUSH_RGBA32F = 3 };
 
@@ -115,6 +119,11 @@ int32_t hush_encode(hush_t *h, const hush_inputs *in, void *stream, hush_surface
 
 /* the `output` getter (hush.js:798): the last frame's output */
 int32_t hush_output(const hush_t *h, hush_surface *out);
+
+/* The MLP alone (no counterpart in the reference): n rows of 20 binary16 features (row-major, 40 bytes a
+   row, 4-byte aligned device memory) -> n x 12 fp32 outputs (pre-activation), with the given precision.
+   For timing the MLP stage in isolation and for training tools. Hidden width 16 only. */
+int32_t hush_mlp_forward(hush_t *h, int32_t precision, const void *features, float *out, int64_t n, void *stream);
 
 /* last error message of `h`, or of the calling thread's last failed hush_create when h is NULL */
 const char *hush_last_error(const hush_t *h);

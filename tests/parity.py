@@ -61,3 +61,16 @@ def compare(gpu: np.ndarray, ref: np.ndarray) -> dict:
     return {"n": int(d.size), "exact": float(np.mean(d == 0)), "max_ulp": float(ulps.max(initial=0)),
             "p9999_ulp": float(np.quantile(ulps, 0.9999)) if d.size else 0.0, "max_rel": float(rel.max(initial=0)),
             "frac_rel_gt_1e-3": float(np.mean(rel > 1e-3))}
+
+
+def tonemap(c: np.ndarray) -> np.ndarray:
+    """c / (1 + lum(c)) (the TAA tone map, hush.js:635): maps HDR into [0, 1) for PSNR"""
+    c = np.maximum(c[..., :3].astype(np.float64), 0.0)
+    l = c[..., 0] * 0.2126 + c[..., 1] * 0.7152 + c[..., 2] * 0.0722
+    return c / (1.0 + l)[..., None]
+
+
+def psnr(a: np.ndarray, ref: np.ndarray) -> float:
+    """PSNR (dB) of the tone-mapped images, peak 1"""
+    mse = float(np.mean((tonemap(a) - tonemap(ref)) ** 2))
+    return float("inf") if mse == 0 else 10.0 * np.log10(1.0 / mse)

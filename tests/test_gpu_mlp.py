@@ -1,0 +1,153 @@
+"""The MLP forms on the GPU through hush_mlp_forward, against numpy models of each precision on real feature
+vectors, and whole-pipeline PSNR of each MLP precision against the FP64 oracle.
+
+Tolerances:
+  fp32         max |gpu - fp64 model| <= 1e-4 (1 + |ref|) (fused fp32 multiply-adds)
+  f16          equal to the oracle's binary16-arithmetic model within 1 binary16 ulp
+  fp16-mfma    model: fp16 inputs and weights (exact), fp32 sums, fp16 hidden activations: <= 2e-3 (1 + |ref|)
+  bf16-mfma    model with bf16 inputs, weights and hidden activations: <= 2e-2 (1 + |ref|)
+  fp8-mfma     against fp32: mean |err| <= 0.3 (E4M3 keeps 3 mantissa bits), correlation >= 0.99
+  pipeline     fp16-mfma within 0.05 dB of the fp32 path's PSNR vs the FP64 oracle (tone-mapped, peak 1;
+               the FP32 path itself sits at ~36 dB of the FP64-ideal oracle because of the binary16
+               round-toward-zero storage of every surface)
+"""
+import numpy as np
+import pytest
+
+import hush_amd
+from hush_amd.hip import DeviceBuffer, lib, check, hipMemcpyHostToDevice, hipMemcpyDeviceToHost
+from oracle.ho import Oracle
+from tests.parity import psnr, upload_frame
+
+pytestmark = pytest.mark.gpu
+
+W16 = hush_amd.default_weights()
+WF = W16.view(np.float16).astype(np.float64)
+W1, B1 = WF[:320].reshape(16, 20), WF[320:336]
+W2, B2 = WF[336:592].reshape(16, 16), WF[592:608]
+W3, B3 = WF[608:800].reshape(12, 16), WF[800:812]
+
+
+def real_features(seed=3):
+    """feature rows of a real scene: F0, F1 and F2.xy of every non-sky block after 3 frames"""
+    from hush_amd.scene import SceneGenerator
+    w, h = 320, 180
+    gen = SceneGenerator("boxes", w, h, seed)
+    hs = hush_amd.Hush(0, {"width": w, "height": h})
+    surfs = None
+    for f in range(3):
+        surfs = upload_frame(gen.render(f), surfs)
+        hs.denoise(surfs)
+    F = np.concatenate([hs.surface(f"F{k}").download().view(np.float16) for k in range(3)], -1)
+    Z = hs.surface("F3").download().view(np.float16)[..., 2]
+    hs.destroy()
+    rows = F.reshape(-1, 24)[:, :20][Z.reshape(-1) >= 0]
+    return np.ascontiguousarray(rows)
+
+
+def run_mlp(hs, feats16, precision):
+    n = feats16.shape[0]
+    fin, fout = DeviceBuffer(feats16.nbytes), DeviceBuffer(n * 12 * 4)
+    check(lib().hipMemcpy(fin.ptr, feats16.ctypes.data, feats16.nbytes, hipMemcpyHostToDevice), "h2d")
+    hs.mlp_forward(fin, fout, n, precision)
+    out = np.empty((n, 12), np.float32)
+    check(lib().hipMemcpy(out.ctypes.data, fout.ptr, out.nbytes, hipMemcpyDeviceToHost), "d2h")
+    return out
+
+
+def bf16(x):
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def model(x, q_in, q_w, q_h):
+    a = np.maximum(q_in(x) @ q_w(W1).T + B1, 0.0)
+    a = np.maximum(q_h(a) @ q_w(W2).T + B2, 0.0)
+    return q_h(a) @ q_w(W3).T + B3
+
+
+@pytest.fixture(scope="module")
+def feats():
+    f = real_features()
+    assert f.shape[0] > 5000
+    return f
+
+
+@pytest.fixture(scope="module")
+def hs():
+    h = hush_amd.Hush(0, {"width": 64, "height": 64})
+    yield h
+    h.destroy()
+
+
+def test_mlp_fp32(hs, feats):
+    x = feats.astype(np.float64)
+    ref = model(x, lambda v: v, lambda v: v, lambda v: v)
+    got = run_mlp(hs, feats, "fp32")
+    assert np.all(np.abs(got - ref) <= 1e-4 * (1 + np.abs(ref)))
+
+
+def test_mlp_f16_matches_oracle(hs, feats):
+    got = run_mlp(hs, feats[:3000], "f16")
+    o = Oracle(8, 8, W16, f16=True)
+    ref = np.stack([o.mlp(r) for r in feats[:3000].astype(np.float32)])
+    ulp = np.spacing(np.abs(ref).astype(np.float16)).astype(np.float64)
+    assert np.all(np.abs(got - ref) <= ulp + 1e-7)
+
+
+def test_mlp_fp16_mfma(hs, feats):
+    x = feats.astype(np.float64)
+    f16 = lambda v: np.asarray(v, np.float16).astype(np.float64)
+    ref = model(x, f16, f16, f16)
+    got = run_mlp(hs, feats, "fp16-mfma")
+    err = np.abs(got - ref) / (1 + np.abs(ref))
+    assert err.max() <= 2e-3, err.max()
+
+
+def test_mlp_bf16_mfma(hs, feats):
+    x = feats.astype(np.float64)
+    ref = model(x, bf16, bf16, bf16)
+    got = run_mlp(hs, feats, "bf16-mfma")
+    err = np.abs(got - ref) / (1 + np.abs(ref))
+    assert err.max() <= 2e-2, err.max()
+
+
+def test_mlp_fp8_mfma(hs, feats):
+    x = feats.astype(np.float64)
+    ref = model(x, lambda v: v, lambda v: v, lambda v: v)
+    got = run_mlp(hs, feats, "fp8-mfma")
+    assert np.mean(np.abs(got - ref)) <= 0.3
+    assert np.corrcoef(got.ravel(), ref.ravel())[0, 1] >= 0.99
+
+
+def test_mlp_tail_rows(hs, feats):
+    """row counts that are not a multiple of the 256-row workgroup: the tail is computed, nothing past it"""
+    for n in (1, 63, 257):
+        for p in ("fp32", "fp16-mfma", "fp8-mfma"):
+            a = run_mlp(hs, feats[:n], p)
+            b = run_mlp(hs, feats[:512], p)[:n]
+            assert np.array_equal(a, b), (n, p)
+
+
+def test_pipeline_psnr_per_precision():
+    """whole pipeline, 8 frames at 320x180: PSNR of each MLP precision against the FP64 oracle"""
+    from hush_amd.scene import SceneGenerator
+    w, h, frames = 320, 180, 8
+    gen = SceneGenerator("boxes", w, h, 21)
+    gs = [gen.render(f) for f in range(frames)]
+    o = Oracle(w, h, W16, mode="fp64")
+    refs = [o.denoise(g) for g in gs]
+    res = {}
+    for p in ("fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma"):
+        hs = hush_amd.Hush(0, {"width": w, "height": h, "precision": p})
+        surfs = None
+        vals = []
+        for g, r in zip(gs, refs):
+            surfs = upload_frame(g, surfs)
+            vals.append(psnr(hs.denoise(surfs).download().astype(np.float32), r))
+        hs.destroy()
+        res[p] = float(np.mean(vals))
+    print("PSNR vs FP64 oracle (dB):", res)
+    assert res["fp32"] > 30
+    assert abs(res["fp16-mfma"] - res["fp32"]) <= 0.05, res
