@@ -101,6 +101,47 @@ def psnr_check(precisions, w=640, h=360, frames=8):
     return out
 
 
+def svgf_leg(dev, stream, hip, W, H, steps, warmup):
+    """the SVGF baseline (src/denoise/svgf.js) over the same resident frames: ms/frame and per-pass ms"""
+    import hush_amd
+    F = len(dev)
+    sv = hush_amd.SVGF(0, W, H)
+    ins = []
+    for f in range(F):
+        g = dict(dev[f])
+        p = dev[f - 1] if f > 0 else dev[f]
+        g["prevNormalDepth"], g["prevMotion"] = p["normalDepth"], p["motion"]
+        ins.append(g)
+
+    def run(n):
+        for i in range(n):
+            if i % F == 0:
+                sv.reset()
+            sv.encode(ins[i % F], stream=stream.handle)
+
+    run(warmup)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream.handle)
+    run(steps)
+    e1.record(stream.handle)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    timer = hush_amd.SVGFTimer()
+    acc, n_t = None, min(steps, 32)
+    for i in range(n_t):
+        if i % F == 0:
+            sv.reset()
+        sv.encode(ins[i % F], timer=timer, stream=stream.handle)
+        r = timer.read()
+        v = np.array([m for _, m in r])
+        acc = v if acc is None else acc + v
+    labels = [l for l, _ in r]
+    sv.destroy()
+    passes = [[l, round(float(m) / n_t, 5)] for l, m in zip(labels, acc)]
+    return {"ms_per_frame": round(ms, 5), "passes": passes, "iterations": 5}
+
+
 def cpu_baseline(frames_g, w, h, n_frames):
     """The oracle (FP32 mirror, C + OpenMP) timed on the host on the first `n_frames` frames."""
     from oracle.ho import Oracle
@@ -131,7 +172,7 @@ def main():
                     help="math policy: ieee (the oracle's operation-exact mirror) or fast")
     ap.add_argument("--cpu-frames", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the MLP-stage, FP16-MFMA frame and PSNR legs")
+    ap.add_argument("--no-extras", action="store_true", help="skip the MLP-stage, FP16-MFMA frame, SVGF and PSNR legs")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run device-to-device copies of this size (the runtime's copy kernel calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
@@ -223,6 +264,7 @@ def main():
         e1.synchronize()
         extras["fp16_mfma_ms_per_frame"] = round(e0.elapsed_ms(e1) / args.steps, 5)
         hm.destroy()
+        extras["svgf_1080p"] = svgf_leg(dev, stream, hip, W, H, min(args.steps, 64), args.warmup)
         extras["psnr_db_vs_fp64_oracle_640x360"] = psnr_check(["fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma"])
         print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
 

@@ -282,5 +282,7 @@ class Hush:
         self._check(self._L.hush_set_debug_mask(self._h, int(bool(on))))
 
 
+from .svgf import SVGF, SVGFError, SVGFTimer, SVGF_INPUTS  # noqa: E402
+
 __all__ = ["Hush", "HushError", "HushTimer", "hiddenOf", "hidden_of", "default_weights", "decode_weights", "INPUTS",
-           "PASS_LABELS", "N_FEAT", "N_OUT", "Surface"]
+           "PASS_LABELS", "N_FEAT", "N_OUT", "Surface", "SVGF", "SVGFError", "SVGFTimer", "SVGF_INPUTS"]

@@ -1,4 +1,4 @@
-"""ctypes declarations of the C ABI in include/hush.h (libhush.so, built in-tree by hush_amd/build.py).
+"""ctypes declarations of the C ABI in include/hush.h and include/svgf.h (libhush.so, built in-tree by hush_amd/build.py).
 
 Loading fails loudly when the library is missing: there is no CPU fallback for the denoiser.
 """
@@ -18,6 +18,10 @@ EXPORTS = [
     "hush_set_ieee_math", "hush_get_ieee_math", "hush_mlp_forward",
     "hush_denoise", "hush_encode", "hush_output", "hush_last_error", "hush_set_timing", "hush_pass_times",
     "hush_debug_surface", "hush_debug_set_state", "hush_debug_run_pass", "hush_set_debug_mask",
+    # include/svgf.h
+    "svgf_default_options", "svgf_create", "svgf_destroy", "svgf_resize", "svgf_reset", "svgf_get_size", "svgf_frame",
+    "svgf_get_options", "svgf_encode", "svgf_output", "svgf_last_error", "svgf_set_timing", "svgf_pass_times",
+    "svgf_debug_surface", "svgf_debug_set_state", "svgf_debug_run_pass",
 ]
 
 
@@ -29,6 +33,18 @@ class hush_surface(ctypes.Structure):
 class hush_inputs(ctypes.Structure):
     _fields_ = [("radiance", hush_surface), ("normal_depth", hush_surface), ("motion", hush_surface),
                 ("material", hush_surface), ("emission", hush_surface)]
+
+
+class svgf_options(ctypes.Structure):
+    _fields_ = [("sigma_l", ctypes.c_float), ("sigma_n", ctypes.c_float), ("sigma_z", ctypes.c_float),
+                ("alpha_color", ctypes.c_float), ("alpha_moments", ctypes.c_float), ("taa_alpha", ctypes.c_float),
+                ("taa", ctypes.c_int32), ("iterations", ctypes.c_int32)]
+
+
+class svgf_inputs(ctypes.Structure):
+    _fields_ = [("radiance", hush_surface), ("normal_depth", hush_surface), ("motion", hush_surface),
+                ("material", hush_surface), ("emission", hush_surface), ("prev_normal_depth", hush_surface),
+                ("prev_motion", hush_surface)]
 
 
 _lib = None
@@ -70,5 +86,24 @@ def load() -> ctypes.CDLL:
     L.hush_debug_set_state.argtypes = [P, ctypes.c_int64, i32]; L.hush_debug_set_state.restype = i32
     L.hush_debug_run_pass.argtypes = [P, i32, I, P]; L.hush_debug_run_pass.restype = i32
     L.hush_set_debug_mask.argtypes = [P, i32]; L.hush_set_debug_mask.restype = i32
+    O = ctypes.POINTER(svgf_options)
+    SI = ctypes.POINTER(svgf_inputs)
+    L.svgf_default_options.argtypes = [O]; L.svgf_default_options.restype = None
+    L.svgf_create.argtypes = [i32, i32, i32, O, ctypes.POINTER(P)]; L.svgf_create.restype = i32
+    L.svgf_destroy.argtypes = [P]; L.svgf_destroy.restype = None
+    L.svgf_resize.argtypes = [P, i32, i32]; L.svgf_resize.restype = i32
+    L.svgf_reset.argtypes = [P]; L.svgf_reset.restype = i32
+    L.svgf_get_size.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]; L.svgf_get_size.restype = i32
+    L.svgf_frame.argtypes = [P]; L.svgf_frame.restype = ctypes.c_int64
+    L.svgf_get_options.argtypes = [P, O]; L.svgf_get_options.restype = i32
+    L.svgf_encode.argtypes = [P, SI, P, S]; L.svgf_encode.restype = i32
+    L.svgf_output.argtypes = [P, S]; L.svgf_output.restype = i32
+    L.svgf_last_error.argtypes = [P]; L.svgf_last_error.restype = ctypes.c_char_p
+    L.svgf_set_timing.argtypes = [P, i32]; L.svgf_set_timing.restype = i32
+    L.svgf_pass_times.argtypes = [P, ctypes.POINTER(f32), ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(i32)]
+    L.svgf_pass_times.restype = i32
+    L.svgf_debug_surface.argtypes = [P, i32, S]; L.svgf_debug_surface.restype = i32
+    L.svgf_debug_set_state.argtypes = [P, ctypes.c_int64, i32]; L.svgf_debug_set_state.restype = i32
+    L.svgf_debug_run_pass.argtypes = [P, i32, SI, P]; L.svgf_debug_run_pass.restype = i32
     _lib = L
     return L

@@ -186,3 +186,78 @@ class Oracle:
 
     def hidden_of(self, n) -> int:
         return int(self._f["hidden_of"](int(n)))
+
+
+SVGF_DEFAULTS = dict(sigmaL=4.0, sigmaN=128.0, sigmaZ=1.0, alphaColor=0.2, alphaMoments=0.2, taaAlpha=0.1, taa=True,
+                     iterations=5)
+SVGF_SURF = {"histColor0": 0, "histColor1": 1, "histMoments0": 2, "histMoments1": 3, "tmpColor": 4, "atrous0": 5,
+             "atrous1": 6, "taa0": 7, "taa1": 8}
+
+
+class SvgfOracle:
+    """CPU restatement of the SVGF class (src/denoise/svgf.js:273-379)."""
+
+    def __init__(self, width, height, mode="fp32", **opts):
+        L = load()
+        pre, self._ct, self._np = MODES[mode]
+        o = dict(SVGF_DEFAULTS, **opts)
+        f = lambda n: getattr(L, pre + n)
+        self._create, self._destroy = f("svgf_create"), f("svgf_destroy")
+        self._create.restype = ctypes.c_void_p
+        self._create.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 6 + [ctypes.c_int, ctypes.c_int]
+        self._destroy.argtypes = [ctypes.c_void_p]
+        self._reset = f("svgf_reset"); self._reset.argtypes = [ctypes.c_void_p]
+        self._set = f("svgf_set_inputs"); self._set.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self._surf = f("svgf_surface"); self._surf.restype = ctypes.c_void_p; self._surf.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self._enc = f("svgf_encode"); self._enc.argtypes = [ctypes.c_void_p]
+        self._frame = f("svgf_frame"); self._frame.argtypes = [ctypes.c_void_p]
+        self._state = f("svgf_set_state"); self._state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        self._pass = f("svgf_run_pass"); self._pass.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self.iterations = int(o["iterations"])
+        self.width, self.height = int(width), int(height)
+        self._c = self._create(self.width, self.height, o["sigmaL"], o["sigmaN"], o["sigmaZ"], o["alphaColor"],
+                               o["alphaMoments"], o["taaAlpha"], int(bool(o["taa"])), int(o["iterations"]))
+
+    def close(self):
+        if self._c:
+            self._destroy(self._c)
+            self._c = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        self._reset(self._c)
+
+    @property
+    def frame(self):
+        return int(self._frame(self._c))
+
+    def surface(self, name):
+        p = self._surf(self._c, SVGF_SURF[name] if isinstance(name, str) else int(name))
+        buf = (self._ct * (self.width * self.height * 4)).from_address(p)
+        return np.ctypeslib.as_array(buf).reshape(self.height, self.width, 4)
+
+    def set_inputs(self, g: dict) -> None:
+        """g: the five inputs plus prevNormalDepth / prevMotion (binary16 bit patterns)"""
+        keys = ("radiance", "normalDepth", "motion", "material", "emission", "prevNormalDepth", "prevMotion")
+        arrs = [np.ascontiguousarray(g[k], dtype=np.uint16) for k in keys]
+        for a in arrs:
+            assert a.shape == (self.height, self.width, 4), a.shape
+        ptrs = (ctypes.c_void_p * 7)(*[a.ctypes.data for a in arrs])
+        self._set(self._c, ptrs)
+
+    def encode(self, g: dict | None = None) -> np.ndarray:
+        if g is not None:
+            self.set_inputs(g)
+        sid = self._enc(self._c)
+        return self.surface(sid).copy()
+
+    def set_state(self, frame: int, reset: bool) -> None:
+        self._state(self._c, int(frame), int(bool(reset)))
+
+    def run_pass(self, p: int) -> None:
+        self._pass(self._c, int(p))

@@ -1,4 +1,4 @@
-"""The C ABI library loads, exports every symbol include/hush.h declares, and reproduces the reference's
+"""The C ABI library loads, exports every symbol include/*.h declares, and reproduces the reference's
 host-side error behaviour (hush.js:743, 765) without touching a GPU."""
 import ctypes
 import os
@@ -14,9 +14,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "hush.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(hush_[a-z_0-9]+)\s*\(", text)))
+    syms = set()
+    inc = os.path.join(ROOT, "include")
+    for name in sorted(os.listdir(inc)):
+        if name.endswith(".h"):
+            text = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, name)).read(), flags=re.S)
+            syms |= set(re.findall(r"\b((?:hush|svgf)_[a-z_0-9]+)\s*\(", text))
+    return sorted(syms)
 
 
 def test_library_exports_every_declared_symbol():
@@ -83,3 +87,21 @@ def test_null_context_errors():
     out = _lib.hush_surface()
     assert L.hush_output(None, ctypes.byref(out)) == -1
     assert L.hush_get_precision(None) == -1
+
+
+def test_svgf_host_errors():
+    L = _lib.load()
+    o = _lib.svgf_options()
+    L.svgf_default_options(ctypes.byref(o))
+    # the defaults of svgf.js:278-281
+    assert (o.sigma_l, o.sigma_n, o.sigma_z, o.taa, o.iterations) == (4.0, 128.0, 1.0, 1, 5)
+    assert abs(o.alpha_color - 0.2) < 1e-7 and abs(o.alpha_moments - 0.2) < 1e-7 and abs(o.taa_alpha - 0.1) < 1e-7
+    p = ctypes.c_void_p()
+    o.iterations = 6
+    assert L.svgf_create(0, 8, 8, ctypes.byref(o), ctypes.byref(p)) == -6 and not p.value
+    assert b"iterations" in L.svgf_last_error(None)
+    o.iterations = 5
+    assert L.svgf_create(0, 0, 8, ctypes.byref(o), ctypes.byref(p)) == -3
+    assert L.svgf_last_error(None) == b"SVGF: bad size 0x8"
+    assert L.svgf_reset(None) == -1
+    assert L.svgf_frame(None) == -1
