@@ -1,4 +1,4 @@
-"""ctypes declarations of the C ABI in include/hush.h and include/svgf.h (libhush.so, built in-tree by hush_amd/build.py).
+"""ctypes declarations of the C ABI in include/hush.h, include/svgf.h and include/hush_train.h (libhush.so, built in-tree by hush_amd/build.py).
 
 Loading fails loudly when the library is missing: there is no CPU fallback for the denoiser.
 """
@@ -22,6 +22,9 @@ EXPORTS = [
     "svgf_default_options", "svgf_create", "svgf_destroy", "svgf_resize", "svgf_reset", "svgf_get_size", "svgf_frame",
     "svgf_get_options", "svgf_encode", "svgf_output", "svgf_last_error", "svgf_set_timing", "svgf_pass_times",
     "svgf_debug_surface", "svgf_debug_set_state", "svgf_debug_run_pass",
+    # include/hush_train.h
+    "hush_train_create", "hush_train_destroy", "hush_train_last_error", "hush_train_prepare", "hush_train_forward",
+    "hush_train_loss", "hush_train_backward", "hush_train_commit", "hush_train_a0",
 ]
 
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "hush.h"
+#include "hush_ctx.h"
 #include "hush_internal.h"
 
 using namespace hush;
@@ -16,11 +17,7 @@ namespace {
 
 thread_local char g_err[512];
 
-enum {
-  S_A0, S_X, S_BH0, S_BH1, S_MOM0, S_MOM1, S_KEY0, S_KEY1, S_HIST0, S_HIST1, S_TAA0, S_TAA1,
-  S_PAR0_0, S_PAR0_1, S_PAR1, S_PAR2, S_F0, S_F1, S_F2, S_F3, S_C0, S_C1, S_CG, S_MASK,
-  S_KD0, S_KD1, S_GH, S_CGD, S_COUNT
-};
+
 
 const char *kPassLabel[8] = {"temporal", "infer+mid", "infer+mid", "infer+mid", "coarse", "coarse", "fine+final", "taa"};
 const char *kInputName[5] = {"radiance", "normalDepth", "motion", "material", "emission"};
@@ -39,33 +36,9 @@ float half_to_float(uint16_t h) {
 
 }  // namespace
 
-struct hush_ctx {
-  int device = 0;
-  int W = 0, H = 0, W2 = 0, H2 = 0, W4 = 0, H4 = 0;
-  int64_t frame = 0;
-  bool reset = true;
-  float exposure = 1.0f;
-  int hidden = 0;
-  int precision = HUSH_MLP_FP32;
-  float *d_w = nullptr;
-  void *d_wq = nullptr;
-  float *d_wscale = nullptr;
-  void *arena = nullptr;
-  Tex tex[S_COUNT] = {};
-  TexU F[4] = {};
-  TexF kd[2] = {}, gh = {}, cgd = {};   // decoded guides (key of each frame, half-res, quarter-res)
-  uint8_t *mask = nullptr;
-  int mask_pitch = 0;
-  bool mask_on = false;
-  bool ieee = true;                // IEEE math policy (the oracle's operation-exact mirror); false = FAST
-  bool timing = false;
-  bool ev_ready = false;
-  hipEvent_t ev[9] = {};
-  char err[512] = {0};
-};
 
-static int fail(hush_t *h, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
-static int fail(hush_t *h, int code, const char *fmt, ...) {
+
+int hush_fail(hush_t *h, int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(h ? h->err : g_err, sizeof(g_err), fmt, ap);
@@ -73,9 +46,11 @@ static int fail(hush_t *h, int code, const char *fmt, ...) {
   return code;
 }
 
-static int hip_fail(hush_t *h, hipError_t e, const char *what) {
-  return fail(h, HUSH_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+int hush_hip_fail(hush_t *h, hipError_t e, const char *what) {
+  return hush_fail(h, HUSH_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
 }
+#define fail hush_fail
+#define hip_fail hush_hip_fail
 
 // the MFMA forms are built for the 16-wide hidden layers of the 20-16-16-12 network (the built-in size)
 static bool precision_built(int p, int hidden) {
@@ -296,7 +271,7 @@ static hush_surface surf_of(const hush_t *h, int id) {
 }
 
 // one pass of the frame at the context's current state (hush.js:858-870)
-static hipError_t run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream_t s) {
+hipError_t hush_run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream_t s) {
   const int cur = (int)(h->frame & 1), prev = cur ^ 1;
   const Tex *t = h->tex;
   switch (pass) {
@@ -378,7 +353,7 @@ extern "C" int32_t hush_encode(hush_t *h, const hush_inputs *in, void *stream, h
   if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
   for (int p = 0; p < 8; p++) {
     if (h->timing) (void)hipEventRecord(h->ev[p], s);
-    e = run_pass(h, p, in, s);
+    e = hush_run_pass(h, p, in, s);
     if (e != hipSuccess) return hip_fail(h, e, kPassLabel[p]);
   }
   if (h->timing) { (void)hipEventRecord(h->ev[8], s); h->ev_ready = true; }
@@ -459,7 +434,7 @@ extern "C" int32_t hush_debug_run_pass(hush_t *h, int32_t pass, const hush_input
   int r = check_inputs(h, in);
   if (r != HUSH_OK) return r;
   hipError_t e = hipSetDevice(h->device);
-  if (e == hipSuccess) e = run_pass(h, pass, in, (hipStream_t)stream);
+  if (e == hipSuccess) e = hush_run_pass(h, pass, in, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(h, e, kPassLabel[pass]);
   return HUSH_OK;
 }

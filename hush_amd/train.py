@@ -1,0 +1,266 @@
+"""Training the Hush MLP on MI355X: the differentiable frame of include/hush_train.h as torch.autograd
+functions, and a small Adam trainer over seeded procedural sequences with clean targets.
+
+    frame = DiffFrame(hush)                       # attaches to a Hush context (its size and history)
+    frame.prepare(inputs)                         # temporal pass + coarse level of this frame
+    out = frame_apply(weights, frame)             # torch (H, W, 4); differentiable in `weights` (812)
+    loss = loss_apply(out, target, prev_out, prev_target, lam, frame)   # FP64-reduced, differentiable in out
+    loss.backward(); frame.commit(out)            # history for the next frame
+
+The trained weights keep the layout of hush-weights.bin (W1 b1 W2 b2 W3 b3, hush.js:20-21);
+`save_weights` writes them as 812 binary16 values that the reference's Hush (and this one) load unchanged.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib
+from . import Hush, HushError, default_weights
+
+N_WEIGHTS = 812
+
+
+def _load():
+    L = _lib.load()
+    if getattr(L, "_train_declared", False):
+        return L
+    P, i32 = ctypes.c_void_p, ctypes.c_int32
+    L.hush_train_create.argtypes = [P, i32, ctypes.POINTER(P)]; L.hush_train_create.restype = i32
+    L.hush_train_destroy.argtypes = [P]; L.hush_train_destroy.restype = None
+    L.hush_train_last_error.argtypes = [P]; L.hush_train_last_error.restype = ctypes.c_char_p
+    L.hush_train_prepare.argtypes = [P, ctypes.POINTER(_lib.hush_inputs), P]; L.hush_train_prepare.restype = i32
+    L.hush_train_forward.argtypes = [P, P, P, P]; L.hush_train_forward.restype = i32
+    L.hush_train_loss.argtypes = [P, P, P, P, P, ctypes.c_double, P, P, P]; L.hush_train_loss.restype = i32
+    L.hush_train_backward.argtypes = [P, P, P, P, P, P]; L.hush_train_backward.restype = i32
+    L.hush_train_commit.argtypes = [P, P, P]; L.hush_train_commit.restype = i32
+    L.hush_train_a0.argtypes = [P, ctypes.POINTER(P)]; L.hush_train_a0.restype = i32
+    L._train_declared = True
+    return L
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class DiffFrame:
+    """The differentiable frame attached to a Hush context; fp64=True for gradient checks."""
+
+    def __init__(self, hush: Hush, fp64: bool = False):
+        import torch
+        self._L = _load()
+        self.hush = hush
+        self.fp64 = bool(fp64)
+        self.dtype = torch.float64 if fp64 else torch.float32
+        p = ctypes.c_void_p()
+        r = self._L.hush_train_create(ctypes.c_void_p(hush._h), int(self.fp64), ctypes.byref(p))
+        if r != 0:
+            raise HushError(self._L.hush_train_last_error(None).decode(), r)
+        self._t = p.value
+        self.width, self.height = hush.width, hush.height
+        self._keep = None
+
+    def _check(self, r):
+        if r != 0:
+            raise HushError(self._L.hush_train_last_error(ctypes.c_void_p(self._t)).decode(), r)
+
+    def prepare(self, inputs) -> None:
+        ins = self.hush._inputs(inputs)
+        self._keep = inputs                 # the surfaces stay referenced until the next prepare
+        self._check(self._L.hush_train_prepare(self._t, ctypes.byref(ins), _stream()))
+
+    def forward(self, weights):
+        import torch
+        assert weights.dtype == self.dtype and weights.is_cuda and weights.numel() == N_WEIGHTS
+        out = torch.empty((self.height, self.width, 4), dtype=self.dtype, device=weights.device)
+        self._check(self._L.hush_train_forward(self._t, _ptr(weights.contiguous()), _ptr(out), _stream()))
+        return out
+
+    def backward(self, weights, grad_out, want_a0: bool = False):
+        import torch
+        gw = torch.empty(N_WEIGHTS, dtype=self.dtype, device=weights.device)
+        ga0 = torch.empty((self.height, self.width, 4), dtype=self.dtype, device=weights.device) if want_a0 else None
+        self._check(self._L.hush_train_backward(self._t, _ptr(weights.contiguous()), _ptr(grad_out.contiguous()), _ptr(gw),
+                                                _ptr(ga0), _stream()))
+        return (gw, ga0) if want_a0 else gw
+
+    def loss(self, out, target, prev_out=None, prev_target=None, lam: float = 0.0):
+        """(loss as a 0-d float64 tensor, dloss/dout)"""
+        import torch
+        g = torch.empty_like(out)
+        l = torch.empty((), dtype=torch.float64, device=out.device)
+        self._check(self._L.hush_train_loss(self._t, _ptr(out.contiguous()), _ptr(target.contiguous()),
+                                            _ptr(prev_out), _ptr(prev_target), float(lam), _ptr(g), _ptr(l), _stream()))
+        return l, g
+
+    def commit(self, out) -> None:
+        self._check(self._L.hush_train_commit(self._t, _ptr(out.contiguous()), _stream()))
+
+    def a0_copy(self):
+        """the prepared A0 as a tensor copy"""
+        import torch
+        p = ctypes.c_void_p()
+        self._check(self._L.hush_train_a0(self._t, ctypes.byref(p)))
+        t = torch.empty((self.height, self.width, 4), dtype=self.dtype, device="cuda")
+        _memcpy_d2d(t.data_ptr(), p.value, t.numel() * t.element_size())
+        return t
+
+    def set_a0(self, a0) -> None:
+        p = ctypes.c_void_p()
+        self._check(self._L.hush_train_a0(self._t, ctypes.byref(p)))
+        a0 = a0.contiguous()
+        _memcpy_d2d(p.value, a0.data_ptr(), a0.numel() * a0.element_size())
+
+    def destroy(self):
+        if getattr(self, "_t", None):
+            import torch
+            torch.cuda.synchronize()
+            self._L.hush_train_destroy(self._t)
+            self._t = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _memcpy_d2d(dst: int, src: int, n: int) -> None:
+    import torch
+    from .hip import check, lib
+    torch.cuda.synchronize()
+    check(lib().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(n), 3), "hipMemcpy D2D")
+
+
+def _autograd():
+    import torch
+
+    class FrameFn(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, weights, frame):
+            ctx.frame = frame
+            ctx.save_for_backward(weights)
+            return frame.forward(weights)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (weights,) = ctx.saved_tensors
+            return ctx.frame.backward(weights, grad_out), None
+
+    class LossFn(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, out, target, prev_out, prev_target, lam, frame):
+            l, g = frame.loss(out, target, prev_out, prev_target, lam)
+            ctx.save_for_backward(g)
+            return l
+
+        @staticmethod
+        def backward(ctx, gl):
+            (g,) = ctx.saved_tensors
+            return g * gl.to(g.dtype), None, None, None, None, None
+
+    return FrameFn, LossFn
+
+
+_FNS = None
+
+
+def frame_apply(weights, frame: DiffFrame):
+    global _FNS
+    _FNS = _FNS or _autograd()
+    return _FNS[0].apply(weights, frame)
+
+
+def loss_apply(out, target, prev_out, prev_target, lam: float, frame: DiffFrame):
+    global _FNS
+    _FNS = _FNS or _autograd()
+    return _FNS[1].apply(out, target, prev_out, prev_target, lam, frame)
+
+
+# ------------------------------------------------------------------------------------------------ weights
+def weights_to_float(w16: np.ndarray) -> np.ndarray:
+    return np.asarray(w16, np.uint16).view(np.float16).astype(np.float64)
+
+
+def random_weights(seed: int, hidden: int = 16) -> np.ndarray:
+    """seeded He-normal init at 20-16-16-12, biases zero, in the hush-weights.bin layout (float64)"""
+    rng = np.random.default_rng(seed)
+    h = hidden
+    w1 = rng.normal(0, np.sqrt(2.0 / 20), (h, 20))
+    w2 = rng.normal(0, np.sqrt(2.0 / h), (h, h))
+    w3 = rng.normal(0, np.sqrt(1.0 / h), (12, h))
+    return np.concatenate([w1.ravel(), np.zeros(h), w2.ravel(), np.zeros(h), w3.ravel(), np.zeros(12)])
+
+
+def to_f16_blob(w: np.ndarray) -> np.ndarray:
+    """812 binary16 values (round to nearest) in the hush-weights.bin layout"""
+    return np.asarray(w, np.float64).astype(np.float16).view(np.uint16)
+
+
+def save_weights(w, path: str) -> None:
+    arr = w.detach().cpu().numpy() if hasattr(w, "detach") else np.asarray(w)
+    to_f16_blob(arr).tofile(path)
+
+
+# ------------------------------------------------------------------------------------------------ trainer
+def upload(g: dict, keys=("radiance", "normalDepth", "motion", "material", "emission")):
+    import torch
+    return {k: torch.from_numpy(np.ascontiguousarray(g[k]).view(np.float16)).cuda() for k in keys}
+
+
+def target_of(g: dict, dtype):
+    import torch
+    return torch.from_numpy(g["clean"]).to(device="cuda", dtype=dtype)
+
+
+def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256), scene_size=(640, 360), lr: float = 1e-3,
+          lam: float = 0.5, seed: int = 0, init: str = "random", log=print):
+    """Adam over seeded procedural sequences ('train' scenes: 300 primitives, 30% moving, camera cuts).
+    Each step: `sequences` crops x `frames` frames; the gradient of every frame accumulates (the history
+    enters each frame as an input).  Returns (weights tensor, [(step, loss)])."""
+    import torch
+    from .scene import SceneGenerator
+    w0 = random_weights(seed) if init == "random" else weights_to_float(default_weights())
+    weights = torch.tensor(w0, dtype=torch.float32, device="cuda", requires_grad=True)
+    opt = torch.optim.Adam([weights], lr=lr)
+    cw, ch = crop
+    hist = []
+    t0 = time.perf_counter()
+    for step in range(steps):
+        opt.zero_grad(set_to_none=False)
+        total = 0.0
+        for s in range(sequences):
+            rs = np.random.default_rng([seed, step, s])
+            gen = SceneGenerator("train", scene_size[0], scene_size[1], int(rs.integers(1 << 30)))
+            x0 = int(rs.integers(0, scene_size[0] - cw + 1))
+            y0 = int(rs.integers(0, scene_size[1] - ch + 1))
+            f0 = int(rs.integers(0, 96))
+            hs = Hush(0, {"width": cw, "height": ch})      # temporal / coarse passes only: no MLP weights used
+            fr = DiffFrame(hs)
+            prev_out = prev_tgt = None
+            for f in range(frames):
+                g = gen.render(f0 + f, crop=(x0, y0, cw, ch), clean=True)
+                if gen.is_cut(f0 + f):
+                    hs.reset()
+                fr.prepare(upload(g))
+                out = frame_apply(weights, fr)
+                tgt = target_of(g, torch.float32)
+                loss = loss_apply(out, tgt, prev_out, prev_tgt, lam if prev_out is not None else 0.0, fr)
+                (loss / (sequences * frames)).backward()
+                total += float(loss)
+                fr.commit(out.detach())
+                prev_out, prev_tgt = out.detach(), tgt
+            fr.destroy()
+            hs.destroy()
+        opt.step()
+        hist.append((step, total / (sequences * frames)))
+        if log and (step % max(1, steps // 20) == 0 or step == steps - 1):
+            log(f"[train] step {step} loss {hist[-1][1]:.5f} ({time.perf_counter() - t0:.1f}s)")
+    return weights.detach(), hist

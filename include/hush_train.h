@@ -1,0 +1,72 @@
+/*
+ * hush_train.h — differentiable frame of the Hush denoiser on MI355X (gfx950): forward and hand-written
+ * backward kernels over the same device surfaces as hush.h, for training the MLP weights.
+ *
+ * No counterpart in the reference (which ships trained weights only, src/denoise/hush-weights.bin); the
+ * trained weights use its layout (W1 b1 W2 b2 W3 b3, row-major (out, in), hush.js:20-21), so the
+ * reference's Hush loads them unchanged once rounded to binary16.
+ *
+ * The differentiable function of one frame.  The temporal pass and the coarse level (which has no learned
+ * parameters) run as in hush_encode, then, in the context's arithmetic (FP32, or FP64 for gradient
+ * checks) and without binary16 rounding between the passes:
+ *   MLP (hush.js:348-366)  ->  mid1, mid2, mid3 a-trous (hush.js:431-476, 366, 753-754)
+ *   fine a-trous on the temporal output  ->  level recombination (hush.js:560-620)  ->  TAA (hush.js:625-677)
+ * The gradient flows to the 812 weights through the fine / mid edge-stopping strengths s and centre
+ * weights c, the detail gain kappa and the coarse blend beta; the temporal history, the previous TAA
+ * output and the MLP features are inputs of the frame (truncated backpropagation through time).  The
+ * backward recomputes every a-trous weight instead of storing it.
+ *
+ * Buffers passed here are plain device arrays, row-major, 4 values per pixel (rgba), of the context's
+ * element type (float, or double when created with fp64 = 1).  Weights and their gradient: 812 values.
+ */
+#ifndef HUSH_TRAIN_H
+#define HUSH_TRAIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "hush.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hush_train_ctx hush_train_t;
+
+/* attaches to a denoiser context (its size, device, history); hidden width 16 only */
+int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out);
+void hush_train_destroy(hush_train_t *t);
+const char *hush_train_last_error(const hush_train_t *t);
+
+/* runs the temporal pass and the coarse level for this frame's inputs at the context's state, and copies
+   the temporal output A0 into the differentiable frame's element type */
+int32_t hush_train_prepare(hush_train_t *t, const hush_inputs *in, void *stream);
+
+/* forward of the prepared frame with `weights` (812 values) -> `out` (W x H x 4) */
+int32_t hush_train_forward(hush_train_t *t, const void *weights, void *out, void *stream);
+
+/* loss = mean over pixels of sum over rgb |tm(out) - tm(target)|
+          + lambda * |(tm(out) - tm(prev_out)) - (tm(target) - tm(prev_target))|   (second term only when
+   prev_out and prev_target are given), tm(c) = c / (1 + max(lum(c), 0)); the sum is reduced in FP64.
+   Writes dloss/dout to grad_out and the loss to *loss_dev (one double in device memory). */
+int32_t hush_train_loss(hush_train_t *t, const void *out, const void *target, const void *prev_out,
+                        const void *prev_target, double lambda, void *grad_out, double *loss_dev, void *stream);
+
+/* backward of the last forward: grad_out (W x H x 4) -> grad_w (812 values, overwritten); grad_a0
+   (optional, W x H x 4) receives the gradient with respect to the temporal output A0 */
+int32_t hush_train_backward(hush_train_t *t, const void *weights, const void *grad_out, void *grad_w, void *grad_a0,
+                            void *stream);
+
+/* ends the frame: the fine level and the MLP's (alpha, gamma, beta, kappa) become the denoiser's history
+   (rounded to binary16 as hush_encode stores them), `out` becomes the previous TAA output, the frame
+   counter advances and the reset flag clears */
+int32_t hush_train_commit(hush_train_t *t, const void *out, void *stream);
+
+/* the prepared A0 in the element type (W x H x 4), writable (input perturbation for gradient checks) */
+int32_t hush_train_a0(hush_train_t *t, void **ptr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HUSH_TRAIN_H */

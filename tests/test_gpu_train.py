@@ -1,0 +1,138 @@
+"""Gradient checks of the differentiable frame (include/hush_train.h) on the GPU.
+
+* FP64 analytic gradients (hand-written backward kernels through TAA, recombination, the fine and mid
+  a-trous levels and the MLP) against FP64 central finite differences of the same forward, for all 812
+  MLP weights and 200 temporal-output (A0) inputs, on 64x64 frames after three frames of history:
+  beyond the difference quotient's own rounding noise (10 |loss| 2^-52 / step), relative error <= 1e-4
+  for >= 99% of the checked values and <= 1e-2 for all of them (the L1 loss, the clip box and the clamps
+  have kinks that a finite step can straddle).
+* The FP32 backward agrees with the FP64 one to 1e-3 (relative to the largest gradient).
+* Adam steps on seeded sequences lower the loss.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(fp64, n_hist=3, kind="spheres", size=64, seed=7):
+    import torch
+    import hush_amd
+    from hush_amd import train as T
+    from hush_amd.scene import SceneGenerator
+    gen = SceneGenerator(kind, size, size, seed)
+    hs = hush_amd.Hush(0, {"width": size, "height": size})
+    fr = T.DiffFrame(hs, fp64=fp64)
+    dt = torch.float64 if fp64 else torch.float32
+    w = torch.tensor(T.weights_to_float(hush_amd.default_weights()), dtype=dt, device="cuda")
+    prev = None
+    for f in range(n_hist):
+        g = gen.render(f, clean=True)
+        fr.prepare(T.upload(g))
+        out = fr.forward(w)
+        fr.commit(out)
+        prev = (out, T.target_of(g, dt))
+    g = gen.render(n_hist, clean=True)
+    fr.prepare(T.upload(g))
+    return hs, fr, w, T.target_of(g, dt), prev
+
+
+def _loss(fr, w, tgt, prev, lam=0.5):
+    out = fr.forward(w)
+    l, g = fr.loss(out, tgt, prev[0] if prev else None, prev[1] if prev else None, lam if prev else 0.0)
+    return l, g
+
+
+def _rel(fd, an, noise=None):
+    """|fd - an| relative to the gradient, less the finite difference's own rounding noise (`noise`:
+    |loss| * 2^-52 / step per value), so gradients below the resolution of a difference quotient do not
+    count as mismatches"""
+    d = np.abs(fd - an)
+    if noise is not None:
+        d = np.maximum(d - 10.0 * noise, 0.0)
+    scale = np.maximum(np.maximum(np.abs(an), np.abs(fd)), 1e-3 * np.abs(an).max())
+    return d / scale
+
+
+def test_fd_weights_and_inputs_fp64():
+    import torch
+    hs, fr, w, tgt, prev = _setup(True)
+    l0, g = _loss(fr, w, tgt, prev)
+    gw, ga0 = fr.backward(w, g, want_a0=True)
+    an = gw.cpu().numpy()
+    assert np.isfinite(an).all() and np.abs(an).max() > 0
+    fd, nz = np.zeros(812), np.zeros(812)
+    for k in range(812):
+        eps = 1e-6 * max(1.0, abs(float(w[k])))
+        nz[k] = abs(float(l0)) * 2.0 ** -52 / eps
+        wp, wm = w.clone(), w.clone()
+        wp[k] += eps
+        wm[k] -= eps
+        fd[k] = (float(_loss(fr, wp, tgt, prev)[0]) - float(_loss(fr, wm, tgt, prev)[0])) / (2 * eps)
+    r = _rel(fd, an, nz)
+    raw = _rel(fd, an)
+    print("weights: raw rel err median", float(np.median(raw)), "max", float(raw.max()),
+          "| beyond FD noise: p99", float(np.quantile(r, 0.99)), "max", float(r.max()))
+    assert np.quantile(r, 0.99) <= 1e-4 and r.max() <= 1e-2, (np.argsort(-r)[:8], r[np.argsort(-r)[:8]])
+    # temporal-output inputs: 200 random texel channels of A0
+    a0 = fr.a0_copy()
+    rng = np.random.default_rng(3)
+    H, W = a0.shape[:2]
+    idx = [(int(rng.integers(H)), int(rng.integers(W)), int(rng.integers(4))) for _ in range(200)]
+    ga = ga0.cpu().numpy()
+    fd_a, an_a = [], []
+    def fd_at(y, x, c, eps):
+        ap, am = a0.clone(), a0.clone()
+        ap[y, x, c] += eps
+        am[y, x, c] -= eps
+        fr.set_a0(ap)
+        lp = float(_loss(fr, w, tgt, prev)[0])
+        fr.set_a0(am)
+        lm = float(_loss(fr, w, tgt, prev)[0])
+        return (lp - lm) / (2 * eps)
+
+    vals, nz_a = [], []
+    for (y, x, c) in idx:
+        v = float(a0[y, x, c])
+        # step relative to the value: the variance channel enters through sqrt(max(gv, 1e-12))
+        eps = 1e-5 * max(abs(v), 1e-3)
+        fd_a.append(fd_at(y, x, c, eps))
+        an_a.append(ga[y, x, c])
+        nz_a.append(abs(float(l0)) * 2.0 ** -52 / eps)
+        vals.append((y, x, c, v, eps))
+    ra = _rel(np.array(fd_a), np.array(an_a), np.array(nz_a))
+    for j in np.argsort(-ra)[:6]:
+        y, x, c, v, eps = vals[j]
+        print("A0 worst", (y, x, c), "value", v, "fd", fd_a[j], "an", an_a[j], "fd(eps/10)", fd_at(y, x, c, eps / 10))
+    fr.set_a0(a0)
+    raw = _rel(np.array(fd_a), np.array(an_a))
+    print("A0: raw rel err median", float(np.median(raw)), "max", float(raw.max()),
+          "| beyond FD noise: p99", float(np.quantile(ra, 0.99)), "max", float(ra.max()))
+    assert np.quantile(ra, 0.99) <= 1e-4 and ra.max() <= 1e-2
+    fr.destroy()
+    hs.destroy()
+
+
+def test_fp32_backward_matches_fp64():
+    res = []
+    for fp64 in (True, False):
+        hs, fr, w, tgt, prev = _setup(fp64, n_hist=0)
+        l, g = _loss(fr, w, tgt, prev)
+        res.append((float(l), fr.backward(w, g).double().cpu().numpy()))
+        fr.destroy()
+        hs.destroy()
+    (l64, g64), (l32, g32) = res
+    assert abs(l64 - l32) <= 1e-4 * abs(l64)
+    err = np.abs(g64 - g32).max() / np.abs(g64).max()
+    print("fp32 vs fp64 gradient: max err / max |g|", err)
+    assert err <= 1e-3
+
+
+def test_adam_lowers_the_loss():
+    from hush_amd import train as T
+    w, hist = T.train(steps=12, sequences=2, frames=4, crop=(64, 64), scene_size=(160, 96), lr=3e-3, seed=1, log=None)
+    losses = [l for _, l in hist]
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-3:]) < np.mean(losses[:3]), losses
+    blob = T.to_f16_blob(w.cpu().numpy())
+    assert blob.size == 812
