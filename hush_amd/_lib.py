@@ -24,7 +24,7 @@ EXPORTS = [
     "svgf_debug_surface", "svgf_debug_set_state", "svgf_debug_run_pass",
     # include/hush_train.h
     "hush_train_create", "hush_train_destroy", "hush_train_last_error", "hush_train_prepare", "hush_train_forward",
-    "hush_train_loss", "hush_train_backward", "hush_train_commit", "hush_train_a0",
+    "hush_train_loss", "hush_train_backward", "hush_train_commit", "hush_train_a0", "hush_train_set_a0_fixed",
 ]
 
 

@@ -32,6 +32,7 @@ struct hush_ctx {
   uint8_t *mask = nullptr;
   int mask_pitch = 0;
   bool mask_on = false;
+  float *train_aux = nullptr;      // set by the differentiable frame around its temporal pass
   bool ieee = true;                // IEEE math policy (the oracle's operation-exact mirror); false = FAST
   bool timing = false;
   bool ev_ready = false;

@@ -18,6 +18,8 @@ struct MfmaPack {
   float sw[3];             // fp8 weight scale per layer (power of two)
 };
 
+constexpr int TEMPORAL_AUX = 20;
+
 struct TemporalArgs {
   Tex rad, nd, mv, mat;              // this frame's G-buffer
   TexF pkd;                          // history: decoded key of the previous frame (normal, (depth, id))
@@ -28,6 +30,7 @@ struct TemporalArgs {
   TexF outGH;                        // decoded half-res guide (normal, packed (Z, count))
   uint8_t *mask;                     // optional per-pixel reprojection bits (debug export)
   int mask_pitch;
+  float *aux;                        // optional temporal-blend inputs for the differentiable frame (W x H x TEMPORAL_AUX)
   int W, H, W2, H2, reset;
   float E;                           // exposure
 };

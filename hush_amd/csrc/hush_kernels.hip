@@ -34,6 +34,10 @@ enum { P0X, P0Y, P0Z, P0W, P1X, P1Y, P1Z, P1W, P2Z, P2W, NSX, NSY, NSZ, P3X, P3Y
 
 // Temporal statistics (moments, variances, cancellation-prone differences such as m2 - m1^2) are
 // ill-conditioned, so this pass is evaluated with IEEE arithmetic under both math policies.
+// AUX (training only): also writes the per-pixel inputs of the temporal blend (TEMPORAL_AUX floats each:
+// history flag, illumination, 3x3 mean and deviation, reprojected colour and moments, luminance, spatial
+// variance, the half-res texel of alpha / gamma), from which the differentiable frame re-blends A0.
+template <bool AUX>
 __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   constexpr bool FAST = false;
   using F = Math<FAST>;
@@ -98,6 +102,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       const float3 d0 = oct_dec(0.0f, 0.0f);
       stf(a.outKD, px, py, make_float4(d0.x, d0.y, d0.z, __uint_as_float(zid)));
       mbits = 0x80;
+      if (AUX) a.aux[((size_t)py * a.W + px) * TEMPORAL_AUX] = -1.0f;
     } else {
       const float3 n = make_float3(nd.x, nd.y, nd.z);
       const int c0 = (ly + 1) * SW + lx + 1;
@@ -205,6 +210,15 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       }
       float tvar = mx(nmy - nmx * nmx, 0.0f);
       float4 A = make_float4(cr, cg, cb, svar + (tvar - svar) * mn(len1 / 4.0f, 1.0f));
+      if (AUX) {
+        float *x = a.aux + ((size_t)py * a.W + px) * TEMPORAL_AUX;
+        const int cxq = clampi((int)floorf((float)px + mv.x + 0.5f), 0, a.W - 1) >> 1;
+        const int cyq = clampi((int)floorf((float)py + mv.y + 0.5f), 0, a.H - 1) >> 1;
+        const float v[TEMPORAL_AUX] = {has ? 1.0f : 0.0f, ce.x, ce.y, ce.z, mux, muy, muz, sdx, sdy, sdz, hcx, hcy, hcz,
+                                       hm.x, hm.y, hm.w, l, svar, (float)(cyq * a.W2 + cxq), 0.0f};
+#pragma unroll
+        for (int k = 0; k < TEMPORAL_AUX; k++) x[k] = v[k];
+      }
       float2 oc = oct_enc(n);
       st(a.outA, px, py, A);
       st(a.outMom, px, py, make_float4(nmx, nmy, nfast, len1));
@@ -1160,7 +1174,8 @@ static dim3 grid_for(int w, int h) { return dim3(((w + TW - 1) / TW) * ((h + TH 
 #define K_TAA(F) k_taa<F>
 
 hipError_t launch_temporal(const TemporalArgs &a, bool, hipStream_t s) {
-  hipLaunchKernelGGL(k_temporal, grid_for(a.W, a.H), dim3(NT), 0, s, a);
+  if (a.aux) hipLaunchKernelGGL(k_temporal<true>, grid_for(a.W, a.H), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL(k_temporal<false>, grid_for(a.W, a.H), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 #include "hush.h"
 #include "hush_ctx.h"
 #include "hush_device.h"
+#include "hush_internal.h"
 #include "hush_train.h"
 
 using namespace hush;
@@ -765,6 +766,95 @@ __global__ __launch_bounds__(NTH) void tk_wsum(const double *part, int nb, T *gw
   gw[k] = (T)s;
 }
 
+
+// ------------------------------------------------------------------------------------------- temporal blend
+// A0 from the temporal pass's blend inputs (aux, TEMPORAL_AUX floats per pixel) and the previous frame's
+// alpha / gamma (hush.js:189-211): colour = lerp(clamp(hc, mu +- gamma sd), illum, a), a = max(alpha, 1/len1);
+// variance = svar + (tvar - svar) min(len1 / 4, 1), tvar from the blended moments
+template <class T>
+__global__ __launch_bounds__(NTH) void tk_blend(const float *aux, const T *pprev, int n, T *a0) {
+  const int i = tid1();
+  if (i >= n) return;
+  const float *x = aux + (size_t)i * TEMPORAL_AUX;
+  if (x[0] < 0.0f) { st4(a0, i, V4<T>{T(0), T(0), T(0), T(0)}); return; }
+  const T ce[3] = {x[1], x[2], x[3]}, l = x[16], svar = x[17];
+  T col[3], nmx = l, nmy = l * l, len1 = T(1);
+  if (x[0] > 0.0f) {
+    const int q = (int)x[18];
+    const T al = pprev[12 * q], ga = pprev[12 * q + 1];
+    len1 = tmin((T)x[15] + T(1), T(32));
+    const T aa = tmax(al, T(1) / len1);
+    for (int k = 0; k < 3; k++) {
+      const T g = ga * (T)x[7 + k], mu = x[4 + k];
+      const T hl = tmin(tmax((T)x[10 + k], mu - g), mu + g);
+      col[k] = hl + (ce[k] - hl) * aa;
+    }
+    nmx = (T)x[13] + (l - (T)x[13]) * aa;
+    nmy = (T)x[14] + (l * l - (T)x[14]) * aa;
+  } else {
+    for (int k = 0; k < 3; k++) col[k] = ce[k];
+  }
+  const T tvar = tmax(nmy - nmx * nmx, T(0));
+  st4(a0, i, V4<T>{col[0], col[1], col[2], svar + (tvar - svar) * tmin(len1 / T(4), T(1))});
+}
+
+// dL/dalpha, dL/dgamma of the previous frame's half-res texels from dL/dA0 (added atomically: the
+// reprojected texel of a pixel follows its motion vector)
+template <class T>
+__global__ __launch_bounds__(NTH) void tk_blend_bwd(const float *aux, const T *pprev, const T *ga0, int n, T *gprev) {
+  const int i = tid1();
+  if (i >= n) return;
+  const float *x = aux + (size_t)i * TEMPORAL_AUX;
+  if (!(x[0] > 0.0f)) return;
+  const V4<T> g = ld4(ga0, i);
+  const T gv[3] = {g.x, g.y, g.z};
+  const int q = (int)x[18];
+  const T al = pprev[12 * q], ga = pprev[12 * q + 1];
+  const T l = x[16], len1 = tmin((T)x[15] + T(1), T(32));
+  const T aa = tmax(al, T(1) / len1);
+  T daa = T(0), dga = T(0);
+  for (int k = 0; k < 3; k++) {
+    const T sd = x[7 + k], mu = x[4 + k], hc = x[10 + k];
+    const T lo = mu - ga * sd, hi = mu + ga * sd;
+    const T mv = tmax(hc, lo);
+    const T hl = tmin(mv, hi);
+    daa += gv[k] * ((T)x[1 + k] - hl);
+    const T dhl = gv[k] * (T(1) - aa);
+    if (!(mv < hi)) dga += dhl * sd;
+    else if (!(hc > lo)) dga -= dhl * sd;
+  }
+  const T nmx = (T)x[13] + (l - (T)x[13]) * aa, nmy = (T)x[14] + (l * l - (T)x[14]) * aa;
+  if (nmy - nmx * nmx > T(0)) daa += g.w * tmin(len1 / T(4), T(1)) * ((l * l - (T)x[14]) - T(2) * nmx * (l - (T)x[13]));
+  if (al > T(1) / len1) atomicAdd(&gprev[12 * q], daa);
+  atomicAdd(&gprev[12 * q + 1], dga);
+}
+
+// (alpha, gamma, beta, kappa) of a stored binary16 par0 texel into the 12-wide parameter layout
+template <class T>
+__global__ __launch_bounds__(NTH) void tk_par_from_tex(Tex src, int W2, int H2, T *par) {
+  const int i = tid1();
+  if (i >= W2 * H2) return;
+  const float4 v = ld(src, i % W2, i / W2);
+  T *p = par + 12 * i;
+  p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = v.w;
+  for (int k = 4; k < 12; k++) p[k] = T(0);
+}
+
+template <class T>
+__global__ __launch_bounds__(NTH) void tk_zero(T *p, long n) {
+  const long i = (long)blockIdx.x * NTH + threadIdx.x;
+  if (i < n) p[i] = T(0);
+}
+
+__global__ __launch_bounds__(NTH) void tk_copy_u4(const TexU src, int W2, int H2, uint4 *dst) {
+  const int i = tid1();
+  if (i < W2 * H2) dst[i] = ldu(src, i % W2, i / W2);
+}
+__global__ __launch_bounds__(NTH) void tk_copy_f4(const TexF src, int W2, int H2, float4 *dst) {
+  const int i = tid1();
+  if (i < W2 * H2) dst[i] = ldf(src, i % W2, i / W2);
+}
+
 // ------------------------------------------------------------------------------------------- state
 template <class T>
 __global__ __launch_bounds__(NTH) void tk_from_tex(Tex src, int W, int H, T *dst) {
@@ -800,6 +890,13 @@ struct hush_train_ctx {
   void *gfine = nullptr, *gill = nullptr, *gpf = nullptr, *coefF = nullptr;
   double *wpart = nullptr, *lpart = nullptr;
   int nwb = 0, nlb = 0;
+  float *aux = nullptr;                 // temporal-blend inputs of the prepared frame
+  uint4 *Fprev[3] = {};                 // the previous frame's features F0..F2 (half res)
+  float4 *GHprev = nullptr;             // and its half-res guide (sky blocks)
+  void *oprev = nullptr, *pprev = nullptr, *gpprev = nullptr, *ga0 = nullptr;
+  bool has_prev = false;                // Fprev holds the committed previous frame
+  int prev_mode = 0;                    // this frame: 0 no history, 1 alpha/gamma from the stored par0, 2 recomputed
+  bool a0_fixed = false;
   hush_inputs in{};
   bool prepared = false, forwarded = false, reset = true;
   char err[512] = {0};
@@ -830,8 +927,8 @@ extern "C" int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out
   t->nlb = (int)((N + NTH - 1) / NTH);
   // half: o, par, gpar, coefH (12 each), mid x3, gmid x3 (4 each); full: a0, fine, X, prev, gX, gfine, gill, gpf
   // (4 each), taac, coefF (12 each); FP64 partials
-  const size_t half = N2 * ts * (4 * 12 + 6 * 4), full = N * ts * (8 * 4 + 2 * 12);
-  const size_t parts = ((size_t)t->nwb * NW + t->nlb + 64) * sizeof(double);
+  const size_t half = N2 * ts * (7 * 12 + 6 * 4) + N2 * 16 * 4, full = N * ts * (9 * 4 + 2 * 12) + N * 4 * TEMPORAL_AUX;
+  const size_t parts = ((size_t)2 * t->nwb * NW + t->nlb + 64) * sizeof(double);
   const size_t total = half + full + parts + 64 * 256;
   e = hipMalloc(&t->arena, total);
   if (e == hipSuccess) e = hipMemset(t->arena, 0, total);
@@ -848,7 +945,11 @@ extern "C" int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out
   t->a0 = take(N * ts * 4); t->fine = take(N * ts * 4); t->X = take(N * ts * 4); t->prev = take(N * ts * 4);
   t->gX = take(N * ts * 4); t->gfine = take(N * ts * 4); t->gill = take(N * ts * 4); t->gpf = take(N * ts * 4);
   t->taac = take(N * ts * 12); t->coefF = take(N * ts * 12);
-  t->wpart = (double *)take((size_t)t->nwb * NW * sizeof(double));
+  t->oprev = take(N2 * ts * 12); t->pprev = take(N2 * ts * 12); t->gpprev = take(N2 * ts * 12); t->ga0 = take(N * ts * 4);
+  t->aux = (float *)take(N * 4 * TEMPORAL_AUX);
+  for (int k = 0; k < 3; k++) t->Fprev[k] = (uint4 *)take(N2 * 16);
+  t->GHprev = (float4 *)take(N2 * 16);
+  t->wpart = (double *)take((size_t)2 * t->nwb * NW * sizeof(double));
   t->lpart = (double *)take((size_t)t->nlb * sizeof(double));
   *out = t;
   return HUSH_OK;
@@ -874,7 +975,16 @@ extern "C" int32_t hush_train_prepare(hush_train_t *t, const hush_inputs *in, vo
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(h->device);
   const int passes[3] = {0, 4, 5};
+  h->train_aux = t->aux;
   for (int k = 0; k < 3 && e == hipSuccess; k++) e = hush_run_pass(h, passes[k], in, s);
+  h->train_aux = nullptr;
+  t->prev_mode = h->reset ? 0 : (t->has_prev ? 2 : 1);
+  if (e == hipSuccess && t->prev_mode == 1) {     // history from outside this context: alpha / gamma as stored
+    const Tex p0 = h->tex[S_PAR0_0 + (int)((h->frame & 1) ^ 1)];
+    if (t->fp64) hipLaunchKernelGGL(tk_par_from_tex<double>, grid1((long)t->W2 * t->H2), dim3(NTH), 0, s, p0, t->W2, t->H2, (double *)t->pprev);
+    else hipLaunchKernelGGL(tk_par_from_tex<float>, grid1((long)t->W2 * t->H2), dim3(NTH), 0, s, p0, t->W2, t->H2, (float *)t->pprev);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) {
     if (t->fp64) hipLaunchKernelGGL(tk_from_tex<double>, grid1((long)t->W * t->H), dim3(NTH), 0, s, h->tex[S_A0], t->W, t->H, (double *)t->a0);
     else hipLaunchKernelGGL(tk_from_tex<float>, grid1((long)t->W * t->H), dim3(NTH), 0, s, h->tex[S_A0], t->W, t->H, (float *)t->a0);
@@ -910,10 +1020,19 @@ static FiltArgs<T> filt_args(hush_train_t *t, int level, const T *gout, T *gsrc)
   return a;
 }
 
+static TexU fprev_tex(hush_train_t *t, int k) { return TexU{t->Fprev[k], t->W2}; }
+
 template <class T>
 static hipError_t forward_t(hush_train_t *t, const T *w, T *out, hipStream_t s) {
   hush_t *h = t->h;
   const long N = (long)t->W * t->H, N2 = (long)t->W2 * t->H2;
+  if (!t->a0_fixed) {
+    // the previous frame's alpha / gamma with these weights (one step back through the temporal blend)
+    if (t->prev_mode == 2)
+      hipLaunchKernelGGL(tk_infer<T>, grid1(N2), dim3(NTH), 0, s, fprev_tex(t, 0), fprev_tex(t, 1), fprev_tex(t, 2), TexF{t->GHprev, t->W2}, t->W2,
+                         t->H2, w, (T *)t->oprev, (T *)t->pprev);
+    hipLaunchKernelGGL(tk_blend<T>, grid1(N), dim3(NTH), 0, s, (const float *)t->aux, (const T *)t->pprev, (int)N, (T *)t->a0);
+  }
   hipLaunchKernelGGL(tk_infer<T>, grid1(N2), dim3(NTH), 0, s, h->F[0], h->F[1], h->F[2], h->gh, t->W2, t->H2, w, (T *)t->o, (T *)t->par);
   // mid1 (R 1, step 1, prefilter, source = block means), mid2 (step 2), mid3 (step 4) (hush.js:366, 753-754)
   hipLaunchKernelGGL((tk_filt<T, 1, 1, true, 0, false, false>), grid1(N2), dim3(NTH), 0, s, filt_args<T>(t, 1, nullptr, nullptr));
@@ -997,8 +1116,10 @@ static hipError_t backward_t(hush_train_t *t, const T *w, const T *gout, T *gw, 
   fa.gmid = (T *)t->gmid[2]; fa.gpar = (T *)t->gpar;
   hipLaunchKernelGGL((tk_final<T, true>), grid1(N), dim3(NTH), 0, s, fa);
   // fine level: dL/ds0, dL/dc0 per pixel (into gpf.zw), optionally dL/dA0
-  hipLaunchKernelGGL((tk_filt<T, 1, 1, true, 1, true, true>), grid1(N), dim3(NTH), 0, s, filt_args<T>(t, 0, (const T *)t->gfine, ga0));
-  if (ga0) hipLaunchKernelGGL((tk_filt_gather<T, 1, 1, true, 1>), grid1(N), dim3(NTH), 0, s, filt_args<T>(t, 0, (const T *)t->gfine, ga0));
+  const bool chain = !t->a0_fixed && t->prev_mode == 2;
+  T *ga = ga0 ? ga0 : (T *)t->ga0;
+  hipLaunchKernelGGL((tk_filt<T, 1, 1, true, 1, true, true>), grid1(N), dim3(NTH), 0, s, filt_args<T>(t, 0, (const T *)t->gfine, ga));
+  if (ga0 || chain) hipLaunchKernelGGL((tk_filt_gather<T, 1, 1, true, 1>), grid1(N), dim3(NTH), 0, s, filt_args<T>(t, 0, (const T *)t->gfine, ga));
   hipLaunchKernelGGL(tk_final_gather<T>, grid1(N2), dim3(NTH), 0, s, fa);
   // mid3 -> mid2 -> mid1
   hipLaunchKernelGGL((tk_filt<T, 1, 4, false, 1, false, true>), grid1(N2), dim3(NTH), 0, s, filt_args<T>(t, 3, (const T *)t->gmid[2], (T *)t->gmid[1]));
@@ -1008,7 +1129,17 @@ static hipError_t backward_t(hush_train_t *t, const T *w, const T *gout, T *gw, 
   hipLaunchKernelGGL((tk_filt<T, 1, 1, true, 0, false, true>), grid1(N2), dim3(NTH), 0, s, filt_args<T>(t, 1, (const T *)t->gmid[0], nullptr));
   hipLaunchKernelGGL(tk_mlp_bwd<T>, dim3(t->nwb), dim3(MB), 0, s, h->F[0], h->F[1], h->F[2], h->gh, t->W2, t->H2, w, (const T *)t->o,
                      (const T *)t->par, (const T *)t->gpar, t->wpart);
-  hipLaunchKernelGGL(tk_wsum<T>, grid1(NW), dim3(NTH), 0, s, (const double *)t->wpart, t->nwb, gw);
+  int nb = t->nwb;
+  if (chain) {
+    // temporal blend -> the previous frame's alpha / gamma -> its MLP (features of that frame)
+    hipLaunchKernelGGL(tk_zero<T>, grid1(N2 * 12), dim3(NTH), 0, s, (T *)t->gpprev, N2 * 12);
+    hipLaunchKernelGGL(tk_blend_bwd<T>, grid1(N), dim3(NTH), 0, s, (const float *)t->aux, (const T *)t->pprev, (const T *)ga, (int)N,
+                       (T *)t->gpprev);
+    hipLaunchKernelGGL(tk_mlp_bwd<T>, dim3(t->nwb), dim3(MB), 0, s, fprev_tex(t, 0), fprev_tex(t, 1), fprev_tex(t, 2), TexF{t->GHprev, t->W2},
+                       t->W2, t->H2, w, (const T *)t->oprev, (const T *)t->pprev, (const T *)t->gpprev, t->wpart + (size_t)t->nwb * NW);
+    nb *= 2;
+  }
+  hipLaunchKernelGGL(tk_wsum<T>, grid1(NW), dim3(NTH), 0, s, (const double *)t->wpart, nb, gw);
   return hipGetLastError();
 }
 
@@ -1035,6 +1166,8 @@ static hipError_t commit_t(hush_train_t *t, const T *out, hipStream_t s) {
   hipLaunchKernelGGL(tk_to_tex<T>, grid1(N2), dim3(NTH), 0, s, (const T *)t->par, 12, t->W2, t->H2, h->tex[S_PAR0_0 + cur]);
   hipLaunchKernelGGL(tk_to_tex<T>, grid1(N), dim3(NTH), 0, s, out, 4, t->W, t->H, h->tex[S_TAA0 + cur]);
   hipLaunchKernelGGL(tk_copy<T>, grid1(N * 4), dim3(NTH), 0, s, out, (T *)t->prev, N * 4);
+  for (int k = 0; k < 3; k++) hipLaunchKernelGGL(tk_copy_u4, grid1(N2), dim3(NTH), 0, s, h->F[k], t->W2, t->H2, t->Fprev[k]);
+  hipLaunchKernelGGL(tk_copy_f4, grid1(N2), dim3(NTH), 0, s, h->gh, t->W2, t->H2, t->GHprev);
   return hipGetLastError();
 }
 
@@ -1048,6 +1181,13 @@ extern "C" int32_t hush_train_commit(hush_train_t *t, const void *out, void *str
   t->h->reset = false;
   t->h->frame++;
   t->prepared = t->forwarded = false;
+  t->has_prev = true;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_train_set_a0_fixed(hush_train_t *t, int32_t fixed) {
+  if (!t) return tfail(nullptr, HUSH_ERR_ARGUMENT, "hush_train_set_a0_fixed: null context");
+  t->a0_fixed = fixed != 0;
   return HUSH_OK;
 }
 

@@ -37,6 +37,7 @@ def _load():
     L.hush_train_backward.argtypes = [P, P, P, P, P, P]; L.hush_train_backward.restype = i32
     L.hush_train_commit.argtypes = [P, P, P]; L.hush_train_commit.restype = i32
     L.hush_train_a0.argtypes = [P, ctypes.POINTER(P)]; L.hush_train_a0.restype = i32
+    L.hush_train_set_a0_fixed.argtypes = [P, i32]; L.hush_train_set_a0_fixed.restype = i32
     L._train_declared = True
     return L
 
@@ -102,6 +103,10 @@ class DiffFrame:
 
     def commit(self, out) -> None:
         self._check(self._L.hush_train_commit(self._t, _ptr(out.contiguous()), _stream()))
+
+    def set_a0_fixed(self, fixed: bool) -> None:
+        """True: forwards take A0 as set (set_a0) instead of re-blending it from the temporal history"""
+        self._check(self._L.hush_train_set_a0_fixed(self._t, int(bool(fixed))))
 
     def a0_copy(self):
         """the prepared A0 as a tensor copy"""

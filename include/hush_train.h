@@ -11,9 +11,12 @@
  * checks) and without binary16 rounding between the passes:
  *   MLP (hush.js:348-366)  ->  mid1, mid2, mid3 a-trous (hush.js:431-476, 366, 753-754)
  *   fine a-trous on the temporal output  ->  level recombination (hush.js:560-620)  ->  TAA (hush.js:625-677)
- * The gradient flows to the 812 weights through the fine / mid edge-stopping strengths s and centre
- * weights c, the detail gain kappa and the coarse blend beta; the temporal history, the previous TAA
- * output and the MLP features are inputs of the frame (truncated backpropagation through time).  The
+ * The temporal output A0 is re-blended (hush.js:189-211) from the temporal pass's reprojected history
+ * and the previous frame's alpha / gamma, which the MLP recomputes from that frame's stored features with
+ * the same weights.  The gradient flows to the 812 weights through the fine / mid edge-stopping strengths
+ * s and centre weights c, the detail gain kappa and the coarse blend beta of this frame, and through the
+ * temporal blend to alpha / gamma of the previous frame (one step of backpropagation through time); the
+ * reprojected history itself, the previous TAA output and this frame's MLP features are inputs.  The
  * backward recomputes every a-trous weight instead of storing it.
  *
  * Buffers passed here are plain device arrays, row-major, 4 values per pixel (rgba), of the context's
@@ -38,8 +41,8 @@ int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out);
 void hush_train_destroy(hush_train_t *t);
 const char *hush_train_last_error(const hush_train_t *t);
 
-/* runs the temporal pass and the coarse level for this frame's inputs at the context's state, and copies
-   the temporal output A0 into the differentiable frame's element type */
+/* runs the temporal pass (also recording its blend inputs) and the coarse level for this frame's inputs at
+   the context's state, and copies the stored temporal output A0 into the element type */
 int32_t hush_train_prepare(hush_train_t *t, const hush_inputs *in, void *stream);
 
 /* forward of the prepared frame with `weights` (812 values) -> `out` (W x H x 4) */
@@ -62,8 +65,12 @@ int32_t hush_train_backward(hush_train_t *t, const void *weights, const void *gr
    counter advances and the reset flag clears */
 int32_t hush_train_commit(hush_train_t *t, const void *out, void *stream);
 
-/* the prepared A0 in the element type (W x H x 4), writable (input perturbation for gradient checks) */
+/* A0 in the element type (W x H x 4): after a forward, the re-blended temporal output */
 int32_t hush_train_a0(hush_train_t *t, void **ptr);
+
+/* fixed = 1: later forwards take A0 as it is in that buffer (writable, for input gradient checks) instead
+   of re-blending it, and the backward stops at A0; 0 (default): re-blend */
+int32_t hush_train_set_a0_fixed(hush_train_t *t, int32_t fixed);
 
 #ifdef __cplusplus
 }

@@ -1,7 +1,7 @@
 """Gradient checks of the differentiable frame (include/hush_train.h) on the GPU.
 
 * FP64 analytic gradients (hand-written backward kernels through TAA, recombination, the fine and mid
-  a-trous levels and the MLP) against FP64 central finite differences of the same forward, for all 812
+  a-trous levels, the temporal blend and the MLP of this and the previous frame) against FP64 central finite differences of the same forward, for all 812
   MLP weights and 200 temporal-output (A0) inputs, on 64x64 frames after three frames of history:
   beyond the difference quotient's own rounding noise (10 |loss| 2^-52 / step), relative error <= 1e-4
   for >= 99% of the checked values and <= 1e-2 for all of them (the L1 loss, the clip box and the clamps
@@ -74,7 +74,8 @@ def test_fd_weights_and_inputs_fp64():
     print("weights: raw rel err median", float(np.median(raw)), "max", float(raw.max()),
           "| beyond FD noise: p99", float(np.quantile(r, 0.99)), "max", float(r.max()))
     assert np.quantile(r, 0.99) <= 1e-4 and r.max() <= 1e-2, (np.argsort(-r)[:8], r[np.argsort(-r)[:8]])
-    # temporal-output inputs: 200 random texel channels of A0
+    # temporal-output inputs: 200 random texel channels of A0, held fixed (no re-blend)
+    fr.set_a0_fixed(True)
     a0 = fr.a0_copy()
     rng = np.random.default_rng(3)
     H, W = a0.shape[:2]
@@ -136,3 +137,31 @@ def test_adam_lowers_the_loss():
     assert np.mean(losses[-3:]) < np.mean(losses[:3]), losses
     blob = T.to_f16_blob(w.cpu().numpy())
     assert blob.size == 812
+
+
+def test_reblend_reproduces_the_temporal_output():
+    """with alpha / gamma as stored (history from hush_encode frames), the differentiable frame's re-blended
+    A0, rounded toward zero to binary16, is the temporal pass's stored A0 bit for bit"""
+    import torch
+    import hush_amd
+    from hush_amd import train as T
+    from hush_amd.scene import SceneGenerator
+    from oracle.hush_np import Q
+    from tests.parity import upload_frame
+    w_, h_ = 96, 64
+    gen = SceneGenerator("boxes", w_, h_, 5)
+    hs = hush_amd.Hush(0, {"width": w_, "height": h_})
+    surfs = None
+    for f in range(3):
+        surfs = upload_frame(gen.render(f), surfs)
+        hs.denoise(surfs)
+    fr = T.DiffFrame(hs)
+    g = gen.render(3, clean=True)
+    fr.prepare(T.upload(g))
+    wt = torch.tensor(T.weights_to_float(hush_amd.default_weights()), dtype=torch.float32, device="cuda")
+    fr.forward(wt)
+    a0 = fr.a0_copy().double().cpu().numpy()
+    ref = hs.surface("A0").download().astype(np.float64)
+    assert np.array_equal(Q(True).st(a0), ref), float(np.mean(Q(True).st(a0) != ref))
+    fr.destroy()
+    hs.destroy()
