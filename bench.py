@@ -79,8 +79,11 @@ def mlp_stage(hs, hip, W, H, reps=20):
 
 
 def psnr_check(precisions, w=640, h=360, frames=8):
-    """PSNR (tone-mapped, peak 1) of each MLP precision's output against the FP64 oracle, mean over an
-    8-frame sequence of the same scene type at 640x360"""
+    """Error of each MLP precision's output against the FP64 oracle over an 8-frame sequence of the same
+    scene type at 640x360: PSNR (tone-mapped, peak 1, mean over frames) against the FP64 ideal oracle (no
+    binary16 storage) and its change from the FP32 path; and the relative error of the HDR output
+    |gpu - ref| / max(|ref|, 1e-3) against the FP64 oracle with the same binary16 storage (which isolates
+    the arithmetic), largest and 99.9th percentile"""
     from oracle.ho import Oracle
     from tests.parity import psnr, upload_frame
     import hush_amd
@@ -89,16 +92,56 @@ def psnr_check(precisions, w=640, h=360, frames=8):
     gs = [gen.render(f) for f in range(frames)]
     o = Oracle(w, h, hush_amd.default_weights(), mode="fp64")
     refs = [o.denoise(g) for g in gs]
+    oq = Oracle(w, h, hush_amd.default_weights(), mode="fp64q")
+    refq = [oq.denoise(g)[..., :3] for g in gs]
     out = {}
     for p in precisions:
         hs = hush_amd.Hush(0, {"width": w, "height": h, "precision": p})
-        surfs, vals = None, []
-        for g, r in zip(gs, refs):
+        surfs, vals, rels = None, [], []
+        for g, r, rq in zip(gs, refs, refq):
             surfs = upload_frame(g, surfs)
-            vals.append(psnr(hs.denoise(surfs).download().astype(np.float32), r))
+            o_ = hs.denoise(surfs).download().astype(np.float64)[..., :3]
+            vals.append(psnr(o_.astype(np.float32), r))
+            rels.append((np.abs(o_ - rq) / np.maximum(np.abs(rq), 1e-3)).ravel())
         hs.destroy()
-        out[p] = round(float(np.mean(vals)), 3)
+        rel = np.concatenate(rels)
+        out[p] = {"psnr_db": round(float(np.mean(vals)), 3), "max_rel_vs_fp64_storage": round(float(rel.max()), 4),
+                  "p999_rel_vs_fp64_storage": round(float(np.quantile(rel, 0.999)), 5)}
+    for p in precisions:
+        out[p]["psnr_delta_vs_fp32_db"] = round(out[p]["psnr_db"] - out["fp32"]["psnr_db"], 3)
     return out
+
+
+def uhd_leg(hip, stream, steps=32, warmup=4, frames=8):
+    """3840x2160 (the same scene generator): ms/frame with the FP32 and the FP16-MFMA MLP, and the MLP
+    stage alone on that frame's feature rows"""
+    import hush_amd
+    from hush_amd.scene import SceneGenerator
+    W, H = 3840, 2160
+    gen = SceneGenerator("boxes", W, H, seed=2)
+    dev = [{k: hip.Surface(W, H).upload(g[k].view(np.float16)) for k in INPUT_KEYS} for g in (gen.render(f) for f in range(frames))]
+    res = {}
+    e0, e1 = hip.Event(), hip.Event()
+    for p in ("fp32", "fp16-mfma"):
+        hs = hush_amd.Hush(0, {"width": W, "height": H, "precision": p})
+        for i in range(warmup):
+            hs.encode(dev[i % frames], stream=stream.handle)
+        stream.synchronize()
+        e0.record(stream.handle)
+        for i in range(steps):
+            if i % frames == 0:
+                hs.reset()
+            hs.encode(dev[i % frames], stream=stream.handle)
+        e1.record(stream.handle)
+        e1.synchronize()
+        res[f"{p}_ms_per_frame"] = round(e0.elapsed_ms(e1) / steps, 5)
+        if p == "fp32":
+            res["mlp_stage"] = mlp_stage(hs, hip, W, H)
+        hs.destroy()
+    for d in dev:
+        for v in d.values():
+            v._buf.free()
+    return res
 
 
 def svgf_leg(dev, stream, hip, W, H, steps, warmup):
@@ -142,6 +185,78 @@ def svgf_leg(dev, stream, hip, W, H, steps, warmup):
     return {"ms_per_frame": round(ms, 5), "passes": passes, "iterations": 5}
 
 
+def train_leg(dev, W, H, reps=8):
+    """the differentiable frame (include/hush_train.h), FP32: (1) one 1920x1080 frame forward + loss +
+    backward + commit on resident inputs; (2) one training step of 8 sequences x 16 frames of 512x512
+    crops (the gradient of every frame accumulated)"""
+    import torch
+    import hush_amd
+    from hush_amd import train as T
+    from hush_amd.scene import SceneGenerator
+    w = torch.tensor(T.weights_to_float(hush_amd.default_weights()), dtype=torch.float32, device="cuda")
+    res = {}
+    hs = hush_amd.Hush(0, {"width": W, "height": H})
+    fr = T.DiffFrame(hs)
+    tgt = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+
+    def frame(f, prev):
+        fr.prepare(dev[f % len(dev)])
+        out = fr.forward(w)
+        l, g = fr.loss(out, tgt, prev, tgt if prev is not None else None, 0.5 if prev is not None else 0.0)
+        fr.backward(w, g)
+        fr.commit(out)
+        return out
+
+    prev = None
+    for f in range(3):
+        prev = frame(f, prev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for f in range(reps):
+        prev = frame(3 + f, prev)
+    e1.record()
+    e1.synchronize()
+    res["fwd_bwd_ms_per_frame_1080p"] = round(e0.elapsed_time(e1) / reps, 3)
+    fr.destroy()
+    hs.destroy()
+    # 8 sequences x 16 frames of 512x512
+    S, F, C = 8, 16, 512
+    gen = SceneGenerator("train", C, C, 123)
+    seq = [T.upload(gen.render(f, clean=True)) for f in range(F)]
+    tg = [torch.from_numpy(gen.render(f, clean=True)["clean"]).cuda() for f in range(F)]
+    ctx = [hush_amd.Hush(0, {"width": C, "height": C}) for _ in range(S)]
+    frs = [T.DiffFrame(c) for c in ctx]
+    gw = torch.zeros(812, dtype=torch.float32, device="cuda")
+
+    def step():
+        gw.zero_()
+        for c, d in zip(ctx, frs):
+            c.reset()
+            p_out = p_tg = None
+            for f in range(F):
+                d.prepare(seq[f])
+                out = d.forward(w)
+                l, g = d.loss(out, tg[f], p_out, p_tg, 0.5 if p_out is not None else 0.0)
+                gw.add_(d.backward(w, g))
+                d.commit(out)
+                p_out, p_tg = out, tg[f]
+
+    step()
+    torch.cuda.synchronize()
+    e0.record()
+    step()
+    e1.record()
+    e1.synchronize()
+    res["step_ms_8seq_x_16frames_512"] = round(e0.elapsed_time(e1), 2)
+    res["step_frames"] = S * F
+    for d in frs:
+        d.destroy()
+    for c in ctx:
+        c.destroy()
+    return res
+
+
 def cpu_baseline(frames_g, w, h, n_frames):
     """The oracle (FP32 mirror, C + OpenMP) timed on the host on the first `n_frames` frames."""
     from oracle.ho import Oracle
@@ -172,7 +287,7 @@ def main():
                     help="math policy: ieee (the oracle's operation-exact mirror) or fast")
     ap.add_argument("--cpu-frames", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the MLP-stage, FP16-MFMA frame, SVGF and PSNR legs")
+    ap.add_argument("--no-extras", action="store_true", help="skip the MLP-stage, FP16-MFMA frame, SVGF, training and PSNR legs")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run device-to-device copies of this size (the runtime's copy kernel calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
@@ -265,7 +380,11 @@ def main():
         extras["fp16_mfma_ms_per_frame"] = round(e0.elapsed_ms(e1) / args.steps, 5)
         hm.destroy()
         extras["svgf_1080p"] = svgf_leg(dev, stream, hip, W, H, min(args.steps, 64), args.warmup)
-        extras["psnr_db_vs_fp64_oracle_640x360"] = psnr_check(["fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma"])
+        extras["train_fp32"] = train_leg(dev, W, H)
+        print(f"[bench] train {extras['train_fp32']}", file=sys.stderr, flush=True)
+        extras["uhd_2160p"] = uhd_leg(hip, stream)
+        print(f"[bench] 2160p {extras['uhd_2160p']}", file=sys.stderr, flush=True)
+        extras["error_vs_fp64_oracle_640x360"] = psnr_check(["fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma"])
         print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
 
     if args.copy_probe_mb:

@@ -784,6 +784,81 @@ __global__ __launch_bounds__(NT) void k_mlp(const uint32_t *__restrict__ feat, f
   const MfmaPack *__restrict__ P = (const MfmaPack *)wq;
   const int lane = tid & 63, kb = lane >> 4, tl = lane & 15;
   const long base = ((long)blockIdx.x * NT + (tid & ~63));          // first row of this wave
+  if (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
+    // the wave's 64 rows as four independent 16-row tiles: every load first, then the four MFMA chains
+    // interleaved layer by layer, so the matrix cores overlap one tile's latency with the others
+    uint2 x0[4], x1[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const long r = base + 16 * g + tl;
+      const bool live = r < n;
+      const uint32_t *row = feat + (live ? r : 0) * 10;
+      x0[g] = live ? *(const uint2 *)(row + 2 * kb) : make_uint2(0, 0);                       // k = 4kb ..
+      x1[g] = (live && kb == 0) ? *(const uint2 *)(row + 8) : make_uint2(0, 0);              // k = 16 .. 19
+    }
+    f32x4 d[4];
+    uint2 hb[4];
+    if (MODE == MLP_FP16_MFMA) {
+      const f16x4 w0 = __builtin_bit_cast(f16x4, P->h[0][lane]), w1 = __builtin_bit_cast(f16x4, P->h[1][lane]);
+      const f16x4 w2 = __builtin_bit_cast(f16x4, P->h[2][lane]), w3 = __builtin_bit_cast(f16x4, P->h[3][lane]);
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w0, __builtin_bit_cast(f16x4, x0[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w1, __builtin_bit_cast(f16x4, x1[g]), d[g], 0, 0, 0);
+      }
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const f32x4 h = bias_relu(d[g], P->b1, kb);
+        hb[g] = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
+      }
+#pragma unroll
+      for (int g = 0; g < 4; g++)
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w2, __builtin_bit_cast(f16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const f32x4 h = bias_relu(d[g], P->b2, kb);
+        hb[g] = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
+      }
+#pragma unroll
+      for (int g = 0; g < 4; g++)
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w3, __builtin_bit_cast(f16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+    } else {
+      const s16x4 w0 = __builtin_bit_cast(s16x4, P->b[0][lane]), w1 = __builtin_bit_cast(s16x4, P->b[1][lane]);
+      const s16x4 w2 = __builtin_bit_cast(s16x4, P->b[2][lane]), w3 = __builtin_bit_cast(s16x4, P->b[3][lane]);
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const float2 a0 = unpack2(x0[g].x), a1 = unpack2(x0[g].y), c0 = unpack2(x1[g].x), c1 = unpack2(x1[g].y);
+        const uint2 y0 = make_uint2(pack_bf2(a0.x, a0.y), pack_bf2(a1.x, a1.y));
+        const uint2 y1 = make_uint2(pack_bf2(c0.x, c0.y), pack_bf2(c1.x, c1.y));
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w0, __builtin_bit_cast(s16x4, y0), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1, __builtin_bit_cast(s16x4, y1), d[g], 0, 0, 0);
+      }
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const f32x4 h = bias_relu(d[g], P->b1, kb);
+        hb[g] = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
+      }
+#pragma unroll
+      for (int g = 0; g < 4; g++)
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2, __builtin_bit_cast(s16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const f32x4 h = bias_relu(d[g], P->b2, kb);
+        hb[g] = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
+      }
+#pragma unroll
+      for (int g = 0; g < 4; g++)
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3, __builtin_bit_cast(s16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const long r = base + 16 * g + tl;
+      if (r < n && kb < 3)
+        *(float4 *)(out + r * 12 + 4 * kb) = make_float4(d[g][0] + P->b3[4 * kb], d[g][1] + P->b3[4 * kb + 1],
+                                                         d[g][2] + P->b3[4 * kb + 2], d[g][3] + P->b3[4 * kb + 3]);
+    }
+    return;
+  }
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     const long r = base + 16 * g + tl;
