@@ -66,6 +66,17 @@ __device__ __forceinline__ void stf(const TexF &t, int x, int y, float4 v) { t.p
 __device__ __forceinline__ float2 pair_of(float w) { return unpack2(__float_as_uint(w)); }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// f32 -> i32 toward zero, saturating (WGSL's conversion; the positive side stops at 2^31 - 128, the
+// largest float below 2^31), NaN -> 0.  A C++ cast of an out-of-range value (a motion vector that
+// overflowed binary16 to +-inf) is undefined, and the compiler then rewrites the clamps around it on the
+// assumption that nothing overflows.
+__device__ __forceinline__ int f2i(float x) {
+  const int r = (int)fminf(fmaxf(x, -2147483648.0f), 2147483520.0f);
+  return x == x ? r : 0;
+}
+// i32 addition with WGSL's two's-complement wrap-around (no signed-overflow assumption)
+__device__ __forceinline__ int addw(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
 __device__ __forceinline__ bool inside(int x, int y, int w, int h) { return x >= 0 && y >= 0 && x < w && y < h; }
 
 // WGSL min/max on non-NaN operands

@@ -142,21 +142,21 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         const float ppx = (float)px + mv.x, ppy = (float)py + mv.y;
         float bx = floorf(ppx), by = floorf(ppy);
         float fx = ppx - bx, fy = ppy - by;
-        int bix = (int)bx, biy = (int)by;
+        int bix = f2i(bx), biy = f2i(by);
         float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
-        int cx = (int)floorf(ppx + 0.5f), cy = (int)floorf(ppy + 0.5f);
+        int cx = f2i(floorf(ppx + 0.5f)), cy = f2i(floorf(ppy + 0.5f));
         // the 12 bilinear loads (decoded key, colour, moments) and par0, issued before any test
         float4 k4[4], c4[4], m4[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          const int qx = clampi(bix + (i & 1), 0, W - 1), qy = clampi(biy + (i >> 1), 0, H - 1);
+          const int qx = clampi(addw(bix, i & 1), 0, W - 1), qy = clampi(addw(biy, i >> 1), 0, H - 1);
           k4[i] = ldf(a.pkd, qx, qy); c4[i] = ld(a.pcol, qx, qy); m4[i] = ld(a.pmom, qx, qy);
         }
         const float4 pq = ld(a.ppar0, clampi(cx, 0, W - 1) >> 1, clampi(cy, 0, H - 1) >> 1);
         float sumW = 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          if (wts[i] > 0.0f && inside(bix + (i & 1), biy + (i >> 1), W, H) && check(k4[i], n, mv.w, mv.z)) {
+          if (wts[i] > 0.0f && inside(addw(bix, i & 1), addw(biy, i >> 1), W, H) && check(k4[i], n, mv.w, mv.z)) {
             mbits |= 1u << i;
             sumW += wts[i];
             hcx += wts[i] * c4[i].x; hcy += wts[i] * c4[i].y; hcz += wts[i] * c4[i].z;
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
           hcx = hcy = hcz = 0.0f;
           hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
           for (int j = 0; j < 9; j++) {
-            int qx = cx + j % 3 - 1, qy = cy + j / 3 - 1;
+            int qx = addw(cx, j % 3 - 1), qy = addw(cy, j / 3 - 1);
             if (inside(qx, qy, W, H) && check(ldf(a.pkd, qx, qy), n, mv.w, mv.z)) {
               n3 += 1.0f;
               float4 cc = ld(a.pcol, qx, qy), mm = ld(a.pmom, qx, qy);
@@ -212,8 +212,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       float4 A = make_float4(cr, cg, cb, svar + (tvar - svar) * mn(len1 / 4.0f, 1.0f));
       if (AUX) {
         float *x = a.aux + ((size_t)py * a.W + px) * TEMPORAL_AUX;
-        const int cxq = clampi((int)floorf((float)px + mv.x + 0.5f), 0, a.W - 1) >> 1;
-        const int cyq = clampi((int)floorf((float)py + mv.y + 0.5f), 0, a.H - 1) >> 1;
+        const int cxq = clampi(f2i(floorf((float)px + mv.x + 0.5f)), 0, a.W - 1) >> 1;
+        const int cyq = clampi(f2i(floorf((float)py + mv.y + 0.5f)), 0, a.H - 1) >> 1;
         const float v[TEMPORAL_AUX] = {has ? 1.0f : 0.0f, ce.x, ce.y, ce.z, mux, muy, muz, sdx, sdy, sdz, hcx, hcy, hcz,
                                        hm.x, hm.y, hm.w, l, svar, (float)(cyq * a.W2 + cxq), 0.0f};
 #pragma unroll
@@ -1209,7 +1209,7 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
     for (int j = 0; j < 4; j++) {
       int ox = j & 1, oy = j >> 1;
       float wb = (ox == 1 ? fx : 1.0f - fx) * (oy == 1 ? fy : 1.0f - fy);
-      float4 t = ld(a.prev, clampi((int)bx + ox, 0, a.W - 1), clampi((int)by + oy, 0, a.H - 1));
+      float4 t = ld(a.prev, clampi(addw(f2i(bx), ox), 0, a.W - 1), clampi(addw(f2i(by), oy), 0, a.H - 1));
       hx += wb * t.x; hy += wb * t.y; hz += wb * t.z;
     }
     float3 hh = to_ycocg(tm(make_float3(hx, hy, hz)));

@@ -531,7 +531,7 @@ __global__ __launch_bounds__(NTH) void tk_taa(TaaTArgs<T> a) {
       for (int j = 0; j < 4; j++) {
         const int ox = j & 1, oy = j >> 1;
         const T wb = (ox ? fx : T(1) - fx) * (oy ? fy : T(1) - fy);
-        const V4<T> t = ld4(a.prev, clampi((int)by + oy, 0, a.H - 1) * a.W + clampi((int)bx + ox, 0, a.W - 1));
+        const V4<T> t = ld4(a.prev, clampi(addw(f2i((float)by), oy), 0, a.H - 1) * a.W + clampi(addw(f2i((float)bx), ox), 0, a.W - 1));
         hist[0] += wb * t.x; hist[1] += wb * t.y; hist[2] += wb * t.z;
       }
       T th[3], eh;

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(NT) void k_sv_temporal(SvTemporalArgs a) {
   const float ir = rad.x / mx(alb.x, 1e-3f), ig = rad.y / mx(alb.y, 1e-3f), ib = rad.z / mx(alb.z, 1e-3f);
   const float l = lum(ir, ig, ib);
   const float ppx = ((float)p.x + 0.5f + mv.x) - 0.5f, ppy = ((float)p.y + 0.5f + mv.y) - 0.5f;
-  const int bx = (int)floorf(ppx), by = (int)floorf(ppy);
+  const int bx = f2i(floorf(ppx)), by = f2i(floorf(ppy));
   const float fx = ppx - floorf(ppx), fy = ppy - floorf(ppy);
   const float3 n = make_float3(nd.x, nd.y, nd.z);
   float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(NT) void k_sv_temporal(SvTemporalArgs a) {
   if (!nohist) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int qx = bx + (i & 1), qy = by + (i >> 1);
+      const int qx = addw(bx, i & 1), qy = addw(by, i >> 1);
       if (consistent(a, qx, qy, n, mv.w, mv.z) && wts[i] > 0.0f) {
         sumW += wts[i];
         const float4 pc = ld(a.pcol, qx, qy), pm = ld(a.pmom, qx, qy);
@@ -119,10 +119,10 @@ __global__ __launch_bounds__(NT) void k_sv_temporal(SvTemporalArgs a) {
       }
     }
     if (sumW < 0.01f) {
-      const int cx = (int)rintf(ppx), cy = (int)rintf(ppy);   // WGSL round(): half to even
+      const int cx = f2i(rintf(ppx)), cy = f2i(rintf(ppy));   // WGSL round(): half to even
       for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++) {
-          const int qx = cx + dx, qy = cy + dy;
+          const int qx = addw(cx, dx), qy = addw(cy, dy);
           if (consistent(a, qx, qy, n, mv.w, mv.z)) {
             sumW += 1.0f;
             const float4 pc = ld(a.pcol, qx, qy), pm = ld(a.pmom, qx, qy);
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(NT) void k_sv_resolve(SvResolveArgs a) {
     for (int j = 0; j < 4; j++) {
       const int ox = j & 1, oy = j >> 1;
       const float wb = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy);
-      const float4 t = ld(a.prev, clampi((int)bx + ox, 0, a.W - 1), clampi((int)by + oy, 0, a.H - 1));
+      const float4 t = ld(a.prev, clampi(addw(f2i(bx), ox), 0, a.W - 1), clampi(addw(f2i(by), oy), 0, a.H - 1));
       h0 += wb * t.x; h1 += wb * t.y; h2 += wb * t.z;
     }
     float3 hh = ycocg(tm3(make_float3(h0, h1, h2)));

@@ -269,3 +269,89 @@ def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256)
         if log and (step % max(1, steps // 20) == 0 or step == steps - 1):
             log(f"[train] step {step} loss {hist[-1][1]:.5f} ({time.perf_counter() - t0:.1f}s)")
     return weights.detach(), hist
+
+
+# ------------------------------------------------------------------------------------------------ validation
+def _tm(c):
+    c = np.maximum(np.asarray(c, np.float64)[..., :3], 0.0)
+    return c / (1.0 + (c[..., 0] * 0.2126 + c[..., 1] * 0.7152 + c[..., 2] * 0.0722))[..., None]
+
+
+def evaluate(weight_sets: dict, seeds=(1001, 1002), size=(640, 360), frames=24, kind="train", warm=4):
+    """PSNR (tone-mapped, peak 1) against the clean target and temporal flicker
+    mean |(tm(out_t) - tm(out_t-1)) - (tm(clean_t) - tm(clean_t-1))| over held-out seeded sequences, for
+    the Hush drop-in path with each weight set (name -> 812 binary16 values), the SVGF baseline and the
+    noisy input; the first `warm` frames of each sequence are not scored (history warm-up)."""
+    from . import SVGF
+    from .hip import Surface
+    from .scene import SceneGenerator
+    W, H = size
+    keys = ("radiance", "normalDepth", "motion", "material", "emission")
+    acc = {n: {"mse": [], "flick": []} for n in list(weight_sets) + ["svgf", "noisy"]}
+    for seed in seeds:
+        gen = SceneGenerator(kind, W, H, seed)
+        gs = [gen.render(f, clean=True) for f in range(frames)]
+        surfs = [{k: Surface(W, H).upload(g[k].view(np.float16)) for k in keys} for g in gs]
+        outs = {}
+        for name, w16 in weight_sets.items():
+            hs = Hush(0, {"width": W, "height": H, "weights": w16})
+            outs[name] = []
+            for f, s in enumerate(surfs):
+                if gen.is_cut(f):
+                    hs.reset()
+                outs[name].append(hs.denoise(s).download().astype(np.float64))
+            hs.destroy()
+        sv = SVGF(0, W, H)
+        outs["svgf"] = []
+        for f, s in enumerate(surfs):
+            if gen.is_cut(f):
+                sv.reset()
+            g = dict(s)
+            p = surfs[f - 1] if f > 0 else s
+            g["prevNormalDepth"], g["prevMotion"] = p["normalDepth"], p["motion"]
+            outs["svgf"].append(sv.encode(g).download().astype(np.float64))
+        sv.destroy()
+        outs["noisy"] = [g["radiance"].view(np.float16).astype(np.float64) + g["emission"].view(np.float16).astype(np.float64)
+                         for g in gs]
+        for name, seq in outs.items():
+            for f in range(warm, frames):
+                t, r = _tm(seq[f]), _tm(gs[f]["clean"])
+                acc[name]["mse"].append(float(np.mean((t - r) ** 2)))
+                if not gen.is_cut(f):
+                    dt = (t - _tm(seq[f - 1])) - (r - _tm(gs[f - 1]["clean"]))
+                    acc[name]["flick"].append(float(np.mean(np.abs(dt))))
+        for d in surfs:
+            for v in d.values():
+                v._buf.free()
+    return {n: {"psnr_db": round(float(10 * np.log10(1.0 / np.mean(v["mse"]))), 3),
+                "flicker": round(float(np.mean(v["flick"])), 6)} for n, v in acc.items()}
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="train the Hush MLP on seeded procedural sequences, then validate")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--sequences", type=int, default=4)
+    ap.add_argument("--frames", type=int, default=8)
+    ap.add_argument("--crop", type=int, nargs=2, default=(256, 256))
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init", choices=["random", "builtin"], default="random")
+    ap.add_argument("--out", default=None, help="write the trained weights (812 binary16, hush-weights.bin layout)")
+    ap.add_argument("--eval-frames", type=int, default=24)
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    w, hist = train(args.steps, args.sequences, args.frames, tuple(args.crop), lr=args.lr, seed=args.seed, init=args.init, log=log)
+    blob = to_f16_blob(w.cpu().numpy())
+    if args.out:
+        blob.tofile(args.out)
+    sets = {"trained": blob, "builtin": default_weights(), "random_init": to_f16_blob(random_weights(args.seed))}
+    res = evaluate(sets, frames=args.eval_frames)
+    print(json.dumps({"steps": args.steps, "sequences": args.sequences, "frames": args.frames, "crop": list(args.crop),
+                      "loss_first": hist[0][1], "loss_last": hist[-1][1], "validation": res}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

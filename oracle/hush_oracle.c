@@ -304,6 +304,17 @@ static void st4(real *s, int w, int x, int y, v4 v) {
   p[0] = st(v.x); p[1] = st(v.y); p[2] = st(v.z); p[3] = st(v.w);
 }
 static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* f32 -> i32 toward zero, saturating (WGSL's conversion; positive side at 2^31 - 128), NaN -> 0, and i32
+   addition with two's-complement wrap: the defined forms of the reprojection's index arithmetic (a motion
+   vector that overflowed binary16 is +-inf) */
+static int f2i(real x) {
+  if (x != x) return 0;
+  if (x < (real)-2147483648.0) return (int)(-2147483647 - 1);
+  if (x > (real)2147483520.0) return 2147483520;
+  return (int)x;
+}
+static int addw(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
 static int inside(int x, int y, int w, int h) { return x >= 0 && y >= 0 && x < w && y < h; }
 
 /* F planes: the 32 binary16 values of F0..F3 per half-res texel (pk = qf16 + RTZ store == st) */
@@ -393,13 +404,13 @@ static void pixel(ho_ctx *c, int px, int py, int cur, int prev) {
     real ppx = (real)px + mv.x, ppy = (real)py + mv.y;
     real bx = FLOOR(ppx), by = FLOOR(ppy);
     real fx = ppx - bx, fy = ppy - by;
-    int bix = (int)bx, biy = (int)by;
+    int bix = f2i(bx), biy = f2i(by);
     real sumW = 0;
     real wts[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
-    int cx = (int)FLOOR(ppx + RX(0.5)), cy = (int)FLOOR(ppy + RX(0.5));
+    int cx = f2i(FLOOR(ppx + RX(0.5))), cy = f2i(FLOOR(ppy + RX(0.5)));
     v4 pq = ld4(ppar, c->W2, clampi(cx, 0, W - 1) >> 1, clampi(cy, 0, H - 1) >> 1);
     for (int i = 0; i < 4; i++) {
-      int qx = bix + (i & 1), qy = biy + (i >> 1);
+      int qx = addw(bix, i & 1), qy = addw(biy, i >> 1);
       int cqx = clampi(qx, 0, W - 1), cqy = clampi(qy, 0, H - 1);
       v4 k4 = ld4(pkey, W, cqx, cqy);
       v4 c4 = ld4(pcol, W, cqx, cqy);
@@ -419,7 +430,7 @@ static void pixel(ho_ctx *c, int px, int py, int cur, int prev) {
       real n3 = 0;
       hc = mk3(0, 0, 0); hm = mk4(0, 0, 0, 0);
       for (int j = 0; j < 9; j++) {
-        int qx = cx + j % 3 - 1, qy = cy + j / 3 - 1;
+        int qx = addw(cx, j % 3 - 1), qy = addw(cy, j / 3 - 1);
         if (inside(qx, qy, W, H) && check(ld4(pkey, W, qx, qy), n, mv.w, mv.z)) {
           n3 += 1;
           v4 cc = ld4(pcol, W, qx, qy), mm = ld4(pmom, W, qx, qy);
@@ -957,7 +968,7 @@ static void pass_taa(ho_ctx *c, real *YC) {
         for (int j = 0; j < 4; j++) {
           int ox = j & 1, oy = j >> 1;
           real wb = (ox == 1 ? fx : 1 - fx) * (oy == 1 ? fy : 1 - fy);
-          v4 t = ld4(ptaa, W, clampi((int)bx + ox, 0, W - 1), clampi((int)by + oy, 0, H - 1));
+          v4 t = ld4(ptaa, W, clampi(addw(f2i(bx), ox), 0, W - 1), clampi(addw(f2i(by), oy), 0, H - 1));
           hist.x += wb * t.x; hist.y += wb * t.y; hist.z += wb * t.z;
         }
         v3 hh = to_ycocg(tm(hist));

@@ -73,6 +73,17 @@ static real maxr(real a, real b) { return a > b ? a : b; }
 static real lum3(const real *c) { return c[0] * RX(0.2126) + c[1] * RX(0.7152) + c[2] * RX(0.0722); }
 static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+/* f32 -> i32 toward zero, saturating (WGSL's conversion; positive side at 2^31 - 128), NaN -> 0, and i32
+   addition with two's-complement wrap: the defined forms of the reprojection's index arithmetic (a motion
+   vector that overflowed binary16 is +-inf) */
+static int f2i(real x) {
+  if (x != x) return 0;
+  if (x < (real)-2147483648.0) return (int)(-2147483647 - 1);
+  if (x > (real)2147483520.0) return 2147483520;
+  return (int)x;
+}
+static int addw(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
+
 typedef struct {
   real sigma_l, sigma_n, sigma_z, alpha_color, alpha_moments, taa_alpha;
   int taa, iterations;
@@ -162,13 +173,13 @@ static void temporal(sv_ctx *c, int cur, int prev) {
       real il[3] = {rad[0] / maxr(alb[0], RX(1e-3)), rad[1] / maxr(alb[1], RX(1e-3)), rad[2] / maxr(alb[2], RX(1e-3))};
       real l = lum3(il);
       real ppx = ((real)x + RX(0.5) + mv[0]) - RX(0.5), ppy = ((real)y + RX(0.5) + mv[1]) - RX(0.5);
-      int bx = (int)FLOOR(ppx), by = (int)FLOOR(ppy);
+      int bx = f2i(FLOOR(ppx)), by = f2i(FLOOR(ppy));
       real fx = ppx - FLOOR(ppx), fy = ppy - FLOOR(ppy);
       real sumW = 0, hc[3] = {0, 0, 0}, hm[3] = {0, 0, 0};
       real wts[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
       int nohist = c->reset;
       for (int i = 0; i < 4 && !nohist; i++) {
-        int qx = bx + (i & 1), qy = by + (i >> 1);
+        int qx = addw(bx, i & 1), qy = addw(by, i >> 1);
         if (consistent(c, qx, qy, nd, mv[3], mv[2]) && wts[i] > 0) {
           sumW += wts[i];
           const real *pc = at(c->hist_color[prev], W, qx, qy), *pm = at(c->hist_mom[prev], W, qx, qy);
@@ -176,10 +187,10 @@ static void temporal(sv_ctx *c, int cur, int prev) {
         }
       }
       if (sumW < RX(0.01) && !nohist) {
-        int cx = (int)RINT(ppx), cy = (int)RINT(ppy);
+        int cx = f2i(RINT(ppx)), cy = f2i(RINT(ppy));
         for (int dy = -1; dy <= 1; dy++)
           for (int dx = -1; dx <= 1; dx++) {
-            int qx = cx + dx, qy = cy + dy;
+            int qx = addw(cx, dx), qy = addw(cy, dy);
             if (consistent(c, qx, qy, nd, mv[3], mv[2])) {
               sumW += 1;
               const real *pc = at(c->hist_color[prev], W, qx, qy), *pm = at(c->hist_mom[prev], W, qx, qy);
@@ -344,7 +355,7 @@ static void resolve(sv_ctx *c, int src, int cur, int prev) {
         for (int j = 0; j < 4; j++) {
           int ox = j & 1, oy = j >> 1;
           real wb = (ox ? fx : 1 - fx) * (oy ? fy : 1 - fy);
-          const real *t = at((real *)ptaa, W, clampi((int)bx + ox, 0, W - 1), clampi((int)by + oy, 0, H - 1));
+          const real *t = at((real *)ptaa, W, clampi(addw(f2i(bx), ox), 0, W - 1), clampi(addw(f2i(by), oy), 0, H - 1));
           for (int k = 0; k < 3; k++) hist[k] += wb * t[k];
         }
         real th[3], hh[3];
