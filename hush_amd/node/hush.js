@@ -144,4 +144,46 @@ class Hush {
 }
 Hush.requiredLimits = REQUIRED_LIMITS;
 
-module.exports = { Hush, hiddenOf, initDevice, DeviceTexture, decodeWeights, defaultWeights, N_FEAT, N_OUT };
+const SVGF_INPUTS = ['radiance', 'normalDepth', 'motion', 'material', 'emission', 'prevNormalDepth', 'prevMotion'];
+
+// The SVGF baseline the reference exports beside Hush (src/denoise/svgf.js:273-379, src/index.d.ts:62-90):
+// new SVGF(device, width, height, { sigmaL = 4, sigmaN = 128, sigmaZ = 1, alphaColor = 0.2,
+// alphaMoments = 0.2, taaAlpha = 0.1, taa = true, iterations = 5 }); encode(g, timer, stream) with
+// g = { radiance, normalDepth, motion, material, emission, prevNormalDepth, prevMotion } (the caller
+// ping-pongs the previous frame's normalDepth / motion) returns this frame's output; `output` is the
+// reference's getter, taa[frame & 1] after the frame advanced.
+class SVGF {
+  constructor(device, width, height, opts = {}) {
+    this.device = device;
+    this._s = native.svgfCreate(device.ordinal, width, height, opts);
+    const i = native.svgfInfo(this._s);
+    this.iterations = i.iterations;
+    this.params = {
+      sigmaL: i.sigmaL, sigmaN: i.sigmaN, sigmaZ: i.sigmaZ, alphaColor: i.alphaColor, alphaMoments: i.alphaMoments,
+      taaAlpha: i.taaAlpha, taa: i.taa,
+    };
+  }
+
+  get width() { return native.svgfInfo(this._s).width; }
+  get height() { return native.svgfInfo(this._s).height; }
+  get frame() { return native.svgfInfo(this._s).frame; }
+  get output() { return new DeviceTexture(this.device, this.width, this.height, native.svgfOutput(this._s)); }
+
+  // reallocates for a new size (no-op if unchanged), frame 0, history reset (svgf.js:298-319)
+  resize(width, height) { native.svgfResize(this._s, width, height); }
+  reset() { native.svgfReset(this._s); }
+  destroy() {
+    if (this._s) native.svgfDestroy(this._s);
+    this._s = null;
+  }
+
+  encode(g, timer, stream) {
+    const surf = {};
+    for (const k of SVGF_INPUTS) surf[k] = g[k] && (g[k].surface || g[k]);
+    const out = native.svgfEncode(this._s, surf, stream || 0n, !!timer);
+    if (timer && typeof timer.record === 'function') timer.record(native.svgfPassTimes(this._s));
+    return new DeviceTexture(this.device, out.width, out.height, out);
+  }
+}
+
+module.exports = { Hush, SVGF, hiddenOf, initDevice, DeviceTexture, decodeWeights, defaultWeights, N_FEAT, N_OUT };

@@ -1,4 +1,4 @@
-// Node.js N-API addon over the Hush C ABI (include/hush.h).
+// Node.js N-API addon over the Hush C ABI (include/hush.h) and the SVGF baseline's (include/svgf.h).
 //
 // The JavaScript `Hush` class (hush_amd/node/hush.js) keeps the reference's API
 // (src/denoise/hush.js:725-875); this addon is the thin native layer under it: context calls map 1:1 onto
@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "hush.h"
+#include "svgf.h"
 
 namespace {
 
@@ -347,12 +348,205 @@ napi_value DeviceCount(napi_env env, napi_callback_info) {
   return r;
 }
 
+// ---- SVGF baseline (include/svgf.h; the reference's SVGF class, src/denoise/svgf.js:273-379)
+
+struct SvHolder {
+  svgf_t *s;
+};
+
+svgf_t *sv_of(napi_env env, napi_value v) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) return nullptr;
+  return static_cast<SvHolder *>(p)->s;
+}
+
+napi_value sv_throw(napi_env env, svgf_t *s, int code) {
+  napi_value err, msg, c;
+  const char *m = svgf_last_error(s);
+  napi_create_string_utf8(env, m ? m : "svgf error", NAPI_AUTO_LENGTH, &msg);
+  napi_create_error(env, nullptr, msg, &err);
+  napi_create_int32(env, code, &c);
+  napi_set_named_property(env, err, "code", c);
+  napi_throw(env, err);
+  return nullptr;
+}
+
+void finalize_sv(napi_env, void *data, void *) {
+  SvHolder *hd = static_cast<SvHolder *>(data);
+  if (!hd) return;
+  if (hd->s) svgf_destroy(hd->s);
+  delete hd;
+}
+
+// svgfCreate(device, width, height, opts {sigmaL, sigmaN, sigmaZ, alphaColor, alphaMoments, taaAlpha, taa, iterations})
+napi_value SvCreate(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!get_args(env, info, 4, a)) return nullptr;
+  svgf_options o;
+  svgf_default_options(&o);
+  napi_valuetype t;
+  napi_typeof(env, a[3], &t);
+  if (t == napi_object) {
+    struct { const char *k; float *f; } fl[] = {{"sigmaL", &o.sigma_l}, {"sigmaN", &o.sigma_n}, {"sigmaZ", &o.sigma_z},
+                                               {"alphaColor", &o.alpha_color}, {"alphaMoments", &o.alpha_moments},
+                                               {"taaAlpha", &o.taa_alpha}};
+    for (auto &e : fl) {
+      bool has = false;
+      napi_has_named_property(env, a[3], e.k, &has);
+      if (has) { napi_value v; napi_get_named_property(env, a[3], e.k, &v); *e.f = (float)num(env, v, *e.f); }
+    }
+    bool has = false;
+    napi_has_named_property(env, a[3], "iterations", &has);
+    if (has) { napi_value v; napi_get_named_property(env, a[3], "iterations", &v); o.iterations = (int32_t)num(env, v, 5); }
+    napi_has_named_property(env, a[3], "taa", &has);
+    if (has) {
+      napi_value v;
+      bool b = true;
+      napi_get_named_property(env, a[3], "taa", &v);
+      napi_coerce_to_bool(env, v, &v);
+      napi_get_value_bool(env, v, &b);
+      o.taa = b ? 1 : 0;
+    }
+  }
+  svgf_t *s = nullptr;
+  int r = svgf_create((int)num(env, a[0]), (int)num(env, a[1], 0), (int)num(env, a[2], 0), &o, &s);
+  if (r != HUSH_OK) return sv_throw(env, nullptr, r);
+  napi_value ext;
+  SvHolder *hd = new SvHolder{s};
+  if (napi_create_external(env, hd, finalize_sv, nullptr, &ext) != napi_ok) {
+    svgf_destroy(s);
+    delete hd;
+    return throw_err(env, "hush addon: napi_create_external failed");
+  }
+  return ext;
+}
+
+napi_value SvDestroy(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!get_args(env, info, 1, a)) return nullptr;
+  void *p = nullptr;
+  if (napi_get_value_external(env, a[0], &p) == napi_ok && p) {
+    SvHolder *hd = static_cast<SvHolder *>(p);
+    if (hd->s) svgf_destroy(hd->s);
+    hd->s = nullptr;
+  }
+  return nullptr;
+}
+
+napi_value SvResize(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return nullptr;
+  svgf_t *s = sv_of(env, a[0]);
+  int r = svgf_resize(s, (int)num(env, a[1]), (int)num(env, a[2]));
+  if (r != HUSH_OK) return sv_throw(env, s, r);
+  return nullptr;
+}
+
+napi_value SvReset(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!get_args(env, info, 1, a)) return nullptr;
+  svgf_t *s = sv_of(env, a[0]);
+  int r = svgf_reset(s);
+  if (r != HUSH_OK) return sv_throw(env, s, r);
+  return nullptr;
+}
+
+napi_value SvInfo(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!get_args(env, info, 1, a)) return nullptr;
+  svgf_t *s = sv_of(env, a[0]);
+  int32_t w = 0, hh = 0;
+  svgf_get_size(s, &w, &hh);
+  svgf_options o{};
+  svgf_get_options(s, &o);
+  napi_value r, v;
+  napi_create_object(env, &r);
+  napi_create_int32(env, w, &v); napi_set_named_property(env, r, "width", v);
+  napi_create_int32(env, hh, &v); napi_set_named_property(env, r, "height", v);
+  napi_create_double(env, (double)svgf_frame(s), &v); napi_set_named_property(env, r, "frame", v);
+  napi_create_int32(env, o.iterations, &v); napi_set_named_property(env, r, "iterations", v);
+  napi_create_double(env, o.sigma_l, &v); napi_set_named_property(env, r, "sigmaL", v);
+  napi_create_double(env, o.sigma_n, &v); napi_set_named_property(env, r, "sigmaN", v);
+  napi_create_double(env, o.sigma_z, &v); napi_set_named_property(env, r, "sigmaZ", v);
+  napi_create_double(env, o.alpha_color, &v); napi_set_named_property(env, r, "alphaColor", v);
+  napi_create_double(env, o.alpha_moments, &v); napi_set_named_property(env, r, "alphaMoments", v);
+  napi_create_double(env, o.taa_alpha, &v); napi_set_named_property(env, r, "taaAlpha", v);
+  napi_get_boolean(env, o.taa != 0, &v); napi_set_named_property(env, r, "taa", v);
+  return r;
+}
+
+// svgfEncode(s, g, stream: BigInt|0, timing: bool) -> output surface
+napi_value SvEncode(napi_env env, napi_callback_info info) {
+  napi_value a[4];
+  if (!get_args(env, info, 4, a)) return nullptr;
+  svgf_t *s = sv_of(env, a[0]);
+  static const char *keys[7] = {"radiance", "normalDepth", "motion", "material", "emission", "prevNormalDepth", "prevMotion"};
+  svgf_inputs in;
+  hush_surface *dst[7] = {&in.radiance, &in.normal_depth, &in.motion, &in.material, &in.emission, &in.prev_normal_depth,
+                          &in.prev_motion};
+  for (int k = 0; k < 7; k++) {
+    napi_value v;
+    bool has = false;
+    napi_has_named_property(env, a[1], keys[k], &has);
+    if (has) napi_get_named_property(env, a[1], keys[k], &v);
+    else napi_get_undefined(env, &v);
+    if (!read_surface(env, v, dst[k])) return throw_err(env, "hush addon: bad surface object");
+  }
+  uint64_t stream = 0;
+  bool lossless = false, timing = false;
+  napi_valuetype t;
+  napi_typeof(env, a[2], &t);
+  if (t == napi_bigint) napi_get_value_bigint_uint64(env, a[2], &stream, &lossless);
+  napi_typeof(env, a[3], &t);
+  if (t == napi_boolean) napi_get_value_bool(env, a[3], &timing);
+  svgf_set_timing(s, timing ? 1 : 0);
+  hush_surface out;
+  int r = svgf_encode(s, &in, (void *)(uintptr_t)stream, &out);
+  if (r != HUSH_OK) return sv_throw(env, s, r);
+  return make_surface(env, out);
+}
+
+napi_value SvOutput(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!get_args(env, info, 1, a)) return nullptr;
+  svgf_t *s = sv_of(env, a[0]);
+  hush_surface out;
+  int r = svgf_output(s, &out);
+  if (r != HUSH_OK) return sv_throw(env, s, r);
+  return make_surface(env, out);
+}
+
+napi_value SvPassTimes(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!get_args(env, info, 1, a)) return nullptr;
+  svgf_t *s = sv_of(env, a[0]);
+  float ms[8];
+  const char *labels[8];
+  int32_t n = 0;
+  int r = svgf_pass_times(s, ms, labels, &n);
+  if (r != HUSH_OK) return sv_throw(env, s, r);
+  napi_value arr;
+  napi_create_array_with_length(env, (size_t)n, &arr);
+  for (int32_t i = 0; i < n; i++) {
+    napi_value o, l, m;
+    napi_create_object(env, &o);
+    napi_create_string_utf8(env, labels[i], NAPI_AUTO_LENGTH, &l);
+    napi_create_double(env, ms[i], &m);
+    napi_set_named_property(env, o, "label", l);
+    napi_set_named_property(env, o, "ms", m);
+    napi_set_element(env, arr, (uint32_t)i, o);
+  }
+  return arr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback fn; } fns[] = {
       {"hiddenOf", HiddenOf}, {"create", Create}, {"destroy", Destroy}, {"resize", Resize}, {"reset", Reset},
       {"setExposure", SetExposure}, {"setPrecision", SetPrecision}, {"info", Info}, {"denoise", Denoise},
       {"output", Output}, {"passTimes", PassTimes}, {"alloc", Alloc}, {"free", Free}, {"upload", Upload},
       {"download", Download}, {"deviceCount", DeviceCount},
+      {"svgfCreate", SvCreate}, {"svgfDestroy", SvDestroy}, {"svgfResize", SvResize}, {"svgfReset", SvReset},
+      {"svgfInfo", SvInfo}, {"svgfEncode", SvEncode}, {"svgfOutput", SvOutput}, {"svgfPassTimes", SvPassTimes},
   };
   for (auto &f : fns) {
     napi_value v;

@@ -30,4 +30,8 @@ assert.throws(() => new Hush(dev, { height: 8 }), /^Error: Hush: bad size undefi
 const native = require(path.join(__dirname, '..', '..', 'hush_amd', 'node', 'hush_napi.node'));
 assert.throws(() => native.create(0, 8, 8, 1, new Uint16Array(7), 0), /hush weights: 7 f16 values fit no MLP size/);
 assert.throws(() => native.create(0, -1, 8, 1, defaultWeights(), 0), /Hush: bad size -1x8/);
+// SVGF: options and size are checked before any device call
+const { SVGF } = require(path.join(__dirname, '..', '..', 'hush_amd', 'node', 'hush.js'));
+assert.throws(() => new SVGF(dev, 0, 8), /SVGF: bad size 0x8/);
+assert.throws(() => new SVGF(dev, 8, 8, { iterations: 6 }), /6 a-trous iterations/);
 console.log('api_errors: ok');
