@@ -182,6 +182,14 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
     }
   }
   for (int i = 0; i < 9; i++) (void)hipEventCreate(&h->ev[i]);
+  e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);    // default priority
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    hip_fail(nullptr, e, "side stream");
+    hush_destroy(h);
+    return HUSH_ERR_DEVICE;
+  }
   int r = hush_resize(h, width, height);
   if (r != HUSH_OK) {
     std::snprintf(g_err, sizeof(g_err), "%s", h->err);
@@ -202,6 +210,9 @@ extern "C" void hush_destroy(hush_t *h) {
   if (h->d_wscale) (void)hipFree(h->d_wscale);
   for (int i = 0; i < 9; i++)
     if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+  if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+  if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+  if (h->side) (void)hipStreamDestroy(h->side);
   delete h;
 }
 
@@ -351,12 +362,28 @@ extern "C" int32_t hush_encode(hush_t *h, const hush_inputs *in, void *stream, h
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(h->device);
   if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
-  for (int p = 0; p < 8; p++) {
-    if (h->timing) (void)hipEventRecord(h->ev[p], s);
-    e = hush_run_pass(h, p, in, s);
-    if (e != hipSuccess) return hip_fail(h, e, kPassLabel[p]);
+  if (h->timing) {                       // timed frames run the passes one after another
+    for (int p = 0; p < 8; p++) {
+      (void)hipEventRecord(h->ev[p], s);
+      e = hush_run_pass(h, p, in, s);
+      if (e != hipSuccess) return hip_fail(h, e, kPassLabel[p]);
+    }
+    (void)hipEventRecord(h->ev[8], s);
+    h->ev_ready = true;
+  } else {
+    // temporal; then coarse1 -> coarse2 on the side stream beside infer -> mid2 -> mid3; fine + taa after both
+    // (the next frame's coarse passes start after its temporal pass, which follows this frame's fine pass)
+    static const int main_passes[3] = {1, 2, 3};
+    e = hush_run_pass(h, 0, in, s);
+    if (e == hipSuccess) e = hipEventRecord(h->fork_ev, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->fork_ev, 0);
+    for (int p = 4; p <= 5 && e == hipSuccess; p++) e = hush_run_pass(h, p, in, h->side);
+    if (e == hipSuccess) e = hipEventRecord(h->join_ev, h->side);
+    for (int k = 0; k < 3 && e == hipSuccess; k++) e = hush_run_pass(h, main_passes[k], in, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, h->join_ev, 0);
+    for (int p = 6; p <= 7 && e == hipSuccess; p++) e = hush_run_pass(h, p, in, s);
+    if (e != hipSuccess) return hip_fail(h, e, "hush_encode");
   }
-  if (h->timing) { (void)hipEventRecord(h->ev[8], s); h->ev_ready = true; }
   const int cur = (int)(h->frame & 1);
   h->reset = false;
   h->frame++;

@@ -37,6 +37,10 @@ struct hush_ctx {
   bool timing = false;
   bool ev_ready = false;
   hipEvent_t ev[9] = {};
+  // the coarse level (passes 4, 5) depends only on the temporal pass: it runs on `side`, concurrently with
+  // infer / mid2 / mid3, forked and joined with events (untimed frames)
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   char err[512] = {0};
 };
 
