@@ -117,7 +117,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         cnt += t.w;
       }
       cnt = mx(cnt, 1.0f);
-      const float mux = F::div(s1x, cnt), muy = F::div(s1y, cnt), muz = F::div(s1z, cnt);
+      const float rcnt = F::rcp(cnt);
+      const float mux = F::divs(s1x, cnt, rcnt), muy = F::divs(s1y, cnt, rcnt), muz = F::divs(s1z, cnt, rcnt);
       const float slm = lum(mux, muy, muz);
       float svx = 0.0f, svy = 0.0f, svz = 0.0f, svar = 0.0f;
 #pragma unroll
@@ -129,9 +130,9 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         svx += t.w * dx * dx; svy += t.w * dy * dy; svz += t.w * dz * dz;
         svar += t.w * dl * dl;
       }
-      const float sdx = F::sqrt(mx(F::div(svx, cnt), 1e-12f)), sdy = F::sqrt(mx(F::div(svy, cnt), 1e-12f)),
-                  sdz = F::sqrt(mx(F::div(svz, cnt), 1e-12f));
-      svar = F::div(svar, cnt);
+      const float sdx = F::sqrt(mx(F::divs(svx, cnt, rcnt), 1e-12f)), sdy = F::sqrt(mx(F::divs(svy, cnt, rcnt), 1e-12f)),
+                  sdz = F::sqrt(mx(F::divs(svz, cnt, rcnt), 1e-12f));
+      svar = F::divs(svar, cnt, rcnt);
       // reprojection (hush.js:144-187)
       bool has = false;
       float hcx = 0.0f, hcy = 0.0f, hcz = 0.0f;
@@ -164,8 +165,9 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
           }
         }
         if (sumW >= 0.01f) {
-          hcx = F::div(hcx, sumW); hcy = F::div(hcy, sumW); hcz = F::div(hcz, sumW);
-          hm.x = F::div(hm.x, sumW); hm.y = F::div(hm.y, sumW); hm.z = F::div(hm.z, sumW); hm.w = F::div(hm.w, sumW);
+          const float rs = F::rcp(sumW);
+          hcx = F::divs(hcx, sumW, rs); hcy = F::divs(hcy, sumW, rs); hcz = F::divs(hcz, sumW, rs);
+          hm.x = F::divs(hm.x, sumW, rs); hm.y = F::divs(hm.y, sumW, rs); hm.z = F::divs(hm.z, sumW, rs); hm.w = F::divs(hm.w, sumW, rs);
           v = sumW; has = true;
         } else {
           float n3 = 0.0f;
@@ -181,8 +183,9 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
             }
           }
           if (n3 > 0.0f) {
-            hcx = F::div(hcx, n3); hcy = F::div(hcy, n3); hcz = F::div(hcz, n3);
-            hm.x = F::div(hm.x, n3); hm.y = F::div(hm.y, n3); hm.z = F::div(hm.z, n3); hm.w = F::div(hm.w, n3);
+            const float rn = F::rcp(n3);
+            hcx = F::divs(hcx, n3, rn); hcy = F::divs(hcy, n3, rn); hcz = F::divs(hcz, n3, rn);
+            hm.x = F::divs(hm.x, n3, rn); hm.y = F::divs(hm.y, n3, rn); hm.z = F::divs(hm.z, n3, rn); hm.w = F::divs(hm.w, n3, rn);
             v = 0.1f; has = true;
             mbits |= 0x10;
           }
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         nmy = hm.y + (l * l - hm.y) * aa;
         nfast = hm.z + (l - hm.z) * mx(1.0f / 6.0f, F::rcp(len1));
         float lh = lum(hcx, hcy, hcz);
-        lag = F::div(fabsf(slm - lh), F::sqrt(mx(F::div(svar, cnt), 1e-12f)) + 0.02f * lh + 1e-3f);
+        lag = F::div(fabsf(slm - lh), F::sqrt(mx(F::divs(svar, cnt, rcnt), 1e-12f)) + 0.02f * lh + 1e-3f);
       } else {
         v = 0.0f;
       }
@@ -315,7 +318,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   }
   var4 = F::div(var4, mx(n4, 1.0f));
   const float ln = mx(F::sqrt(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
-  const float3 Nn = make_float3(F::div(Nx, ln), F::div(Ny, ln), F::div(Nz, ln));
+  const float rln = F::rcp(ln);
+  const float3 Nn = make_float3(F::divs(Nx, ln, rln), F::divs(Ny, ln, rln), F::divs(Nz, ln, rln));
   float zmax = -1e30f, zmin = 1e30f, mind = 1e30f;
 #pragma unroll
   for (int j = 0; j < 4; j++)
@@ -398,7 +402,8 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
         sv += ww * ww * cq.w;
       }
     }
-  return make_float4(F::div(scx, sw), F::div(scy, sw), F::div(scz, sw), F::div(sv, sw * sw));
+  const float rsw = F::rcp(sw);
+  return make_float4(F::divs(scx, sw, rsw), F::divs(scy, sw, rsw), F::divs(scz, sw, rsw), F::div(sv, sw * sw));
 }
 
 // decoded half-res guide from an F3 texel: octDec(oct normal), block depth (hush.js:337)
@@ -1004,9 +1009,11 @@ __device__ __forceinline__ void down(const TexU &F2, const TexF &GH, int QX, int
   if (N == 0.0f) { col = make_float4(0.0f, 0.0f, 0.0f, 0.0f); g = make_float4(0.0f, 0.0f, -1.0f, 0.0f); return; }
   using F = Math<false>;   // stored level values: IEEE under both policies (they set the slope sl)
   float ln = mx(F::sqrt(nx * nx + ny * ny + nz * nz), 1e-6f);
-  float2 o = oct_enc_m<FAST>(make_float3(F::div(nx, ln), F::div(ny, ln), F::div(nz, ln)));
-  col = make_float4(qf16(F::div(clx, N)), qf16(F::div(cly, N)), qf16(F::div(clz, N)), qf16(F::div(F::div(vr, N), N)));
-  g = make_float4(qf16(o.x), qf16(o.y), qf16(F::div(z, N)), 0.0f);
+  const float rln = F::rcp(ln), rN = F::rcp(N);
+  float2 o = oct_enc_m<FAST>(make_float3(F::divs(nx, ln, rln), F::divs(ny, ln, rln), F::divs(nz, ln, rln)));
+  col = make_float4(qf16(F::divs(clx, N, rN)), qf16(F::divs(cly, N, rN)), qf16(F::divs(clz, N, rN)),
+                    qf16(F::divs(F::divs(vr, N, rN), N, rN)));
+  g = make_float4(qf16(o.x), qf16(o.y), qf16(F::divs(z, N, rN)), 0.0f);
 }
 
 template <bool FAST>
@@ -1108,9 +1115,10 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   }
   float ix = fx, iy = fy, iz = fz;
   if (swh > 1e-4f) {
-    ix = F::div(uBx, swh) + pr.w * (fx - F::div(uAx, swh));
-    iy = F::div(uBy, swh) + pr.w * (fy - F::div(uAy, swh));
-    iz = F::div(uBz, swh) + pr.w * (fz - F::div(uAz, swh));
+    const float rh = F::rcp(swh);
+    ix = F::divs(uBx, swh, rh) + pr.w * (fx - F::divs(uAx, swh, rh));
+    iy = F::divs(uBy, swh, rh) + pr.w * (fy - F::divs(uAy, swh, rh));
+    iz = F::divs(uBz, swh, rh) + pr.w * (fz - F::divs(uAz, swh, rh));
   }
   // quarter-res coarse level (hush.js:604-619)
   const float cpx = ((float)px + 0.5f) / 4.0f - 0.5f, cpy = ((float)py + 0.5f) / 4.0f - 0.5f;
@@ -1128,7 +1136,10 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
     uCx += w * cc.x; uCy += w * cc.y; uCz += w * cc.z;
   }
   float upx = ix, upy = iy, upz = iz;
-  if (swq > 1e-4f) { upx = F::div(uCx, swq); upy = F::div(uCy, swq); upz = F::div(uCz, swq); }
+  if (swq > 1e-4f) {
+    const float rq = F::rcp(swq);
+    upx = F::divs(uCx, swq, rq); upy = F::divs(uCy, swq, rq); upz = F::divs(uCz, swq, rq);
+  }
   ix = ix + (upx - ix) * pr.z;
   iy = iy + (upy - iy) * pr.z;
   iz = iz + (upz - iz) * pr.z;
@@ -1142,12 +1153,14 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
 // Karis tone map and its inverse, IEEE under both policies: itm(tm(x)) lands on x exactly for most texels
 // (the reset frame), where any negative error would drop a round-toward-zero store by a full binary16 ulp.
 __device__ __forceinline__ float3 tm(float3 c) {
-  float d = 1.0f + lum(c.x, c.y, c.z);
-  return make_float3(c.x / d, c.y / d, c.z / d);
+  using M = Math<false>;
+  const float d = 1.0f + lum(c.x, c.y, c.z), r = 1.0f / d;
+  return make_float3(M::divs(c.x, d, r), M::divs(c.y, d, r), M::divs(c.z, d, r));
 }
 __device__ __forceinline__ float3 itm(float3 c) {
-  float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f);
-  return make_float3(c.x / d, c.y / d, c.z / d);
+  using M = Math<false>;
+  const float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f), r = 1.0f / d;
+  return make_float3(M::divs(c.x, d, r), M::divs(c.y, d, r), M::divs(c.z, d, r));
 }
 __device__ __forceinline__ float3 to_ycocg(float3 c) {
   return make_float3(c.x * 0.25f + c.y * 0.5f + c.z * 0.25f, c.x * 0.5f - c.z * 0.5f, -c.x * 0.25f + c.y * 0.5f - c.z * 0.25f);
@@ -1186,7 +1199,8 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
     float4 s = sX[c0 + (j / 3 - 1) * SW + (j % 3 - 1)];
     m1x += s.x; m1y += s.y; m1z += s.z;
   }
-  m1x = F::div(m1x, 9.0f); m1y = F::div(m1y, 9.0f); m1z = F::div(m1z, 9.0f);
+  const float r9 = 1.0f / 9.0f;                                // RN(1/9)
+  m1x = F::divs(m1x, 9.0f, r9); m1y = F::divs(m1y, 9.0f, r9); m1z = F::divs(m1z, 9.0f, r9);
   float vx = 0.0f, vy = 0.0f, vz = 0.0f;
 #pragma unroll
   for (int j = 0; j < 9; j++) {
@@ -1194,8 +1208,8 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
     float dx = s.x - m1x, dy = s.y - m1y, dz = s.z - m1z;
     vx += dx * dx; vy += dy * dy; vz += dz * dz;
   }
-  const float sdx = F::sqrt(mx(F::div(vx, 9.0f), 1e-12f)), sdy = F::sqrt(mx(F::div(vy, 9.0f), 1e-12f)),
-              sdz = F::sqrt(mx(F::div(vz, 9.0f), 1e-12f));
+  const float sdx = F::sqrt(mx(F::divs(vx, 9.0f, r9), 1e-12f)), sdy = F::sqrt(mx(F::divs(vy, 9.0f, r9), 1e-12f)),
+              sdz = F::sqrt(mx(F::divs(vz, 9.0f, r9), 1e-12f));
   const float4 mv = ld(a.mv, px, py);
   const float szx = (float)a.W, szy = (float)a.H;
   const float ux = ((float)px + 0.5f + mv.x) / szx, uy = ((float)py + 0.5f + mv.y) / szy;

@@ -1,0 +1,20 @@
+"""The kernels' shared-reciprocal division (Math<false>::divs, hush_amd/csrc/hush_device.h) is the correctly
+rounded quotient: the same IEEE operations in C (hardware FMA) against IEEE division on 10^7 random operand
+pairs over wide exponent ranges and every quotient of small integers."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="no C compiler")
+def test_shared_reciprocal_division_is_correctly_rounded(tmp_path):
+    exe = str(tmp_path / "divs_check")
+    # -ffp-contract=off: the product a*y stays a separate rounding, as in the kernels
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-mfma", os.path.join(HERE, "divs_check.c"), "-o", exe, "-lm"],
+                   check=True)
+    r = subprocess.run([exe, "10000000"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip() == "0", r.stdout + r.stderr
