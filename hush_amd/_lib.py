@@ -1,4 +1,4 @@
-"""ctypes declarations of the C ABI in include/hush.h, include/svgf.h and include/hush_train.h (libhush.so, built in-tree by hush_amd/build.py).
+"""ctypes declarations of the C ABI in include/*.h (libhush.so, built in-tree by hush_amd/build.py).
 
 Loading fails loudly when the library is missing: there is no CPU fallback for the denoiser.
 """
@@ -25,6 +25,8 @@ EXPORTS = [
     # include/hush_train.h
     "hush_train_create", "hush_train_destroy", "hush_train_last_error", "hush_train_prepare", "hush_train_forward",
     "hush_train_loss", "hush_train_backward", "hush_train_commit", "hush_train_a0", "hush_train_set_a0_fixed",
+    # include/hush_scene.h
+    "hush_scene_create", "hush_scene_destroy", "hush_scene_last_error", "hush_scene_render",
 ]
 
 
@@ -36,6 +38,13 @@ class hush_surface(ctypes.Structure):
 class hush_inputs(ctypes.Structure):
     _fields_ = [("radiance", hush_surface), ("normal_depth", hush_surface), ("motion", hush_surface),
                 ("material", hush_surface), ("emission", hush_surface)]
+
+
+class hush_scene_frame(ctypes.Structure):
+    _fields_ = [("cams", ctypes.c_void_p), ("prims", ctypes.c_void_p), ("off", ctypes.c_void_p), ("lst", ctypes.c_void_p),
+                ("n_prims", ctypes.c_int32), ("n_list", ctypes.c_int32), ("seed", ctypes.c_uint64), ("frame", ctypes.c_int32),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("x0", ctypes.c_int32), ("y0", ctypes.c_int32),
+                ("w", ctypes.c_int32), ("h", ctypes.c_int32)]
 
 
 class svgf_options(ctypes.Structure):
@@ -108,5 +117,9 @@ def load() -> ctypes.CDLL:
     L.svgf_debug_surface.argtypes = [P, i32, S]; L.svgf_debug_surface.restype = i32
     L.svgf_debug_set_state.argtypes = [P, ctypes.c_int64, i32]; L.svgf_debug_set_state.restype = i32
     L.svgf_debug_run_pass.argtypes = [P, i32, SI, P]; L.svgf_debug_run_pass.restype = i32
+    L.hush_scene_create.argtypes = [i32, ctypes.POINTER(P)]; L.hush_scene_create.restype = i32
+    L.hush_scene_destroy.argtypes = [P]; L.hush_scene_destroy.restype = None
+    L.hush_scene_last_error.argtypes = [P]; L.hush_scene_last_error.restype = ctypes.c_char_p
+    L.hush_scene_render.argtypes = [P, ctypes.POINTER(hush_scene_frame), I, P, sz, P]; L.hush_scene_render.restype = i32
     _lib = L
     return L

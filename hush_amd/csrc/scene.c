@@ -167,6 +167,8 @@ EXPORT void hs_destroy(hs_scene *s) {
 }
 
 EXPORT int hs_num_prims(const hs_scene *s) { return s->n; }
+/* the per-scene seed every random number of the scene hashes (what a renderer elsewhere needs) */
+EXPORT unsigned long long hs_seed(const hs_scene *s) { return s->seed; }
 EXPORT int hs_cut_interval(const hs_scene *s) { return s->kind == HS_TRAIN ? 48 : 0; }
 /* 1 when `frame` starts a new camera segment (the caller resets the denoiser's history) */
 EXPORT int hs_is_cut(const hs_scene *s, int frame) { return s->kind == HS_TRAIN && frame % 48 == 0; }
@@ -283,11 +285,25 @@ static void put(uint16_t *dst, size_t i, double a, double b, double c, double d)
  * Outputs are w*h*4 binary16 arrays (row-major, tight pitch); `clean` (optional) receives w*h*4 floats:
  * the expected radiance albedo * irradiance * E[noise] plus emission (the noise-free target).
  */
-EXPORT int hs_render(const hs_scene *s, int frame, int x0, int y0, int w, int h, uint16_t *rad, uint16_t *nd,
-                     uint16_t *mvo, uint16_t *mat, uint16_t *emi, float *clean) {
-  if (!s || w <= 0 || h <= 0 || x0 < 0 || y0 < 0 || x0 + w > s->W || y0 + h > s->H) return -1;
+/* per-frame set-up shared by the CPU renderer and the GPU one (hush_amd/csrc/scene_gpu.hip): cameras,
+   primitive positions and motions, and per-16x16-tile primitive lists of the crop window (conservative
+   projected bounds) */
+typedef struct {
+  camera cam, pcam;
+  d3 *pos, *dpos;
+  int tw, th;
+  int *off, *lst;
+} frame_setup;
+
+static void setup_free(frame_setup *fs) {
+  free(fs->pos); free(fs->dpos); free(fs->off); free(fs->lst);
+}
+
+static void setup_frame(const hs_scene *s, int frame, int x0, int y0, int w, int h, frame_setup *fs) {
   const double f = frame;
-  camera cam = make_camera(s, f), pcam = make_camera(s, f - 1);
+  fs->cam = make_camera(s, f);
+  fs->pcam = make_camera(s, f - 1);
+  const camera cam = fs->cam;
   const int n = s->n;
   d3 *pos = (d3 *)malloc(sizeof(d3) * (n > 0 ? n : 1));
   d3 *dpos = (d3 *)malloc(sizeof(d3) * (n > 0 ? n : 1));
@@ -295,7 +311,6 @@ EXPORT int hs_render(const hs_scene *s, int frame, int x0, int y0, int w, int h,
     pos[i] = prim_pos(&s->p[i], f);
     dpos[i] = sub(pos[i], prim_pos(&s->p[i], f - 1));
   }
-  /* screen-tile lists of primitives (conservative projected bounds) */
   int tw = (w + TILE - 1) / TILE, th = (h + TILE - 1) / TILE;
   int *cnt = (int *)calloc((size_t)tw * th + 1, sizeof(int));
   int *bx = (int *)malloc(sizeof(int) * 4 * (n > 0 ? n : 1));
@@ -337,6 +352,60 @@ EXPORT int hs_render(const hs_scene *s, int frame, int x0, int y0, int w, int h,
         int t = ty * tw + tx;
         lst[off[t] + cnt[t]++] = i;
       }
+  free(cnt); free(bx);
+  fs->pos = pos; fs->dpos = dpos; fs->tw = tw; fs->th = th; fs->off = off; fs->lst = lst;
+}
+
+/* The set-up of frame `frame` for a renderer elsewhere (the GPU one).  Layouts, all doubles unless noted:
+     cams   2 x 14: current then previous camera, each pos.xyz fwd.xyz right.xyz up.xyz tx ty
+     prims  n x HS_PRIM_DOUBLES: type, id, pos.xyz, dpos.xyz, h.xyz, albedo.xyz, roughness, emit.xyz
+     off    (ntiles + 1) ints, lst off[ntiles] ints (ntiles = ceil(w/16) * ceil(h/16), row-major tiles)
+   Returns the list length; when it exceeds lst_cap nothing is written to lst (call again with room). */
+#define HS_PRIM_DOUBLES 20
+EXPORT int hs_frame_setup(const hs_scene *s, int frame, int x0, int y0, int w, int h, double *cams, double *prims,
+                          int *off, int *lst, int lst_cap) {
+  if (!s || w <= 0 || h <= 0 || x0 < 0 || y0 < 0 || x0 + w > s->W || y0 + h > s->H) return -1;
+  frame_setup fs;
+  setup_frame(s, frame, x0, y0, w, h, &fs);
+  const camera *cs[2] = {&fs.cam, &fs.pcam};
+  for (int k = 0; k < 2; k++) {
+    const camera *c = cs[k];
+    double *o = cams + 14 * k;
+    o[0] = c->pos.x; o[1] = c->pos.y; o[2] = c->pos.z; o[3] = c->fwd.x; o[4] = c->fwd.y; o[5] = c->fwd.z;
+    o[6] = c->right.x; o[7] = c->right.y; o[8] = c->right.z; o[9] = c->up.x; o[10] = c->up.y; o[11] = c->up.z;
+    o[12] = c->tx; o[13] = c->ty;
+  }
+  for (int i = 0; i < s->n; i++) {
+    const prim *q = &s->p[i];
+    double *o = prims + HS_PRIM_DOUBLES * i;
+    o[0] = q->type; o[1] = q->id;
+    o[2] = fs.pos[i].x; o[3] = fs.pos[i].y; o[4] = fs.pos[i].z;
+    o[5] = fs.dpos[i].x; o[6] = fs.dpos[i].y; o[7] = fs.dpos[i].z;
+    o[8] = q->h.x; o[9] = q->h.y; o[10] = q->h.z;
+    o[11] = q->albedo.x; o[12] = q->albedo.y; o[13] = q->albedo.z; o[14] = q->rough;
+    o[15] = q->emit.x; o[16] = q->emit.y; o[17] = q->emit.z; o[18] = 0; o[19] = 0;
+  }
+  const int nt = fs.tw * fs.th, len = fs.off[nt];
+  memcpy(off, fs.off, sizeof(int) * ((size_t)nt + 1));
+  if (len <= lst_cap) memcpy(lst, fs.lst, sizeof(int) * (size_t)(len > 0 ? len : 0));
+  setup_free(&fs);
+  return len;
+}
+
+/*
+ * Renders frame `frame` into the crop window [x0, x0+w) x [y0, y0+h) of the scene's full W x H image.
+ * Outputs are w*h*4 binary16 arrays (row-major, tight pitch); `clean` (optional) receives w*h*4 floats:
+ * the expected radiance albedo * irradiance * E[noise] plus emission (the noise-free target).
+ */
+EXPORT int hs_render(const hs_scene *s, int frame, int x0, int y0, int w, int h, uint16_t *rad, uint16_t *nd,
+                     uint16_t *mvo, uint16_t *mat, uint16_t *emi, float *clean) {
+  if (!s || w <= 0 || h <= 0 || x0 < 0 || y0 < 0 || x0 + w > s->W || y0 + h > s->H) return -1;
+  frame_setup fs;
+  setup_frame(s, frame, x0, y0, w, h, &fs);
+  const camera cam = fs.cam, pcam = fs.pcam;
+  const d3 *pos = fs.pos, *dpos = fs.dpos;
+  const int tw = fs.tw;
+  const int *off = fs.off, *lst = fs.lst;
   const double noise_mean = exp(0.5) * (1.0 + 0.02 * 49.0);
 #pragma omp parallel for schedule(dynamic, 4)
   for (int yy = 0; yy < h; yy++) {
@@ -423,6 +492,6 @@ EXPORT int hs_render(const hs_scene *s, int frame, int x0, int y0, int w, int h,
       put(emi, idx, er, eg, eb, 0);
     }
   }
-  free(pos); free(dpos); free(cnt); free(bx); free(off); free(lst);
+  setup_free(&fs);
   return 0;
 }

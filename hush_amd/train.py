@@ -226,12 +226,13 @@ def target_of(g: dict, dtype):
 
 
 def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256), scene_size=(640, 360), lr: float = 1e-3,
-          lam: float = 0.5, seed: int = 0, init: str = "random", log=print):
+          lam: float = 0.5, seed: int = 0, init: str = "random", log=print, gpu_scenes: bool = True):
     """Adam over seeded procedural sequences ('train' scenes: 300 primitives, 30% moving, camera cuts).
     Each step: `sequences` crops x `frames` frames; the gradient of every frame accumulates (the history
-    enters each frame as an input).  Returns (weights tensor, [(step, loss)])."""
+    enters each frame as an input).  With gpu_scenes the crops and their clean targets are rendered on the
+    GPU straight into device tensors.  Returns (weights tensor, [(step, loss)])."""
     import torch
-    from .scene import SceneGenerator
+    from .scene import GpuSceneGenerator, SceneGenerator
     w0 = random_weights(seed) if init == "random" else weights_to_float(default_weights())
     weights = torch.tensor(w0, dtype=torch.float32, device="cuda", requires_grad=True)
     opt = torch.optim.Adam([weights], lr=lr)
@@ -243,7 +244,8 @@ def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256)
         total = 0.0
         for s in range(sequences):
             rs = np.random.default_rng([seed, step, s])
-            gen = SceneGenerator("train", scene_size[0], scene_size[1], int(rs.integers(1 << 30)))
+            gseed = int(rs.integers(1 << 30))
+            gen = (GpuSceneGenerator if gpu_scenes else SceneGenerator)("train", scene_size[0], scene_size[1], gseed)
             x0 = int(rs.integers(0, scene_size[0] - cw + 1))
             y0 = int(rs.integers(0, scene_size[1] - ch + 1))
             f0 = int(rs.integers(0, 96))
@@ -251,12 +253,17 @@ def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256)
             fr = DiffFrame(hs)
             prev_out = prev_tgt = None
             for f in range(frames):
-                g = gen.render(f0 + f, crop=(x0, y0, cw, ch), clean=True)
+                if gpu_scenes:
+                    ins = {k: torch.empty((ch, cw, 4), dtype=torch.float16, device="cuda") for k in GpuSceneGenerator.KEYS}
+                    tgt = torch.empty((ch, cw, 4), dtype=torch.float32, device="cuda")
+                    gen.render(f0 + f, ins, crop=(x0, y0, cw, ch), clean=tgt)
+                else:
+                    g = gen.render(f0 + f, crop=(x0, y0, cw, ch), clean=True)
+                    ins, tgt = upload(g), target_of(g, torch.float32)
                 if gen.is_cut(f0 + f):
                     hs.reset()
-                fr.prepare(upload(g))
+                fr.prepare(ins)
                 out = frame_apply(weights, fr)
-                tgt = target_of(g, torch.float32)
                 loss = loss_apply(out, tgt, prev_out, prev_tgt, lam if prev_out is not None else 0.0, fr)
                 (loss / (sequences * frames)).backward()
                 total += float(loss)
@@ -264,6 +271,8 @@ def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256)
                 prev_out, prev_tgt = out.detach(), tgt
             fr.destroy()
             hs.destroy()
+            if gpu_scenes:
+                gen.close()
         opt.step()
         hist.append((step, total / (sequences * frames)))
         if log and (step % max(1, steps // 20) == 0 or step == steps - 1):
