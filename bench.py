@@ -97,7 +97,8 @@ def psnr_check(precisions, w=640, h=360, frames=8):
     refq = [oq.denoise(g)[..., :3] for g in gs]
     out = {}
     for p in precisions:
-        hs = hush_amd.Hush(0, {"width": w, "height": h, "precision": p})
+        prec, comp = (p[:-len("+compensated")], True) if p.endswith("+compensated") else (p, False)
+        hs = hush_amd.Hush(0, {"width": w, "height": h, "precision": prec, "compensated": comp})
         surfs, vals, rels, tmax = None, [], [], 0.0
         for g, r, rq in zip(gs, refs, refq):
             surfs = upload_frame(g, surfs)
@@ -416,7 +417,8 @@ def main():
         print(f"[bench] train {extras['train_fp32']}", file=sys.stderr, flush=True)
         extras["uhd_2160p"] = uhd_leg(hip, stream)
         print(f"[bench] 2160p {extras['uhd_2160p']}", file=sys.stderr, flush=True)
-        extras["error_vs_fp64_oracle_640x360"] = psnr_check(["fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma"])
+        extras["error_vs_fp64_oracle_640x360"] = psnr_check(["fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma", "fp32+compensated",
+                                                              "fp16-mfma+compensated"])
         print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
 
     if args.copy_probe_mb:

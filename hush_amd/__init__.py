@@ -116,6 +116,7 @@ class Hush:
         weights = opts.get("weights")
         precision = opts.get("precision", "f16" if opts.get("f16") else "fp32")
         ieee = bool(opts.get("ieee_math", True))
+        compensated = bool(opts.get("compensated", False))
         limits = getattr(device, "limits", None)
         if limits is not None:
             for k, v in self.requiredLimits.items():
@@ -137,6 +138,8 @@ class Hush:
         self.device = ordinal
         if not ieee:
             self.ieee_math = False
+        if compensated:
+            self.compensated = True
         self._timer_on = False
 
     # ---- helpers
@@ -203,6 +206,16 @@ class Hush:
     @ieee_math.setter
     def ieee_math(self, on: bool) -> None:
         self._check(self._L.hush_set_ieee_math(self._h, int(bool(on))))
+
+    @property
+    def compensated(self) -> bool:
+        """True: Neumaier-compensated moment sums in the temporal statistics (a precision form; False, the
+        default, is the reference's plain FP32 summation)."""
+        return bool(self._L.hush_get_compensated(self._h))
+
+    @compensated.setter
+    def compensated(self, on: bool) -> None:
+        self._check(self._L.hush_set_compensated(self._h, int(bool(on))))
 
     def denoise(self, inputs, timer: HushTimer | None = None, stream: int | None = None) -> Surface:
         """Runs one frame (hush.js:813-822) and returns the output surface (rgba16float, linear HDR)."""

@@ -255,6 +255,13 @@ extern "C" int32_t hush_set_ieee_math(hush_t *h, int32_t on) {
 }
 extern "C" int32_t hush_get_ieee_math(const hush_t *h) { return h ? (h->ieee ? 1 : 0) : -1; }
 
+extern "C" int32_t hush_set_compensated(hush_t *h, int32_t on) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_compensated: null context");
+  h->compensated = on != 0;
+  return HUSH_OK;
+}
+extern "C" int32_t hush_get_compensated(const hush_t *h) { return h ? (h->compensated ? 1 : 0) : -1; }
+
 extern "C" const char *hush_last_error(const hush_t *h) { return h ? h->err : g_err; }
 
 static Tex in_tex(const hush_surface &s) { return Tex{(uint2 *)s.ptr, (int)(s.pitch / 8)}; }
@@ -292,7 +299,7 @@ hipError_t hush_run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream_t
       a.pkd = h->kd[prev]; a.pcol = t[S_HIST0 + prev]; a.pmom = t[S_MOM0 + prev]; a.ppar0 = t[S_PAR0_0 + prev];
       a.outA = t[S_A0]; a.outMom = t[S_MOM0 + cur]; a.outKey = t[S_KEY0 + cur]; a.outKD = h->kd[cur];
       a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.F3 = h->F[3]; a.outGH = h->gh;
-      a.mask = h->mask_on ? h->mask : nullptr; a.mask_pitch = h->mask_pitch; a.aux = h->train_aux;
+      a.mask = h->mask_on ? h->mask : nullptr; a.mask_pitch = h->mask_pitch; a.aux = h->train_aux; a.compensated = h->compensated ? 1 : 0;
       a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.reset = h->reset ? 1 : 0; a.E = h->exposure;
       return launch_temporal(a, !h->ieee, s);
     }

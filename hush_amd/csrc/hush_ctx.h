@@ -33,7 +33,8 @@ struct hush_ctx {
   int mask_pitch = 0;
   bool mask_on = false;
   float *train_aux = nullptr;      // set by the differentiable frame around its temporal pass
-  bool ieee = true;                // IEEE math policy (the oracle's operation-exact mirror); false = FAST
+  bool ieee = true;
+  bool compensated = false;        // compensated (Neumaier) moment sums in the temporal statistics                // IEEE math policy (the oracle's operation-exact mirror); false = FAST
   bool timing = false;
   bool ev_ready = false;
   hipEvent_t ev[9] = {};

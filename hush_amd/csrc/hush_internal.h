@@ -31,6 +31,7 @@ struct TemporalArgs {
   uint8_t *mask;                     // optional per-pixel reprojection bits (debug export)
   int mask_pitch;
   float *aux;                        // optional temporal-blend inputs for the differentiable frame (W x H x TEMPORAL_AUX)
+  int compensated;                   // Neumaier-compensated moment sums
   int W, H, W2, H2, reset;
   float E;                           // exposure
 };

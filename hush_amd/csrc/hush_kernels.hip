@@ -37,7 +37,20 @@ enum { P0X, P0Y, P0Z, P0W, P1X, P1Y, P1Z, P1W, P2Z, P2W, NSX, NSY, NSZ, P3X, P3Y
 // AUX (training only): also writes the per-pixel inputs of the temporal blend (TEMPORAL_AUX floats each:
 // history flag, illumination, 3x3 mean and deviation, reprojected colour and moments, luminance, spatial
 // variance, the half-res texel of alpha / gamma), from which the differentiable frame re-blends A0.
-template <bool AUX>
+// Running sum; KAHAN: Neumaier-compensated (the rounding error of every addition is carried separately and
+// added back at the end), the compensated-moment form of the temporal statistics.
+template <bool KAHAN> struct Sum {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float x) {
+    if (!KAHAN) { s += x; return; }
+    const float t = s + x;
+    c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ float get() const { return KAHAN ? s + c : s; }
+};
+
+template <bool AUX, bool KAHAN>
 __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   constexpr bool FAST = false;
   using F = Math<FAST>;
@@ -109,27 +122,31 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       const float4 ce = sI[c0];
       const float l = sL[c0];
       // 3x3 stats (hush.js:125-142)
-      float s1x = 0.0f, s1y = 0.0f, s1z = 0.0f, cnt = 0.0f;
+      Sum<KAHAN> a1x, a1y, a1z;
+      float cnt = 0.0f;
 #pragma unroll
       for (int j = 0; j < 9; j++) {
         float4 t = sI[c0 + (j / 3 - 1) * SW + (j % 3 - 1)];
-        s1x += t.x * t.w; s1y += t.y * t.w; s1z += t.z * t.w;
+        a1x.add(t.x * t.w); a1y.add(t.y * t.w); a1z.add(t.z * t.w);
         cnt += t.w;
       }
+      const float s1x = a1x.get(), s1y = a1y.get(), s1z = a1z.get();
       cnt = mx(cnt, 1.0f);
       const float rcnt = F::rcp(cnt);
       const float mux = F::divs(s1x, cnt, rcnt), muy = F::divs(s1y, cnt, rcnt), muz = F::divs(s1z, cnt, rcnt);
       const float slm = lum(mux, muy, muz);
-      float svx = 0.0f, svy = 0.0f, svz = 0.0f, svar = 0.0f;
+      Sum<KAHAN> avx, avy, avz, avl;
 #pragma unroll
       for (int j = 0; j < 9; j++) {
         int idx = c0 + (j / 3 - 1) * SW + (j % 3 - 1);
         float4 t = sI[idx];
         float dx = t.x - mux, dy = t.y - muy, dz = t.z - muz;
         float dl = sL[idx] - slm;
-        svx += t.w * dx * dx; svy += t.w * dy * dy; svz += t.w * dz * dz;
-        svar += t.w * dl * dl;
+        avx.add(t.w * dx * dx); avy.add(t.w * dy * dy); avz.add(t.w * dz * dz);
+        avl.add(t.w * dl * dl);
       }
+      const float svx = avx.get(), svy = avy.get(), svz = avz.get();
+      float svar = avl.get();
       const float sdx = F::sqrt(mx(F::divs(svx, cnt, rcnt), 1e-12f)), sdy = F::sqrt(mx(F::divs(svy, cnt, rcnt), 1e-12f)),
                   sdz = F::sqrt(mx(F::divs(svz, cnt, rcnt), 1e-12f));
       svar = F::divs(svar, cnt, rcnt);
@@ -259,8 +276,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   const int X = (x0 >> 1) + bxl, Y = (y0 >> 1) + byl;
   if (X >= a.W2 || Y >= a.H2) return;
   float nv = 0.0f;
-  float S0[4] = {0, 0, 0, 0}, S1[4] = {0, 0, 0, 0}, S3[4] = {0, 0, 0, 0}, AC[4] = {0, 0, 0, 0};
-  float Nx = 0.0f, Ny = 0.0f, Nz = 0.0f, MV = 0.0f, Z = 0.0f;
+  Sum<KAHAN> qS0[4], qS1[4], qS3[4], qAC[4], qNx, qNy, qNz, qMV, qZ;
   bool ok[4];
   float la[4], zs[4], nsx[4], nsy[4], nsz[4];
 #pragma unroll
@@ -274,11 +290,15 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     nsx[j] = sP[NSX][t]; nsy[j] = sP[NSY][t]; nsz[j] = sP[NSZ][t];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      S0[c] += sP[P0X + c][t]; S1[c] += sP[P1X + c][t]; S3[c] += sP[P3X + c][t]; AC[c] += sP[P4X + c][t];
+      qS0[c].add(sP[P0X + c][t]); qS1[c].add(sP[P1X + c][t]); qS3[c].add(sP[P3X + c][t]); qAC[c].add(sP[P4X + c][t]);
     }
-    Nx += nsx[j]; Ny += nsy[j]; Nz += nsz[j];
-    MV += sP[P2W][t]; Z += sP[P2Z][t];
+    qNx.add(nsx[j]); qNy.add(nsy[j]); qNz.add(nsz[j]);
+    qMV.add(sP[P2W][t]); qZ.add(sP[P2Z][t]);
   }
+  float S0[4], S1[4], S3[4], AC[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) { S0[c] = qS0[c].get(); S1[c] = qS1[c].get(); S3[c] = qS3[c].get(); AC[c] = qAC[c].get(); }
+  float Nx = qNx.get(), Ny = qNy.get(), Nz = qNz.get(), MV = qMV.get(), Z = qZ.get();
   if (nv == 0.0f) {
     stu(a.F0, X, Y, make_uint4(0, 0, 0, 0));
     stu(a.F1, X, Y, make_uint4(0, 0, 0, 0));
@@ -294,29 +314,30 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   for (int c = 0; c < 4; c++) { S0[c] *= k; S1[c] *= k; S3[c] *= k; AC[c] *= k; }
   Nx *= k; Ny *= k; Nz *= k; MV *= k; Z *= k;
   const float Lacc = S0[0];
-  float VA = 0.0f;
+  Sum<KAHAN> qVA;
 #pragma unroll
-  for (int j = 0; j < 4; j++) if (ok[j]) VA += (la[j] - Lacc) * (la[j] - Lacc);
-  VA *= k;
+  for (int j = 0; j < 4; j++) if (ok[j]) qVA.add((la[j] - Lacc) * (la[j] - Lacc));
+  float VA = qVA.get() * k;
   // 4x4 neighbourhood (block +-1) of the exposed, truncated luminance (hush.js:253-265)
-  float s1 = 0.0f, n4 = 0.0f;
+  Sum<KAHAN> q1;
+  float n4 = 0.0f;
   const int b0 = (2 * byl) * SW + 2 * bxl;      // tile index of pixel (2*bxl - 1, 2*byl - 1) incl. apron
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     int idx = b0 + (j >> 2) * SW + (j & 3);
     float w = sI[idx].w;
-    s1 += w * E * qf16(sL[idx]);
+    q1.add(w * E * qf16(sL[idx]));
     n4 += w;
   }
-  const float mu4 = F::div(s1, mx(n4, 1.0f));
-  float var4 = 0.0f;
+  const float mu4 = F::div(q1.get(), mx(n4, 1.0f));
+  Sum<KAHAN> q4;
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     int idx = b0 + (j >> 2) * SW + (j & 3);
     float d = sQ[idx] - mu4;
-    var4 += sI[idx].w * d * d;
+    q4.add(sI[idx].w * d * d);
   }
-  var4 = F::div(var4, mx(n4, 1.0f));
+  float var4 = F::div(q4.get(), mx(n4, 1.0f));
   const float ln = mx(F::sqrt(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
   const float rln = F::rcp(ln);
   const float3 Nn = make_float3(F::divs(Nx, ln, rln), F::divs(Ny, ln, rln), F::divs(Nz, ln, rln));
@@ -1263,8 +1284,13 @@ static dim3 grid_for(int w, int h) { return dim3(((w + TW - 1) / TW) * ((h + TH 
 #define K_TAA(F) k_taa<F>
 
 hipError_t launch_temporal(const TemporalArgs &a, bool, hipStream_t s) {
-  if (a.aux) hipLaunchKernelGGL(k_temporal<true>, grid_for(a.W, a.H), dim3(NT), 0, s, a);
-  else hipLaunchKernelGGL(k_temporal<false>, grid_for(a.W, a.H), dim3(NT), 0, s, a);
+  if (a.compensated) {
+    if (a.aux) hipLaunchKernelGGL((k_temporal<true, true>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_temporal<false, true>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
+  } else {
+    if (a.aux) hipLaunchKernelGGL((k_temporal<true, false>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_temporal<false, false>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
+  }
   return hipGetLastError();
 }
 

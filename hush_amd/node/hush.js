@@ -74,7 +74,7 @@ function initDevice(ordinal = 0) {
 const REQUIRED_LIMITS = Object.freeze({ maxStorageTexturesPerShaderStage: 8 });
 
 class Hush {
-  // options: { width, height, exposure = 1, weights = built-in, f16 = false, precision = 'fp32' }
+  // options: { width, height, exposure = 1, weights = built-in, f16 = false, precision = 'fp32', compensated = false }
   constructor(device, options = {}) {
     const { width, height, exposure = 1, weights, f16 = false } = options;
     for (const [k, v] of Object.entries(REQUIRED_LIMITS)) {
@@ -90,6 +90,7 @@ class Hush {
     const precision = options.precision !== undefined ? options.precision : (f16 ? 'f16' : 'fp32');
     this.f16 = precision === 'f16';
     this._h = native.create(device.ordinal, width, height, exposure, u16, PRECISION[precision]);
+    if (options.compensated) native.setCompensated(this._h, true);   // compensated moment sums (a precision form)
     this._exposure = exposure;
     this._timed = false;
   }

@@ -199,6 +199,18 @@ napi_value SetPrecision(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+napi_value SetCompensated(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return nullptr;
+  hush_t *h = ctx_of(env, a[0]);
+  bool on = false;
+  napi_coerce_to_bool(env, a[1], &a[1]);
+  napi_get_value_bool(env, a[1], &on);
+  int r = hush_set_compensated(h, on ? 1 : 0);
+  if (r != HUSH_OK) return throw_status(env, h, r);
+  return nullptr;
+}
+
 napi_value Info(napi_env env, napi_callback_info info) {
   napi_value a[1];
   if (!get_args(env, info, 1, a)) return nullptr;
@@ -542,7 +554,7 @@ napi_value SvPassTimes(napi_env env, napi_callback_info info) {
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback fn; } fns[] = {
       {"hiddenOf", HiddenOf}, {"create", Create}, {"destroy", Destroy}, {"resize", Resize}, {"reset", Reset},
-      {"setExposure", SetExposure}, {"setPrecision", SetPrecision}, {"info", Info}, {"denoise", Denoise},
+      {"setExposure", SetExposure}, {"setPrecision", SetPrecision}, {"setCompensated", SetCompensated}, {"info", Info}, {"denoise", Denoise},
       {"output", Output}, {"passTimes", PassTimes}, {"alloc", Alloc}, {"free", Free}, {"upload", Upload},
       {"download", Download}, {"deviceCount", DeviceCount},
       {"svgfCreate", SvCreate}, {"svgfDestroy", SvDestroy}, {"svgfResize", SvResize}, {"svgfReset", SvReset},

@@ -110,6 +110,12 @@ int32_t hush_get_precision(const hush_t *h);
 int32_t hush_set_ieee_math(hush_t *h, int32_t on);
 int32_t hush_get_ieee_math(const hush_t *h);
 
+/* Compensated moment sums (no counterpart in the reference): 1 = the temporal pass's neighbourhood and
+   block sums (3x3 mean and variances, 4x4 luminance moments, 2x2 block means) carry their rounding error
+   (Neumaier summation); 0 (default) = plain FP32 sums in the reference's order, the drop-in. */
+int32_t hush_set_compensated(hush_t *h, int32_t on);
+int32_t hush_get_compensated(const hush_t *h);
+
 /* denoise(encoder, inputs) (hush.js:813-822): validates the five inputs, enqueues the 8 passes on
    `stream` and returns the output surface in *out (may be NULL). */
 int32_t hush_denoise(hush_t *h, const hush_inputs *in, void *stream, hush_surface *out);

@@ -170,3 +170,32 @@ def test_decoded_guides_match_reference_surfaces(ieee):
         assert np.array_equal(cgd[..., :3], _oct_dec_np(cg[..., 0], cg[..., 1]))
         assert np.array_equal(cgd[..., 3], cg[..., 2].astype(np.float32))
     hs.destroy()
+
+
+def test_compensated_sums_error_vs_fp64():
+    """Compensated moment sums (a precision form): the output stays within the drop-in tolerance of the
+    FP32 oracle, and its error against the FP64 oracle with the same binary16 storage is no larger than the
+    plain FP32 path's (storage rounding dominates both)"""
+    import hush_amd
+    w, h, frames = 160, 96, 6
+    gen = _gen("boxes", w, h, 31)
+    gs = [gen.render(f) for f in range(frames)]
+    oq = Oracle(w, h, hush_amd.default_weights(), mode="fp64q")
+    refs = [oq.denoise(g) for g in gs]
+    o32 = Oracle(w, h, hush_amd.default_weights())
+    r32 = [o32.denoise(g) for g in gs]
+    errs = {}
+    for comp in (False, True):
+        hs = hush_amd.Hush(0, {"width": w, "height": h, "compensated": comp})
+        assert hs.compensated == comp
+        surfs, e = None, []
+        for g, r, r2 in zip(gs, refs, r32):
+            surfs = upload_frame(g, surfs)
+            out = hs.denoise(surfs).download().astype(np.float64)
+            e.append(np.abs(out[..., :3] - r[..., :3]) / np.maximum(np.abs(r[..., :3]), 1e-3))
+            st = compare(out[..., :3].astype(np.float32), r2[..., :3])
+            assert st["frac_rel_gt_1e-3"] <= 1e-2, (comp, st)
+        hs.destroy()
+        errs[comp] = float(np.quantile(np.concatenate([x.ravel() for x in e]), 0.999))
+    print("p99.9 relative error vs FP64 (binary16 storage): plain", errs[False], "compensated", errs[True])
+    assert errs[True] <= errs[False] * 1.1 + 1e-6, errs
