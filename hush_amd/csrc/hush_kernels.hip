@@ -742,12 +742,77 @@ __device__ void mlp_mfma(const MfmaPack *__restrict__ P, const uint32_t *u, floa
   for (int j = 0; j < 12; j++) o[j] = sO[lane * 16 + j];
 }
 
+// The MFMA chains of one wave's 64 texels as four independent 16-texel tiles (binary16 or bfloat16 MFMA):
+// x0[g], x1[g] are lane (kb, tl)'s B fragments of tile g (feature words 2kb, 2kb+1 and, for kb = 0, words
+// 8, 9 as binary16 pairs); d[g] receives units 4kb .. 4kb+3 of the output layer before its bias.  The four
+// chains are interleaved layer by layer so the matrix cores overlap one tile's latency with the others.
+template <int MODE>
+__device__ __forceinline__ void mlp_mfma_x4(const MfmaPack *__restrict__ P, const uint2 *x0, const uint2 *x1,
+                                            f32x4 *d, int lane) {
+  const int kb = lane >> 4;
+  uint2 hb[4];
+  if (MODE == MLP_FP16_MFMA) {
+    const f16x4 w0 = __builtin_bit_cast(f16x4, P->h[0][lane]), w1 = __builtin_bit_cast(f16x4, P->h[1][lane]);
+    const f16x4 w2 = __builtin_bit_cast(f16x4, P->h[2][lane]), w3 = __builtin_bit_cast(f16x4, P->h[3][lane]);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w0, __builtin_bit_cast(f16x4, x0[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+      d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w1, __builtin_bit_cast(f16x4, x1[g]), d[g], 0, 0, 0);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const f32x4 h = bias_relu(d[g], P->b1, kb);
+      hb[g] = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
+    }
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+      d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w2, __builtin_bit_cast(f16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const f32x4 h = bias_relu(d[g], P->b2, kb);
+      hb[g] = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
+    }
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+      d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w3, __builtin_bit_cast(f16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+  } else {
+    const s16x4 w0 = __builtin_bit_cast(s16x4, P->b[0][lane]), w1 = __builtin_bit_cast(s16x4, P->b[1][lane]);
+    const s16x4 w2 = __builtin_bit_cast(s16x4, P->b[2][lane]), w3 = __builtin_bit_cast(s16x4, P->b[3][lane]);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const float2 a0 = unpack2(x0[g].x), a1 = unpack2(x0[g].y), c0 = unpack2(x1[g].x), c1 = unpack2(x1[g].y);
+      const uint2 y0 = make_uint2(pack_bf2(a0.x, a0.y), pack_bf2(a1.x, a1.y));
+      const uint2 y1 = make_uint2(pack_bf2(c0.x, c0.y), pack_bf2(c1.x, c1.y));
+      d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w0, __builtin_bit_cast(s16x4, y0), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+      d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1, __builtin_bit_cast(s16x4, y1), d[g], 0, 0, 0);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const f32x4 h = bias_relu(d[g], P->b1, kb);
+      hb[g] = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
+    }
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+      d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2, __builtin_bit_cast(s16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const f32x4 h = bias_relu(d[g], P->b2, kb);
+      hb[g] = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
+    }
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+      d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3, __builtin_bit_cast(s16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+  }
+}
+
+// Binary16 and bfloat16 MFMA forms.  Lane (kb, tl) of wave v loads its B fragments of the wave's four
+// 16-texel tiles straight from F0 / F1 / F2 (tile g = texels 16(g&1) .. +15 of tile row 2v + (g>>1)), the
+// four MFMA chains run interleaved, and the D fragments (+ bias) go through LDS at a 48-byte texel stride
+// (16-byte stores and loads, conflict free) so each lane ends with the 12 outputs of its own texel.
 template <int MODE, bool FAST>
 __global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
   __shared__ float4 sA[SN], sG[SN];
-  __shared__ uint32_t sX[4][64 * 16];
-  __shared__ float sO[4][64 * 16];
   const int tid = threadIdx.x;
   const int2 org = tile_origin<TW, TH>(a.W2, a.H2);
   const int x0 = org.x, y0 = org.y;
@@ -760,24 +825,66 @@ __global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
   }
   const int lx = tid & 31, ly = tid >> 5;
   const int X = x0 + lx, Y = y0 + ly;
-  const bool in_img = inside(X, Y, a.W2, a.H2);
-  uint32_t u[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (in_img) {
-    const uint4 f0 = ldu(a.F0, X, Y), f1 = ldu(a.F1, X, Y), f2 = ldu(a.F2, X, Y);
-    u[0] = f0.x; u[1] = f0.y; u[2] = f0.z; u[3] = f0.w; u[4] = f1.x; u[5] = f1.y; u[6] = f1.z; u[7] = f1.w;
-    u[8] = f2.x; u[9] = f2.y;
+  const int wv = tid >> 6, lane = tid & 63, kb = lane >> 4, tl = lane & 15;
+  const MfmaPack *__restrict__ P = (const MfmaPack *)a.wq;
+  if constexpr (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
+    __shared__ float4 sO[4][64 * 3];
+    uint2 fx0[4], fx1[4];
+    const TexU &Fk = kb < 2 ? a.F0 : a.F1;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const int gx = x0 + 16 * (g & 1) + tl, gy = y0 + 2 * wv + (g >> 1);
+      const bool live = inside(gx, gy, a.W2, a.H2);
+      const size_t o = live ? (size_t)gy : 0;
+      const uint2 *rowk = (const uint2 *)(Fk.p + o * Fk.pitch + (live ? gx : 0));
+      const uint2 *row2 = (const uint2 *)(a.F2.p + o * a.F2.pitch + (live ? gx : 0));
+      fx0[g] = live ? rowk[kb & 1] : make_uint2(0, 0);                   // feature words 2kb, 2kb+1
+      fx1[g] = (live && kb == 0) ? row2[0] : make_uint2(0, 0);          // words 8, 9
+    }
+    f32x4 d[4];
+    mlp_mfma_x4<MODE>(P, fx0, fx1, d, lane);
+    if (kb < 3) {
+#pragma unroll
+      for (int g = 0; g < 4; g++)
+        sO[wv][(16 * g + tl) * 3 + kb] = make_float4(d[g][0] + P->b3[4 * kb], d[g][1] + P->b3[4 * kb + 1],
+                                                     d[g][2] + P->b3[4 * kb + 2], d[g][3] + P->b3[4 * kb + 3]);
+    }
+    __syncthreads();
+    if (!inside(X, Y, a.W2, a.H2)) return;
+    const int tx = lx + AP, ty = ly + AP;
+    if (sG[ty * SW + tx].w < 0.0f) {
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      st(a.out0, X, Y, z); st(a.out1, X, Y, z); st(a.out2, X, Y, z); st(a.outMid, X, Y, z);
+      return;
+    }
+    float o[12];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const float4 v = sO[wv][lane * 3 + j];
+      o[4 * j] = v.x; o[4 * j + 1] = v.y; o[4 * j + 2] = v.z; o[4 * j + 3] = v.w;
+    }
+    infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
+  } else {
+    __shared__ uint32_t sX[4][64 * 16];
+    __shared__ float sOq[4][64 * 16];
+    const bool in_img = inside(X, Y, a.W2, a.H2);
+    uint32_t u[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (in_img) {
+      const uint4 f0 = ldu(a.F0, X, Y), f1 = ldu(a.F1, X, Y), f2 = ldu(a.F2, X, Y);
+      u[0] = f0.x; u[1] = f0.y; u[2] = f0.z; u[3] = f0.w; u[4] = f1.x; u[5] = f1.y; u[6] = f1.z; u[7] = f1.w;
+      u[8] = f2.x; u[9] = f2.y;
+    }
+    float o[12];
+    mlp_mfma<MODE>(P, u, o, sX[wv], sOq[wv], lane);   // holds the tile barrier too
+    if (!in_img) return;
+    const int tx = lx + AP, ty = ly + AP;
+    if (sG[ty * SW + tx].w < 0.0f) {
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      st(a.out0, X, Y, z); st(a.out1, X, Y, z); st(a.out2, X, Y, z); st(a.outMid, X, Y, z);
+      return;
+    }
+    infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
   }
-  float o[12];
-  const int wv = tid >> 6;
-  mlp_mfma<MODE>((const MfmaPack *)a.wq, u, o, sX[wv], sO[wv], tid & 63);   // holds the tile barrier too
-  if (!in_img) return;
-  const int tx = lx + AP, ty = ly + AP;
-  if (sG[ty * SW + tx].w < 0.0f) {
-    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    st(a.out0, X, Y, z); st(a.out1, X, Y, z); st(a.out2, X, Y, z); st(a.outMid, X, Y, z);
-    return;
-  }
-  infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
 }
 
 // The MLP alone over n feature rows (20 binary16 each, row-major, 40 B) -> n x 12 fp32 outputs.  Used to
@@ -823,59 +930,7 @@ __global__ __launch_bounds__(NT) void k_mlp(const uint32_t *__restrict__ feat, f
       x1[g] = (live && kb == 0) ? *(const uint2 *)(row + 8) : make_uint2(0, 0);              // k = 16 .. 19
     }
     f32x4 d[4];
-    uint2 hb[4];
-    if (MODE == MLP_FP16_MFMA) {
-      const f16x4 w0 = __builtin_bit_cast(f16x4, P->h[0][lane]), w1 = __builtin_bit_cast(f16x4, P->h[1][lane]);
-      const f16x4 w2 = __builtin_bit_cast(f16x4, P->h[2][lane]), w3 = __builtin_bit_cast(f16x4, P->h[3][lane]);
-#pragma unroll
-      for (int g = 0; g < 4; g++) {
-        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w0, __builtin_bit_cast(f16x4, x0[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w1, __builtin_bit_cast(f16x4, x1[g]), d[g], 0, 0, 0);
-      }
-#pragma unroll
-      for (int g = 0; g < 4; g++) {
-        const f32x4 h = bias_relu(d[g], P->b1, kb);
-        hb[g] = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
-      }
-#pragma unroll
-      for (int g = 0; g < 4; g++)
-        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w2, __builtin_bit_cast(f16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-#pragma unroll
-      for (int g = 0; g < 4; g++) {
-        const f32x4 h = bias_relu(d[g], P->b2, kb);
-        hb[g] = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
-      }
-#pragma unroll
-      for (int g = 0; g < 4; g++)
-        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w3, __builtin_bit_cast(f16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-    } else {
-      const s16x4 w0 = __builtin_bit_cast(s16x4, P->b[0][lane]), w1 = __builtin_bit_cast(s16x4, P->b[1][lane]);
-      const s16x4 w2 = __builtin_bit_cast(s16x4, P->b[2][lane]), w3 = __builtin_bit_cast(s16x4, P->b[3][lane]);
-#pragma unroll
-      for (int g = 0; g < 4; g++) {
-        const float2 a0 = unpack2(x0[g].x), a1 = unpack2(x0[g].y), c0 = unpack2(x1[g].x), c1 = unpack2(x1[g].y);
-        const uint2 y0 = make_uint2(pack_bf2(a0.x, a0.y), pack_bf2(a1.x, a1.y));
-        const uint2 y1 = make_uint2(pack_bf2(c0.x, c0.y), pack_bf2(c1.x, c1.y));
-        d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w0, __builtin_bit_cast(s16x4, y0), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-        d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1, __builtin_bit_cast(s16x4, y1), d[g], 0, 0, 0);
-      }
-#pragma unroll
-      for (int g = 0; g < 4; g++) {
-        const f32x4 h = bias_relu(d[g], P->b1, kb);
-        hb[g] = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
-      }
-#pragma unroll
-      for (int g = 0; g < 4; g++)
-        d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2, __builtin_bit_cast(s16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-#pragma unroll
-      for (int g = 0; g < 4; g++) {
-        const f32x4 h = bias_relu(d[g], P->b2, kb);
-        hb[g] = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
-      }
-#pragma unroll
-      for (int g = 0; g < 4; g++)
-        d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3, __builtin_bit_cast(s16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-    }
+    mlp_mfma_x4<MODE>(P, x0, x1, d, lane);
 #pragma unroll
     for (int g = 0; g < 4; g++) {
       const long r = base + 16 * g + tl;

@@ -151,3 +151,45 @@ def test_pipeline_psnr_per_precision():
     print("PSNR vs FP64 oracle (dB):", res)
     assert res["fp32"] > 30
     assert abs(res["fp16-mfma"] - res["fp32"]) <= 0.05, res
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16-mfma", "bf16-mfma"])
+def test_in_frame_inference_matches_mlp_stage(precision):
+    """The frame's inference pass (F0..F2 -> par0, par1, par2, hush.js:312-368) against the MLP stage on the
+    same feature rows (hush_mlp_forward, the same per-texel arithmetic) followed by the activations of
+    hush.js:357-366 in FP64: every texel within one binary16 ulp of the round-toward-zero stored value, sky
+    texels zero.  A wrong texel or fragment mapping moves whole texels.  Half-res 93x61, so the 32x8 tiles
+    are ragged on both edges."""
+    from hush_amd.scene import SceneGenerator
+    w, h = 186, 122
+    gen = SceneGenerator("boxes", w, h, 5)
+    hs = hush_amd.Hush(0, {"width": w, "height": h, "precision": precision})
+    surfs = None
+    for f in range(3):
+        surfs = upload_frame(gen.render(f), surfs)
+        hs.denoise(surfs)
+    hs.run_pass(1, surfs)
+    hush_amd.hip.synchronize()
+    cur = hs.frame & 1
+    F = np.concatenate([hs.surface(f"F{k}").download().view(np.float16) for k in range(3)], -1)
+    H2, W2 = F.shape[:2]
+    assert (W2, H2) == (93, 61)
+    sky = hs.surface("F3").download().view(np.float16)[..., 2].astype(np.float32) < 0
+    got = np.stack([hs.surface(n).download().astype(np.float64) for n in (f"par0_{cur}", "par1", "par2")], 2)
+    rows = np.ascontiguousarray(F.reshape(-1, 24)[:, :20])
+    o = run_mlp(hs, rows, precision).astype(np.float64).reshape(H2, W2, 12)
+    hs.destroy()
+    sig = lambda x: 1.0 / (1.0 + np.exp(-x))
+    ref = np.empty((H2, W2, 3, 4))
+    ref[..., 0, 0] = 1 / 32 + 31 / 32 * sig(o[..., 0])
+    ref[..., 0, 1] = 0.5 + 7.5 * sig(o[..., 1])
+    ref[..., 0, 2] = sig(o[..., 2])
+    ref[..., 0, 3] = sig(o[..., 3])
+    ref[..., 1, :] = 0.25 * np.exp(np.clip(o[..., 4:8], -5, 5))
+    ref[..., 2, :] = np.exp(np.clip(o[..., 8:12], -4, 4))
+    ref = ref.reshape(H2, W2, 12)
+    got = got.reshape(H2, W2, 12)
+    assert np.all(got[sky] == 0)
+    tol = np.abs(ref) * 2.0 ** -10 * 1.01 + 1e-7
+    bad = (np.abs(got - ref) > tol) & ~sky[..., None]
+    assert sky.sum() < sky.size and not bad.any(), (precision, int(bad.sum()), np.argwhere(bad)[:5])
