@@ -72,6 +72,30 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   if (in_img) { nd = ld(a.nd, px, py); mv = ld(a.mv, px, py); mr = ld(a.mat, px, py); rd = ld(a.rad, px, py); }
   const bool use_hist = in_img && !a.reset && nd.w >= 0.0f;
 
+  // reprojection fetches (hush.js:144-170): the four bilinear taps of the decoded key, colour and moments
+  // and the nearest half-res par0 texel, issued before the tile staging so that their latency overlaps the
+  // staging, its barrier and the 3x3 statistics; colour, moments and par0 stay packed until used
+  float fx = 0.0f, fy = 0.0f;
+  int bix = 0, biy = 0, cx = 0, cy = 0;
+  float4 k4[4];
+  uint2 c4r[4], m4r[4], pqr = make_uint2(0u, 0u);
+  if (use_hist) {
+    const float ppx = (float)px + mv.x, ppy = (float)py + mv.y;
+    const float bx = floorf(ppx), by = floorf(ppy);
+    fx = ppx - bx; fy = ppy - by;
+    bix = f2i(bx); biy = f2i(by);
+    cx = f2i(floorf(ppx + 0.5f)); cy = f2i(floorf(ppy + 0.5f));
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const size_t qx = clampi(addw(bix, i & 1), 0, a.W - 1), qy = clampi(addw(biy, i >> 1), 0, a.H - 1);
+      k4[i] = a.pkd.p[qy * a.pkd.pitch + qx];
+      c4r[i] = a.pcol.p[qy * a.pcol.pitch + qx];
+      m4r[i] = a.pmom.p[qy * a.pmom.pitch + qx];
+    }
+    const size_t hx = clampi(cx, 0, a.W - 1) >> 1, hy = clampi(cy, 0, a.H - 1) >> 1;
+    pqr = a.ppar0.p[hy * a.ppar0.pitch + hx];
+  }
+
   // tile staging (hush.js:287-294): each thread its own pixel, then 84 threads the 1-texel apron
   {
     float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -157,20 +181,11 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       float v = 0.0f, alpha = 1.0f, gamma = 1.0f;
       if (use_hist) {
         const int W = a.W, H = a.H;
-        const float ppx = (float)px + mv.x, ppy = (float)py + mv.y;
-        float bx = floorf(ppx), by = floorf(ppy);
-        float fx = ppx - bx, fy = ppy - by;
-        int bix = f2i(bx), biy = f2i(by);
-        float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
-        int cx = f2i(floorf(ppx + 0.5f)), cy = f2i(floorf(ppy + 0.5f));
-        // the 12 bilinear loads (decoded key, colour, moments) and par0, issued before any test
-        float4 k4[4], c4[4], m4[4];
+        const float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+        float4 c4[4], m4[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int qx = clampi(addw(bix, i & 1), 0, W - 1), qy = clampi(addw(biy, i >> 1), 0, H - 1);
-          k4[i] = ldf(a.pkd, qx, qy); c4[i] = ld(a.pcol, qx, qy); m4[i] = ld(a.pmom, qx, qy);
-        }
-        const float4 pq = ld(a.ppar0, clampi(cx, 0, W - 1) >> 1, clampi(cy, 0, H - 1) >> 1);
+        for (int i = 0; i < 4; i++) { c4[i] = unpack4(c4r[i]); m4[i] = unpack4(m4r[i]); }
+        const float2 pq = unpack2(pqr.x);
         float sumW = 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
