@@ -1143,28 +1143,65 @@ __global__ __launch_bounds__(NT) void k_coarse1(Coarse1Args a) {
 // histColor[cur]; joint-bilateral upsample of the mid (Bh[0]) and coarse (C[0]) levels; recombine with
 // kappa and beta; remodulate -> X.
 
+// joint-bilateral weight of one upsampling tap; the depth term's divisor 0.05 z + 1e-3 is the pixel's own,
+// shared by its eight taps (jd, rjd = rcp(jd))
 template <bool FAST>
-__device__ __forceinline__ float jbw(float frx, float fry, int j, float3 nq, float zq, float3 n, float z) {
+__device__ __forceinline__ float jbw(float frx, float fry, int j, float3 nq, float zq, float3 n, float z, float jd,
+                                     float rjd) {
   float wb = ((j & 1) == 1 ? frx : 1.0f - frx) * ((j >> 1) == 1 ? fry : 1.0f - fry);
   float dn = mx(dot3(nq, n), 0.0f);
   float dn2 = dn * dn;
   float dn4 = dn2 * dn2;
-  return wb * Math<FAST>::exp(Math<FAST>::div(-fabsf(zq - z), 0.05f * z + 1e-3f)) * dn4 * dn4;
+  return wb * Math<FAST>::exp(Math<FAST>::divs(-fabsf(zq - z), jd, rjd)) * dn4 * dn4;
 }
+
+// The upsampling taps of a 32x8 tile fall in an 18x6 window of the half-res level (origin x0/2 - 1,
+// y0/2 - 1: pixel 2k takes texels k-1, k and pixel 2k+1 texels k, k+1) and a 10x4 window of the
+// quarter-res level (origin x0/4 - 1, y0/4 - 1), and its par texels are the 16x4 half-res texels
+// (x0/2, y0/2) ..; all three are staged in LDS once per tile.
+constexpr int HWW = TW / 2 + 2, HWH = TH / 2 + 2, HWN = HWW * HWH;      // 18 x 6
+constexpr int QWW = TW / 4 + 2, QWH = TH / 4 + 2, QWN = QWW * QWH;      // 10 x 4
+constexpr int PWN = (TW / 2) * (TH / 2);                                // 16 x 4
 
 template <bool FAST>
 __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   using F = Math<FAST>;
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
   __shared__ float4 sA[SN], sG[SN];
+  __shared__ float4 sHG[HWN], sHM[HWN];   // half-res: decoded guide (normal, depth); mid rgb + F2.z low
+  __shared__ float2 sHA[HWN];             // half-res: F2.z high, F2.w low
+  __shared__ float4 sQG[QWN], sQC[QWN];   // quarter-res: decoded guide; coarse
+  __shared__ float4 sPar[PWN];            // par1.x, par2.x, par0.z, par0.w
   const int tid = threadIdx.x;
   const int2 org = tile_origin<TW, TH>(a.W, a.H);
   const int x0 = org.x, y0 = org.y;
+  const int hx0 = (x0 >> 1) - 1, hy0 = (y0 >> 1) - 1, qx0 = (x0 >> 2) - 1, qy0 = (y0 >> 2) - 1;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
     sA[i] = ld(a.A0, qx, qy);
     const float4 kd = ldf(a.kd, qx, qy);
     sG[i] = make_float4(kd.x, kd.y, kd.z, pair_of(kd.w).x);
+  }
+  // windows: texels outside the level are staged clamped and never read (the taps skip them)
+  if (tid < HWN) {
+    const int qx = clampi(hx0 + tid % HWW, 0, a.W2 - 1), qy = clampi(hy0 + tid / HWW, 0, a.H2 - 1);
+    const float4 gh = ldf(a.GH, qx, qy);
+    const uint4 g2 = ldu(a.F2, qx, qy);
+    const float4 m = ld(a.mid, qx, qy);
+    const float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
+    sHG[tid] = make_float4(gh.x, gh.y, gh.z, pair_of(gh.w).x);
+    sHM[tid] = make_float4(m.x, m.y, m.z, a01.x);
+    sHA[tid] = make_float2(a01.y, a23.x);
+  } else if (tid < HWN + QWN) {
+    const int i = tid - HWN;
+    const int qx = clampi(qx0 + i % QWW, 0, a.W4 - 1), qy = clampi(qy0 + i / QWW, 0, a.H4 - 1);
+    sQG[i] = ldf(a.cgd, qx, qy);
+    sQC[i] = ld(a.coarse, qx, qy);
+  } else if (tid < HWN + QWN + PWN) {
+    const int i = tid - HWN - QWN;
+    const int qx = clampi((x0 >> 1) + i % (TW / 2), 0, a.W2 - 1), qy = clampi((y0 >> 1) + i / (TW / 2), 0, a.H2 - 1);
+    const float4 p0 = ld(a.par0, qx, qy);
+    sPar[i] = make_float4(ld(a.par1, qx, qy).x, ld(a.par2, qx, qy).x, p0.z, p0.w);
   }
   __syncthreads();
   const int lx = tid & 31, ly = tid >> 5;
@@ -1178,13 +1215,13 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
     st(a.outX, px, py, make_float4(emi.x, emi.y, emi.z, 0.0f));
     return;
   }
-  const int hx = px >> 1, hy = py >> 1;
-  const float4 p1 = ld(a.par1, hx, hy), p2 = ld(a.par2, hx, hy), pr = ld(a.par0, hx, hy);
-  const float4 A1 = filt<1, 1, true, SW, FAST>(sA, sG, tx, ty, px, py, a.W, a.H, p1.x, p2.x);
+  const float4 pp = sPar[(ly >> 1) * (TW / 2) + (lx >> 1)];       // par1.x, par2.x, par0.z, par0.w
+  const float4 A1 = filt<1, 1, true, SW, FAST>(sA, sG, tx, ty, px, py, a.W, a.H, pp.x, pp.y);
   st(a.outHist, px, py, A1);
   const float fx = qf16(A1.x), fy = qf16(A1.y), fz = qf16(A1.z);
   const float3 n = make_float3(gp.x, gp.y, gp.z);
   const float z = gp.w;
+  const float jd = 0.05f * z + 1e-3f, rjd = F::rcp(jd);
   // half-res mid level (hush.js:585-602)
   const float hpx = ((float)px + 0.5f) / 2.0f - 0.5f, hpy = ((float)py + 0.5f) / 2.0f - 0.5f;
   const float hbx = floorf(hpx), hby = floorf(hpy);
@@ -1193,23 +1230,22 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   for (int j = 0; j < 4; j++) {
     int qx = (int)hbx + (j & 1), qy = (int)hby + (j >> 1);
     if (qx < 0 || qy < 0 || qx >= a.W2 || qy >= a.H2) continue;
-    const float4 gh = ldf(a.GH, qx, qy);
-    const float2 oz = pair_of(gh.w);
-    if (oz.x < 0.0f) continue;
-    float w = jbw<FAST>(hpx - hbx, hpy - hby, j, make_float3(gh.x, gh.y, gh.z), oz.x, n, z);
-    uint4 g2 = ldu(a.F2, qx, qy);
-    float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
+    const int wi = (qy - hy0) * HWW + (qx - hx0);
+    const float4 gh = sHG[wi];
+    if (gh.w < 0.0f) continue;
+    float w = jbw<FAST>(hpx - hbx, hpy - hby, j, make_float3(gh.x, gh.y, gh.z), gh.w, n, z, jd, rjd);
+    const float4 m = sHM[wi];
+    const float2 aa = sHA[wi];
     swh += w;
-    float4 m = ld(a.mid, qx, qy);
     uBx += w * m.x; uBy += w * m.y; uBz += w * m.z;
-    uAx += w * a01.x; uAy += w * a01.y; uAz += w * a23.x;
+    uAx += w * m.w; uAy += w * aa.x; uAz += w * aa.y;
   }
   float ix = fx, iy = fy, iz = fz;
   if (swh > 1e-4f) {
     const float rh = F::rcp(swh);
-    ix = F::divs(uBx, swh, rh) + pr.w * (fx - F::divs(uAx, swh, rh));
-    iy = F::divs(uBy, swh, rh) + pr.w * (fy - F::divs(uAy, swh, rh));
-    iz = F::divs(uBz, swh, rh) + pr.w * (fz - F::divs(uAz, swh, rh));
+    ix = F::divs(uBx, swh, rh) + pp.w * (fx - F::divs(uAx, swh, rh));
+    iy = F::divs(uBy, swh, rh) + pp.w * (fy - F::divs(uAy, swh, rh));
+    iz = F::divs(uBz, swh, rh) + pp.w * (fz - F::divs(uAz, swh, rh));
   }
   // quarter-res coarse level (hush.js:604-619)
   const float cpx = ((float)px + 0.5f) / 4.0f - 0.5f, cpy = ((float)py + 0.5f) / 4.0f - 0.5f;
@@ -1219,11 +1255,12 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   for (int j = 0; j < 4; j++) {
     int qx = (int)cbx + (j & 1), qy = (int)cby + (j >> 1);
     if (qx < 0 || qy < 0 || qx >= a.W4 || qy >= a.H4) continue;
-    const float4 g = ldf(a.cgd, qx, qy);
+    const int wi = (qy - qy0) * QWW + (qx - qx0);
+    const float4 g = sQG[wi];
     if (g.w < 0.0f) continue;
-    float w = jbw<FAST>(cpx - cbx, cpy - cby, j, make_float3(g.x, g.y, g.z), g.w, n, z);
+    float w = jbw<FAST>(cpx - cbx, cpy - cby, j, make_float3(g.x, g.y, g.z), g.w, n, z, jd, rjd);
     swq += w;
-    float4 cc = ld(a.coarse, qx, qy);
+    const float4 cc = sQC[wi];
     uCx += w * cc.x; uCy += w * cc.y; uCz += w * cc.z;
   }
   float upx = ix, upy = iy, upz = iz;
@@ -1231,9 +1268,9 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
     const float rq = F::rcp(swq);
     upx = F::divs(uCx, swq, rq); upy = F::divs(uCy, swq, rq); upz = F::divs(uCz, swq, rq);
   }
-  ix = ix + (upx - ix) * pr.z;
-  iy = iy + (upy - iy) * pr.z;
-  iz = iz + (upz - iz) * pr.z;
+  ix = ix + (upx - ix) * pp.z;
+  iy = iy + (upy - iy) * pp.z;
+  iz = iz + (upz - iz) * pp.z;
   st(a.outX, px, py, make_float4(ix * alb.x + emi.x, iy * alb.y + emi.y, iz * alb.z + emi.z, 0.0f));
 }
 
