@@ -341,6 +341,7 @@ hipError_t hush_run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream_t
       TaaArgs a;
       a.X = t[S_X]; a.mv = in_tex(in->motion); a.prev = t[S_TAA0 + prev]; a.out = t[S_TAA0 + cur];
       a.W = h->W; a.H = h->H; a.reset = h->reset ? 1 : 0;
+      a.rW = 1.0f / (float)h->W; a.rH = 1.0f / (float)h->H;
       return launch_taa(a, !h->ieee, s);
     }
     default:
@@ -476,5 +477,24 @@ extern "C" int32_t hush_debug_run_pass(hush_t *h, int32_t pass, const hush_input
 extern "C" int32_t hush_set_debug_mask(hush_t *h, int32_t enabled) {
   if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_debug_mask: null context");
   h->mask_on = enabled != 0;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_debug_math_selftest(int32_t which, uint64_t *mismatches, uint32_t *first) {
+  if (which < 0 || which > 2 || !mismatches || !first)
+    return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_debug_math_selftest: bad argument");
+  struct Counters { unsigned long long bad; uint32_t first, pad; } host = {0ull, 0xffffffffu, 0u};
+  Counters *dev = nullptr;
+  hipError_t e = hipMalloc((void **)&dev, sizeof(Counters));
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hush_debug_math_selftest: hipMalloc");
+  e = hipMemcpy(dev, &host, sizeof(Counters), hipMemcpyHostToDevice);
+  // 2^32 inputs in 16 launches of 2^28 (2^20 workgroups of 256)
+  for (uint32_t k = 0; k < 16 && e == hipSuccess; k++)
+    e = launch_math_selftest(which, k << 28, 1u << 20, &dev->bad, &dev->first, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(&host, dev, sizeof(Counters), hipMemcpyDeviceToHost);
+  hipFree(dev);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hush_debug_math_selftest");
+  *mismatches = host.bad;
+  *first = host.first;
   return HUSH_OK;
 }

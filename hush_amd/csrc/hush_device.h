@@ -120,12 +120,47 @@ template <> struct Math<false> {
   static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
   static __device__ __forceinline__ float exp(float x) { return expf(x); }
   static __device__ __forceinline__ float log(float x) { return logf(x); }
+  // The next three are sqrtf / expf with the range handling that their call sites' arguments never need
+  // left out; on the stated domains they are bit-identical to sqrtf / expf (checked over every binary32
+  // input of the domain by hush_debug_math_selftest).
+  // Square root of x >= 2^-96, +0, +inf or NaN: the hardware root y, then whichever of y and its two
+  // neighbours has the residual x - y'^2 that brackets the exact root (sqrtf's own correction), without
+  // the rescaling of arguments below 2^-96 and the +-0 / +inf pass-through (9 VALU instructions, not 17).
+  static __device__ __forceinline__ float sqrtn(float x) {
+    const float y = __builtin_amdgcn_sqrtf(x);
+    const int iy = __float_as_int(y);
+    const float ym = __int_as_float(iy - 1), yp = __int_as_float(iy + 1);
+    const float vm = __builtin_fmaf(-ym, y, x), vp = __builtin_fmaf(-yp, y, x);
+    const float r = vm <= 0.0f ? ym : y;
+    return vp > 0.0f ? yp : r;
+  }
+  // e^x = 2^(x log2 e): x log2 e split into its nearest integer e and a remainder carried to ~48 bits
+  // (log2 e as a two-float constant), 2^remainder in hardware, scaled by 2^e; results below 2^-149 flush
+  // to zero.  expn: x <= 0 or NaN (no overflow test); expm: -103.28 <= x <= 88.72 (no range test).
+  static __device__ __forceinline__ float exp_core(float x) {
+    const float L2E = __int_as_float(0x3fb8aa3b), L2E_LO = __int_as_float(0x32a5705f);
+    const float ph = x * L2E;
+    float pl = __builtin_fmaf(x, L2E, -ph);
+    const float e = __builtin_rintf(ph);
+    pl = __builtin_fmaf(x, L2E_LO, pl);
+    const float t = (ph - e) + pl;
+    // e >= -200 keeps the conversion defined for every x (NaN included); below -149 the result is 0 anyway
+    return __builtin_ldexpf(__builtin_amdgcn_exp2f(t), (int)fmaxf(e, -200.0f));
+  }
+  static __device__ __forceinline__ float expn(float x) {
+    const float r = exp_core(x);
+    return x < __int_as_float(0xc2ce8ed0) ? 0.0f : r;        // -103.27893 = ln(2^-149)
+  }
+  static __device__ __forceinline__ float expm(float x) { return exp_core(x); }
 };
 template <> struct Math<true> {
   static __device__ __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float divs(float a, float, float y) { return a * y; }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float sqrtn(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float expn(float x) { return exp(x); }
+  static __device__ __forceinline__ float expm(float x) { return exp(x); }
   // e^x = 2^t * 2^lo with t = fl(x log2 e) and lo its rounding error (x log2 e carried to ~48 bits)
   static __device__ __forceinline__ float exp(float x) {
     const float L2E = 1.44269502e+00f, L2E_LO = 1.92596299e-08f;
@@ -161,7 +196,8 @@ __device__ __forceinline__ float3 oct_dec(float ex, float ey) {
   float t = mx(-vz, 0.0f);
   vx += vx >= 0.0f ? -t : t;
   vy += vy >= 0.0f ? -t : t;
-  float l = mx(sqrtf(vx * vx + vy * vy + vz * vz), 1e-20f);
+  // the squared norm is at least 1/3 for every finite (ex, ey)
+  float l = mx(Math<false>::sqrtn(vx * vx + vy * vy + vz * vz), 1e-20f);
   const float r = 1.0f / l;
   return make_float3(Math<false>::divs(vx, l, r), Math<false>::divs(vy, l, r), Math<false>::divs(vz, l, r));
 }

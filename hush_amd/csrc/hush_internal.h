@@ -77,9 +77,12 @@ struct FineArgs {
 struct TaaArgs {
   Tex X, mv, prev, out;
   int W, H, reset;
+  float rW, rH;                      // RN(1 / W), RN(1 / H)
 };
 
 hipError_t launch_temporal(const TemporalArgs &a, bool fast, hipStream_t s);
+hipError_t launch_math_selftest(int which, uint32_t base, uint32_t blocks, unsigned long long *bad, uint32_t *first,
+                                hipStream_t s);
 hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s);
 hipError_t launch_mid2(const AtrousArgs &a, bool fast, hipStream_t s);
 hipError_t launch_mid3(const AtrousArgs &a, bool fast, hipStream_t s);

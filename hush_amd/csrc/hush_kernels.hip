@@ -171,8 +171,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       }
       const float svx = avx.get(), svy = avy.get(), svz = avz.get();
       float svar = avl.get();
-      const float sdx = F::sqrt(mx(F::divs(svx, cnt, rcnt), 1e-12f)), sdy = F::sqrt(mx(F::divs(svy, cnt, rcnt), 1e-12f)),
-                  sdz = F::sqrt(mx(F::divs(svz, cnt, rcnt), 1e-12f));
+      const float sdx = F::sqrtn(mx(F::divs(svx, cnt, rcnt), 1e-12f)), sdy = F::sqrtn(mx(F::divs(svy, cnt, rcnt), 1e-12f)),
+                  sdz = F::sqrtn(mx(F::divs(svz, cnt, rcnt), 1e-12f));
       svar = F::divs(svar, cnt, rcnt);
       // reprojection (hush.js:144-187)
       bool has = false;
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         nmy = hm.y + (l * l - hm.y) * aa;
         nfast = hm.z + (l - hm.z) * mx(1.0f / 6.0f, F::rcp(len1));
         float lh = lum(hcx, hcy, hcz);
-        lag = F::div(fabsf(slm - lh), F::sqrt(mx(F::divs(svar, cnt, rcnt), 1e-12f)) + 0.02f * lh + 1e-3f);
+        lag = F::div(fabsf(slm - lh), F::sqrtn(mx(F::divs(svar, cnt, rcnt), 1e-12f)) + 0.02f * lh + 1e-3f);
       } else {
         v = 0.0f;
       }
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       P[P0X * NT] = E * lum(aqx, aqy, aqz); P[P0Y * NT] = E * qf16(l); P[P0Z * NT] = mx(m2 - m1 * m1, 0.0f); P[P0W * NT] = len1;
       P[P1X * NT] = qf16(v); P[P1Y * NT] = qf16(lag); P[P1Z * NT] = E * qf16(nfast); P[P1W * NT] = m1;
       float ocx = qf16(oc.x), ocy = qf16(oc.y);
-      P[P2Z * NT] = qf16(nd.w); P[P2W * NT] = qf16(F::sqrt(mv.x * mv.x + mv.y * mv.y));
+      P[P2Z * NT] = qf16(nd.w); P[P2W * NT] = qf16(F::sqrtn(mv.x * mv.x + mv.y * mv.y));
       float3 ns = (ocx == ok.x && ocy == ok.y) ? dec : oct_dec_m<FAST>(ocx, ocy);
       P[NSX * NT] = ns.x; P[NSY * NT] = ns.y; P[NSZ * NT] = ns.z;
       P[P3X * NT] = mr.x; P[P3Y * NT] = mr.y; P[P3Z * NT] = mr.z; P[P3W * NT] = mr.w;
@@ -408,7 +408,7 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
     gv = center.w;
   }
   const float lc = lum(center.x, center.y, center.z);
-  const float sl = F::div(s, F::sqrt(mx(gv, 1e-12f)) + 1e-6f);
+  const float sl = F::div(s, F::sqrtn(mx(gv, 1e-12f)) + 1e-6f);
   const float zl = sG[c - 1].w, zr = sG[c + 1].w, zu = sG[c - SW].w, zd = sG[c + SW].w;
   const float gx = mn(zr > 0.0f ? fabsf(zr - gp.w) : 1e9f, zl > 0.0f ? fabsf(zl - gp.w) : 1e9f);
   const float gy = mn(zd > 0.0f ? fabsf(zd - gp.w) : 1e9f, zu > 0.0f ? fabsf(zu - gp.w) : 1e9f);
@@ -432,7 +432,7 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
         const float4 cq = sA[c + o];
         const float ww = (float)(Kern<R>::h[dx + R] * Kern<R>::h[dy + R])
                        * pow128(mx(dot3(make_float3(gq.x, gq.y, gq.z), make_float3(gp.x, gp.y, gp.z)), 0.0f))
-                       * F::exp(-fabsf(gq.w - gp.w) * iz[dx * dx + dy * dy] - sl * fabsf(lum(cq.x, cq.y, cq.z) - lc));
+                       * F::expn(-fabsf(gq.w - gp.w) * iz[dx * dx + dy * dy] - sl * fabsf(lum(cq.x, cq.y, cq.z) - lc));
         sw += ww;
         scx += ww * cq.x; scy += ww * cq.y; scz += ww * cq.z;
         sv += ww * ww * cq.w;
@@ -520,8 +520,8 @@ __device__ __forceinline__ void infer_outputs(const InferArgs &a, const float4 *
   float s[4], c[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    s[k] = 0.25f * F::exp(mn(mx(o[4 + k], -5.0f), 5.0f));
-    c[k] = F::exp(mn(mx(o[8 + k], -4.0f), 4.0f));
+    s[k] = 0.25f * F::expm(mn(mx(o[4 + k], -5.0f), 5.0f));
+    c[k] = F::expm(mn(mx(o[8 + k], -4.0f), 4.0f));
   }
   st(a.out0, X, Y, make_float4(alpha, gamma, beta, kappa));
   st(a.out1, X, Y, make_float4(s[0], s[1], s[2], s[3]));
@@ -1152,7 +1152,7 @@ __device__ __forceinline__ float jbw(float frx, float fry, int j, float3 nq, flo
   float dn = mx(dot3(nq, n), 0.0f);
   float dn2 = dn * dn;
   float dn4 = dn2 * dn2;
-  return wb * Math<FAST>::exp(Math<FAST>::divs(-fabsf(zq - z), jd, rjd)) * dn4 * dn4;
+  return wb * Math<FAST>::expn(Math<FAST>::divs(-fabsf(zq - z), jd, rjd)) * dn4 * dn4;
 }
 
 // The upsampling taps of a 32x8 tile fall in an 18x6 window of the half-res level (origin x0/2 - 1,
@@ -1303,18 +1303,28 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
   const int tid = threadIdx.x;
   const int2 org = tile_origin<TW, TH>(a.W, a.H);
   const int x0 = org.x, y0 = org.y;
-  for (int i = tid; i < SN; i += NT) {
-    int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
-    float4 x = ld(a.X, qx, qy);
-    float3 y = to_ycocg(tm(make_float3(x.x, x.y, x.z)));
-    sX[i] = make_float4(y.x, y.y, y.z, 0.0f);
-  }
-  __syncthreads();
   const int lx = tid & 31, ly = tid >> 5;
   const int px = x0 + lx, py = y0 + ly;
-  if (px >= a.W || py >= a.H) return;
-  const float4 xc = ld(a.X, px, py);
+  // tile staging of to_ycocg(tm(X)), clamp to edge: each thread its own texel (whose tone-mapped colour it
+  // keeps), then 84 threads the 1-texel apron
+  const float4 xc = ld(a.X, clampi(px, 0, a.W - 1), clampi(py, 0, a.H - 1));
   const float3 cur = tm(make_float3(xc.x, xc.y, xc.z));
+  {
+    const float3 y = to_ycocg(cur);
+    sX[(ly + AP) * SW + lx + AP] = make_float4(y.x, y.y, y.z, 0.0f);
+  }
+  if (tid < 2 * SW + 2 * TH) {
+    int tx, ty;
+    if (tid < SW) { tx = tid; ty = 0; }
+    else if (tid < 2 * SW) { tx = tid - SW; ty = SH - 1; }
+    else if (tid < 2 * SW + TH) { tx = 0; ty = tid - 2 * SW + 1; }
+    else { tx = SW - 1; ty = tid - 2 * SW - TH + 1; }
+    const float4 x = ld(a.X, clampi(x0 - AP + tx, 0, a.W - 1), clampi(y0 - AP + ty, 0, a.H - 1));
+    const float3 y = to_ycocg(tm(make_float3(x.x, x.y, x.z)));
+    sX[ty * SW + tx] = make_float4(y.x, y.y, y.z, 0.0f);
+  }
+  __syncthreads();
+  if (px >= a.W || py >= a.H) return;
   if (a.reset) {
     float3 r = itm(cur);
     st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
@@ -1336,11 +1346,13 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
     float dx = s.x - m1x, dy = s.y - m1y, dz = s.z - m1z;
     vx += dx * dx; vy += dy * dy; vz += dz * dz;
   }
-  const float sdx = F::sqrt(mx(F::divs(vx, 9.0f, r9), 1e-12f)), sdy = F::sqrt(mx(F::divs(vy, 9.0f, r9), 1e-12f)),
-              sdz = F::sqrt(mx(F::divs(vz, 9.0f, r9), 1e-12f));
+  const float sdx = F::sqrtn(mx(F::divs(vx, 9.0f, r9), 1e-12f)), sdy = F::sqrtn(mx(F::divs(vy, 9.0f, r9), 1e-12f)),
+              sdz = F::sqrtn(mx(F::divs(vz, 9.0f, r9), 1e-12f));
   const float4 mv = ld(a.mv, px, py);
   const float szx = (float)a.W, szy = (float)a.H;
-  const float ux = ((float)px + 0.5f + mv.x) / szx, uy = ((float)py + 0.5f + mv.y) / szy;
+  // (px + 0.5 + mv) / size with the host's correctly rounded 1 / size: equal to the quotient for every finite
+  // numerator; a non-finite one fails the range test below either way
+  const float ux = Math<false>::divs((float)px + 0.5f + mv.x, szx, a.rW), uy = Math<false>::divs((float)py + 0.5f + mv.y, szy, a.rH);
   float3 res = cur;
   if (ux >= 0.0f && uy >= 0.0f && ux <= 1.0f && uy <= 1.0f) {
     float ppx = ux * szx - 0.5f, ppy = uy * szy - 0.5f;
@@ -1462,6 +1474,34 @@ hipError_t launch_fine_final(const FineArgs &a, bool fast, hipStream_t s) {
 }
 hipError_t launch_taa(const TaaArgs &a, bool fast, hipStream_t s) {
   HUSH_LAUNCH(K_TAA, grid_for(a.W, a.H), a);
+  return hipGetLastError();
+}
+
+// ============================================================================ reduced-range math self-test
+// one binary32 input per thread: inputs base .. base + NT * gridDim.x - 1 (as bit patterns)
+__global__ __launch_bounds__(NT) void k_math_selftest(int which, uint32_t base, unsigned long long *bad,
+                                                      uint32_t *first) {
+  const uint32_t u = base + blockIdx.x * NT + threadIdx.x;
+  const float x = __uint_as_float(u);
+  float r, ref;
+  if (which == 0) {
+    if (!(x >= 0x1p-96f || u == 0u || x != x)) return;
+    r = Math<false>::sqrtn(x); ref = sqrtf(x);
+  } else if (which == 1) {
+    if (!(x <= 0.0f || x != x)) return;
+    r = Math<false>::expn(x); ref = expf(x);
+  } else {
+    if (!(x >= __int_as_float(0xc2ce8ed0) && x <= __int_as_float(0x42b17218))) return;
+    r = Math<false>::expm(x); ref = expf(x);
+  }
+  if (__float_as_uint(r) == __float_as_uint(ref) || (r != r && ref != ref)) return;
+  atomicAdd(bad, 1ull);
+  atomicMin(first, u);
+}
+
+hipError_t launch_math_selftest(int which, uint32_t base, uint32_t blocks, unsigned long long *bad, uint32_t *first,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_math_selftest, dim3(blocks), dim3(NT), 0, s, which, base, bad, first);
   return hipGetLastError();
 }
 
