@@ -120,9 +120,16 @@ template <> struct Math<false> {
   static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
   static __device__ __forceinline__ float exp(float x) { return expf(x); }
   static __device__ __forceinline__ float log(float x) { return logf(x); }
+  // Correctly rounded 1 / b for normal b with |b| < 2^126 (and NaN for NaN): the hardware reciprocal, then
+  // one Newton step with an FMA residual, 3 VALU instructions instead of 11.  Zero, infinite and
+  // subnormal b are outside the domain (the step returns NaN where 1 / b is infinite or zero).
+  static __device__ __forceinline__ float rcpn(float b) {
+    const float y = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+  }
   // The next three are sqrtf / expf with the range handling that their call sites' arguments never need
   // left out; on the stated domains they are bit-identical to sqrtf / expf (checked over every binary32
-  // input of the domain by hush_debug_math_selftest).
+  // input of the domain by hush_debug_math_selftest, as is rcpn against 1 / b).
   // Square root of x >= 2^-96, +0, +inf or NaN: the hardware root y, then whichever of y and its two
   // neighbours has the residual x - y'^2 that brackets the exact root (sqrtf's own correction), without
   // the rescaling of arguments below 2^-96 and the +-0 / +inf pass-through (9 VALU instructions, not 17).
@@ -159,6 +166,7 @@ template <> struct Math<true> {
   static __device__ __forceinline__ float divs(float a, float, float y) { return a * y; }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float sqrtn(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float rcpn(float b) { return __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float expn(float x) { return exp(x); }
   static __device__ __forceinline__ float expm(float x) { return exp(x); }
   // e^x = 2^t * 2^lo with t = fl(x log2 e) and lo its rounding error (x log2 e carried to ~48 bits)
@@ -198,7 +206,7 @@ __device__ __forceinline__ float3 oct_dec(float ex, float ey) {
   vy += vy >= 0.0f ? -t : t;
   // the squared norm is at least 1/3 for every finite (ex, ey)
   float l = mx(Math<false>::sqrtn(vx * vx + vy * vy + vz * vz), 1e-20f);
-  const float r = 1.0f / l;
+  const float r = Math<false>::rcpn(l);            // l in [0.57, 2.3e5] for stored binary16 inputs
   return make_float3(Math<false>::divs(vx, l, r), Math<false>::divs(vy, l, r), Math<false>::divs(vz, l, r));
 }
 

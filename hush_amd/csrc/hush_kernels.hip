@@ -156,7 +156,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       }
       const float s1x = a1x.get(), s1y = a1y.get(), s1z = a1z.get();
       cnt = mx(cnt, 1.0f);
-      const float rcnt = F::rcp(cnt);
+      const float rcnt = F::rcpn(cnt);
       const float mux = F::divs(s1x, cnt, rcnt), muy = F::divs(s1y, cnt, rcnt), muz = F::divs(s1z, cnt, rcnt);
       const float slm = lum(mux, muy, muz);
       Sum<KAHAN> avx, avy, avz, avl;
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
           }
         }
         if (sumW >= 0.01f) {
-          const float rs = F::rcp(sumW);
+          const float rs = F::rcpn(sumW);
           hcx = F::divs(hcx, sumW, rs); hcy = F::divs(hcy, sumW, rs); hcz = F::divs(hcz, sumW, rs);
           hm.x = F::divs(hm.x, sumW, rs); hm.y = F::divs(hm.y, sumW, rs); hm.z = F::divs(hm.z, sumW, rs); hm.w = F::divs(hm.w, sumW, rs);
           v = sumW; has = true;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
             }
           }
           if (n3 > 0.0f) {
-            const float rn = F::rcp(n3);
+            const float rn = F::rcpn(n3);
             hcx = F::divs(hcx, n3, rn); hcy = F::divs(hcy, n3, rn); hcz = F::divs(hcz, n3, rn);
             hm.x = F::divs(hm.x, n3, rn); hm.y = F::divs(hm.y, n3, rn); hm.z = F::divs(hm.z, n3, rn); hm.w = F::divs(hm.w, n3, rn);
             v = 0.1f; has = true;
@@ -233,11 +233,11 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         float gx = gamma * sdx, gy = gamma * sdy, gz = gamma * sdz;
         float hlx = mn(mx(hcx, mux - gx), mux + gx), hly = mn(mx(hcy, muy - gy), muy + gy), hlz = mn(mx(hcz, muz - gz), muz + gz);
         len1 = mn(hm.w + 1.0f, 32.0f);
-        float aa = mx(alpha, F::rcp(len1));
+        float aa = mx(alpha, F::rcpn(len1));
         cr = hlx + (ce.x - hlx) * aa; cg = hly + (ce.y - hly) * aa; cb = hlz + (ce.z - hlz) * aa;
         nmx = hm.x + (l - hm.x) * aa;
         nmy = hm.y + (l * l - hm.y) * aa;
-        nfast = hm.z + (l - hm.z) * mx(1.0f / 6.0f, F::rcp(len1));
+        nfast = hm.z + (l - hm.z) * mx(1.0f / 6.0f, F::rcpn(len1));
         float lh = lum(hcx, hcy, hcz);
         lag = F::div(fabsf(slm - lh), F::sqrtn(mx(F::divs(svar, cnt, rcnt), 1e-12f)) + 0.02f * lh + 1e-3f);
       } else {
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     stf(a.outGH, X, Y, make_float4(d0.x, d0.y, d0.z, __uint_as_float(g3.y)));
     return;
   }
-  const float k = F::rcp(nv);
+  const float k = F::rcpn(nv);
 #pragma unroll
   for (int c = 0; c < 4; c++) { S0[c] *= k; S1[c] *= k; S3[c] *= k; AC[c] *= k; }
   Nx *= k; Ny *= k; Nz *= k; MV *= k; Z *= k;
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   }
   float var4 = F::div(q4.get(), mx(n4, 1.0f));
   const float ln = mx(F::sqrt(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
-  const float rln = F::rcp(ln);
+  const float rln = F::rcpn(ln);
   const float3 Nn = make_float3(F::divs(Nx, ln, rln), F::divs(Ny, ln, rln), F::divs(Nz, ln, rln));
   float zmax = -1e30f, zmin = 1e30f, mind = 1e30f;
 #pragma unroll
@@ -416,7 +416,7 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
   float iz[9];
 #pragma unroll
   for (int d2 = 1; d2 <= 2 * R * R; d2++)
-    if (tap_dist(d2) > 0.0) iz[d2] = F::rcp(zg * (float)(tap_dist(d2) * STEP) + 1e-3f);
+    if (tap_dist(d2) > 0.0) iz[d2] = F::rcpn(zg * (float)(tap_dist(d2) * STEP) + 1e-3f);
   const float cw = (float)(Kern<R>::h[R] * Kern<R>::h[R]) * cc;
   float sw = cw;
   float scx = center.x * cw, scy = center.y * cw, scz = center.z * cw;
@@ -438,7 +438,7 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
         sv += ww * ww * cq.w;
       }
     }
-  const float rsw = F::rcp(sw);
+  const float rsw = F::rcpn(sw);
   return make_float4(F::divs(scx, sw, rsw), F::divs(scy, sw, rsw), F::divs(scz, sw, rsw), F::div(sv, sw * sw));
 }
 
@@ -1100,7 +1100,7 @@ __device__ __forceinline__ void down(const TexU &F2, const TexF &GH, int QX, int
   if (N == 0.0f) { col = make_float4(0.0f, 0.0f, 0.0f, 0.0f); g = make_float4(0.0f, 0.0f, -1.0f, 0.0f); return; }
   using F = Math<false>;   // stored level values: IEEE under both policies (they set the slope sl)
   float ln = mx(F::sqrt(nx * nx + ny * ny + nz * nz), 1e-6f);
-  const float rln = F::rcp(ln), rN = F::rcp(N);
+  const float rln = F::rcpn(ln), rN = F::rcpn(N);
   float2 o = oct_enc_m<FAST>(make_float3(F::divs(nx, ln, rln), F::divs(ny, ln, rln), F::divs(nz, ln, rln)));
   col = make_float4(qf16(F::divs(clx, N, rN)), qf16(F::divs(cly, N, rN)), qf16(F::divs(clz, N, rN)),
                     qf16(F::divs(F::divs(vr, N, rN), N, rN)));
@@ -1221,7 +1221,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   const float fx = qf16(A1.x), fy = qf16(A1.y), fz = qf16(A1.z);
   const float3 n = make_float3(gp.x, gp.y, gp.z);
   const float z = gp.w;
-  const float jd = 0.05f * z + 1e-3f, rjd = F::rcp(jd);
+  const float jd = 0.05f * z + 1e-3f, rjd = F::rcpn(jd);
   // half-res mid level (hush.js:585-602)
   const float hpx = ((float)px + 0.5f) / 2.0f - 0.5f, hpy = ((float)py + 0.5f) / 2.0f - 0.5f;
   const float hbx = floorf(hpx), hby = floorf(hpy);
@@ -1242,7 +1242,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   }
   float ix = fx, iy = fy, iz = fz;
   if (swh > 1e-4f) {
-    const float rh = F::rcp(swh);
+    const float rh = F::rcpn(swh);
     ix = F::divs(uBx, swh, rh) + pp.w * (fx - F::divs(uAx, swh, rh));
     iy = F::divs(uBy, swh, rh) + pp.w * (fy - F::divs(uAy, swh, rh));
     iz = F::divs(uBz, swh, rh) + pp.w * (fz - F::divs(uAz, swh, rh));
@@ -1265,7 +1265,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   }
   float upx = ix, upy = iy, upz = iz;
   if (swq > 1e-4f) {
-    const float rq = F::rcp(swq);
+    const float rq = F::rcpn(swq);
     upx = F::divs(uCx, swq, rq); upy = F::divs(uCy, swq, rq); upz = F::divs(uCz, swq, rq);
   }
   ix = ix + (upx - ix) * pp.z;
@@ -1490,9 +1490,13 @@ __global__ __launch_bounds__(NT) void k_math_selftest(int which, uint32_t base, 
   } else if (which == 1) {
     if (!(x <= 0.0f || x != x)) return;
     r = Math<false>::expn(x); ref = expf(x);
-  } else {
+  } else if (which == 2) {
     if (!(x >= __int_as_float(0xc2ce8ed0) && x <= __int_as_float(0x42b17218))) return;
     r = Math<false>::expm(x); ref = expf(x);
+  } else {
+    const uint32_t e = (u >> 23) & 0xffu;
+    if (!((e >= 1u && e <= 252u) || x != x)) return;          // normal, |x| < 2^126, or NaN
+    r = Math<false>::rcpn(x); ref = 1.0f / x;
   }
   if (__float_as_uint(r) == __float_as_uint(ref) || (r != r && ref != ref)) return;
   atomicAdd(bad, 1ull);
