@@ -159,6 +159,16 @@ template <> struct Math<false> {
     return x < __int_as_float(0xc2ce8ed0) ? 0.0f : r;        // -103.27893 = ln(2^-149)
   }
   static __device__ __forceinline__ float expm(float x) { return exp_core(x); }
+  // ln x for every x except a subnormal: the hardware log2, times ln 2 carried as a two-float constant,
+  // infinities and NaN passed through (logf without its rescaling of subnormal arguments).  log1p_'s
+  // 1 + x is never subnormal: near -1 the sum is exact, so it is 0 or at least 2^-24 in magnitude.
+  static __device__ __forceinline__ float logn(float x) {
+    const float L2 = __builtin_amdgcn_logf(x);
+    const float LN2 = __int_as_float(0x3f317217), LN2_LO = __int_as_float(0x3377d1cf);
+    const float h = L2 * LN2;
+    const float lo = __builtin_fmaf(L2, LN2_LO, __builtin_fmaf(L2, LN2, -h));
+    return fabsf(L2) < __builtin_inff() ? h + lo : L2;
+  }
 };
 template <> struct Math<true> {
   static __device__ __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
@@ -178,9 +188,10 @@ template <> struct Math<true> {
     return e == __builtin_inff() ? e : __builtin_fmaf(e, lo * 0.693147182f, e);   // no inf * lo on overflow
   }
   static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.693147182f; }
+  static __device__ __forceinline__ float logn(float x) { return log(x); }
 };
 
-template <bool FAST> __device__ __forceinline__ float log1p_(float x) { return Math<FAST>::log(1.0f + x); }  // hush.js:47
+template <bool FAST> __device__ __forceinline__ float log1p_(float x) { return Math<FAST>::logn(1.0f + x); }  // hush.js:47
 
 __device__ __forceinline__ float pow128(float x) {                            // hush.js:70-73
   float x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8, x32 = x16 * x16, x64 = x32 * x32;

@@ -1493,10 +1493,13 @@ __global__ __launch_bounds__(NT) void k_math_selftest(int which, uint32_t base, 
   } else if (which == 2) {
     if (!(x >= __int_as_float(0xc2ce8ed0) && x <= __int_as_float(0x42b17218))) return;
     r = Math<false>::expm(x); ref = expf(x);
-  } else {
+  } else if (which == 3) {
     const uint32_t e = (u >> 23) & 0xffu;
     if (!((e >= 1u && e <= 252u) || x != x)) return;          // normal, |x| < 2^126, or NaN
     r = Math<false>::rcpn(x); ref = 1.0f / x;
+  } else {
+    if ((u & 0x7fffffffu) >= 1u && (u & 0x7fffffffu) < 0x00800000u) return;   // every input but a subnormal
+    r = Math<false>::logn(x); ref = logf(x);
   }
   if (__float_as_uint(r) == __float_as_uint(ref) || (r != r && ref != ref)) return;
   atomicAdd(bad, 1ull);

@@ -157,13 +157,14 @@ int32_t hush_debug_run_pass(hush_t *h, int32_t pass, const hush_inputs *in, void
 /* enable the per-pixel reprojection mask: bit i (0..3) = bilinear tap i valid, bit 4 = 3x3 fallback
    used, bit 5 = history used (has), bit 7 = sky */
 int32_t hush_set_debug_mask(hush_t *h, int32_t enabled);
-/* self-test of the reduced-range square root, exponential and reciprocal that the kernels use where their
-   arguments allow it, against the toolchain's own sqrtf / expf / IEEE division evaluated on the current
-   device, over every binary32 input of the domain: which = 0 square root over x >= 2^-96, +0, +inf and
-   NaN; 1 exponential over x <= 0 and NaN; 2 exponential over -103.27893 <= x <= 88.72283; 3 reciprocal
-   over normal x with |x| < 2^126 and NaN.  *mismatches = number of inputs whose
-   results differ in any bit (any NaN equals any NaN), *first = the smallest such input's bits (0xffffffff
-   when there is none).  Synchronous; allocates 16 bytes of device memory for the counters. */
+/* self-test of the reduced-range square root, exponential, reciprocal and logarithm that the kernels use
+   where their arguments allow it, against the toolchain's own sqrtf / expf / IEEE division / logf evaluated
+   on the current device, over every binary32 input of the domain: which = 0 square root over x >= 2^-96,
+   +0, +inf and NaN; 1 exponential over x <= 0 and NaN; 2 exponential over -103.27893 <= x <= 88.72283;
+   3 reciprocal over normal x with |x| < 2^126 and NaN; 4 logarithm over every x but a subnormal.
+   *mismatches = number of inputs whose results differ in any bit (any NaN equals any NaN), *first = the
+   smallest such input's bits (0xffffffff when there is none).  Synchronous; allocates 16 bytes of device
+   memory for the counters. */
 int32_t hush_debug_math_selftest(int32_t which, uint64_t *mismatches, uint32_t *first);
 
 #ifdef __cplusplus

@@ -1,11 +1,12 @@
-"""The reduced-range square root, exponentials and reciprocal the kernels use where their arguments allow it
-(hush_device.h: Math<false>::sqrtn, expn, expm, rcpn) against the toolchain's sqrtf / expf / IEEE division on
-the GPU, over every binary32 input of each domain (hush_debug_math_selftest): bit-identical, so the IEEE
-policy stays the operation-exact mirror of the oracle.  The domains are those the call sites guarantee: square roots of
-max(v, 1e-12), of |mv|^2 (zero or at least 2^-48) and of the octahedral decode's squared norm (at least
-1/3); exponentials of the non-positive edge-stopping and joint-bilateral exponents and of the MLP outputs
-clamped to [-5, 5]; reciprocals of sums and counts bounded away from 0 and infinity (at least 1e-6, at
-most ~1.5e6)."""
+"""The reduced-range square root, exponentials, reciprocal and logarithm the kernels use where their
+arguments allow it (hush_device.h: Math<false>::sqrtn, expn, expm, rcpn, logn) against the toolchain's
+sqrtf / expf / IEEE division / logf on the GPU, over every binary32 input of each domain
+(hush_debug_math_selftest): bit-identical, so the IEEE policy stays the operation-exact mirror of the
+oracle.  The domains are those the call sites guarantee: square roots of max(v, 1e-12), of |mv|^2 (zero
+or at least 2^-48) and of the octahedral decode's squared norm (at least 1/3); exponentials of the
+non-positive edge-stopping and joint-bilateral exponents and of the MLP outputs clamped to [-5, 5];
+reciprocals of sums and counts bounded away from 0 and infinity (at least 1e-6, at most ~1.5e6);
+logarithms of 1 + x (zero or at least 2^-24 in magnitude near x = -1, never subnormal)."""
 import ctypes
 
 import pytest
@@ -16,7 +17,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("which,name", [(0, "sqrt, x >= 2^-96, +0, +inf, NaN"), (1, "exp, x <= 0, NaN"),
-                                        (2, "exp, -103.28 <= x <= 88.72"), (3, "rcp, normal |x| < 2^126, NaN")])
+                                        (2, "exp, -103.28 <= x <= 88.72"), (3, "rcp, normal |x| < 2^126, NaN"),
+                                        (4, "log, all but subnormals")])
 def test_reduced_range_math_matches_toolchain(which, name):
     L = _lib.load()
     bad, first = ctypes.c_uint64(0), ctypes.c_uint32(0)
