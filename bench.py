@@ -320,6 +320,10 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the MLP-stage, FP16-MFMA frame, SVGF, training and PSNR legs")
+    ap.add_argument("--mlp-stage-only", action="store_true",
+                    help="after the warm-up frames, time only the MLP stage (hush_mlp_forward over the frame's half-res "
+                         "feature rows, every precision, --steps launches each) and print it as one JSON line; for "
+                         "profiling the MLP kernels on their own")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run device-to-device copies of this size (the runtime's copy kernel calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
@@ -356,6 +360,9 @@ def main():
 
     run(args.warmup)
     stream.synchronize()
+    if args.mlp_stage_only:
+        print(json.dumps({"mlp_stage": mlp_stage(hs, hip, W, H, reps=args.steps), "width": W, "height": H}), flush=True)
+        return
     e0, e1 = hip.Event(), hip.Event()
     hip.synchronize()
     e0.record(stream.handle)
