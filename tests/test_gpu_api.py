@@ -1,0 +1,161 @@
+"""The Hush state machine on the GPU through the Python API (hush_amd.Hush over the C ABI), end to end
+against the FP32-mirror oracle run through the same calls: resize in the middle of a sequence (hush.js:
+764-794: reallocates and resets; same size is a no-op), reset (hush.js:801), an exposure change between
+frames, sizes down to 1x1 and sizes one texel past a tile edge, MLP hidden widths other than 16
+(hiddenOf, hush.js:22-25, with the generic-width kernel), the shader-f16 MLP variant (hush.js:348-356
+with f16: true), and the reference's input errors (hush.js:813-822).
+
+Tolerance (as the end-to-end parity test): HDR output within 1e-3 relative (max(|ref|, 1e-3)) for
+>= 99.9% of values, none beyond 1e-2; reprojection masks bit-exact.  A reset frame must be bit-identical
+to the first frame of a fresh context."""
+import numpy as np
+import pytest
+
+import hush_amd
+from oracle.ho import Oracle
+from tests.parity import compare, upload_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(w, h, seed, kind="boxes"):
+    from hush_amd.scene import SceneGenerator
+    return SceneGenerator(kind, w, h, seed)
+
+
+def _check(stats, what):
+    for st in stats:
+        assert st["mask_exact"] == 1.0, (what, st)
+        assert st["frac_rel_gt_1e-3"] <= 1e-3, (what, st)
+        assert st["max_rel"] <= 1e-2, (what, st)
+
+
+def _step(hs, o, g, surfs):
+    surfs = upload_frame(g, surfs)
+    out = hs.denoise(surfs).download().astype(np.float32)
+    ref = o.denoise(g)
+    st = compare(out[..., :3], ref[..., :3])
+    st["mask_exact"] = float(np.mean(hs.surface("mask").download()[..., 0] == o.mask()))
+    return st, surfs
+
+
+def _pair(w, h, **opts):
+    weights = opts.get("weights")
+    hs = hush_amd.Hush(0, dict({"width": w, "height": h}, **opts))
+    hs.set_debug_mask(True)
+    o = Oracle(w, h, hush_amd.default_weights() if weights is None else weights,
+               exposure=opts.get("exposure", 1.0), f16=opts.get("precision") == "f16")
+    return hs, o
+
+
+def test_resize_midsequence_matches_oracle():
+    hs, o = _pair(64, 48)
+    stats, surfs = [], None
+    g1 = _scene(64, 48, 31)
+    for f in range(3):
+        st, surfs = _step(hs, o, g1.render(f), surfs)
+        stats.append(st)
+    hs.resize(97, 61)
+    o.resize(97, 61)
+    g2 = _scene(97, 61, 32)
+    surfs = None
+    for f in range(3):
+        st, surfs = _step(hs, o, g2.render(f), surfs)
+        stats.append(st)
+    n = hs.frame
+    hs.resize(97, 61)                      # same size: no reallocation, history kept
+    o.resize(97, 61)
+    assert hs.frame == n
+    for f in range(3, 5):
+        st, surfs = _step(hs, o, g2.render(f), surfs)
+        stats.append(st)
+    hs.destroy()
+    _check(stats, "resize")
+
+
+def test_reset_equals_fresh_first_frame():
+    w, h = 80, 56
+    gen = _scene(w, h, 33)
+    frames = [gen.render(f) for f in range(5)]
+    a = hush_amd.Hush(0, {"width": w, "height": h})
+    surfs = None
+    for g in frames[:4]:
+        surfs = upload_frame(g, surfs)
+        a.denoise(surfs)
+    a.reset()
+    surfs = upload_frame(frames[4], surfs)
+    out_a = a.denoise(surfs).download()
+    b = hush_amd.Hush(0, {"width": w, "height": h})
+    out_b = b.denoise(surfs).download()
+    a.destroy()
+    b.destroy()
+    assert np.array_equal(out_a.view(np.uint16), out_b.view(np.uint16))
+
+
+def test_exposure_change_matches_oracle():
+    hs, o = _pair(96, 64, exposure=1.0)
+    gen = _scene(96, 64, 34, "spheres")
+    stats, surfs = [], None
+    for f in range(4):
+        if f == 2:
+            hs.exposure = 2.5
+            o.set_exposure(2.5)
+        st, surfs = _step(hs, o, gen.render(f), surfs)
+        stats.append(st)
+    assert abs(hs.exposure - 2.5) < 1e-12
+    hs.destroy()
+    _check(stats, "exposure")
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (2, 1), (1, 5), (3, 3), (33, 9), (65, 17)])
+def test_small_and_ragged_sizes_match_oracle(w, h):
+    hs, o = _pair(w, h)
+    gen = _scene(w, h, 35)
+    stats, surfs = [], None
+    for f in range(3):
+        st, surfs = _step(hs, o, gen.render(f), surfs)
+        stats.append(st)
+    hs.destroy()
+    _check(stats, f"{w}x{h}")
+
+
+@pytest.mark.parametrize("hidden", [8, 32])
+def test_other_hidden_widths_match_oracle(hidden):
+    from hush_amd.train import random_weights, to_f16_blob
+    wts = to_f16_blob(random_weights(40 + hidden, hidden))
+    assert hush_amd.hiddenOf(wts.size) == hidden
+    hs, o = _pair(96, 64, weights=wts)
+    assert hs.hidden == hidden
+    gen = _scene(96, 64, 36)
+    stats, surfs = [], None
+    for f in range(3):
+        st, surfs = _step(hs, o, gen.render(f), surfs)
+        stats.append(st)
+    hs.destroy()
+    _check(stats, f"hidden {hidden}")
+
+
+def test_f16_mlp_variant_matches_oracle():
+    hs, o = _pair(96, 64, precision="f16")
+    gen = _scene(96, 64, 37)
+    stats, surfs = [], None
+    for f in range(3):
+        st, surfs = _step(hs, o, gen.render(f), surfs)
+        stats.append(st)
+    hs.destroy()
+    _check(stats, "f16")
+
+
+def test_input_errors_match_reference():
+    from hush_amd.hip import Surface
+    from tests.parity import INPUT_KEYS
+    hs = hush_amd.Hush(0, {"width": 64, "height": 48})
+    good = {k: Surface(64, 48) for k in INPUT_KEYS}
+    bad = dict(good, motion=Surface(32, 48))
+    with pytest.raises(hush_amd.HushError, match=r"^Hush\.denoise: 'motion' is 32x48, the denoiser 64x48 \(call resize\)$"):
+        hs.denoise(bad)
+    missing = {k: v for k, v in good.items() if k != "material"}
+    with pytest.raises(hush_amd.HushError, match=r"^Hush\.denoise: missing input 'material'$"):
+        hs.denoise(missing)
+    hs.denoise(good)                                   # the context is still usable
+    hs.destroy()
