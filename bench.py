@@ -78,23 +78,26 @@ def mlp_stage(hs, hip, W, H, reps=20):
     return res
 
 
-def psnr_check(precisions, w=640, h=360, frames=8):
-    """Error of each MLP precision's output against the FP64 oracle over an 8-frame sequence of the same
-    scene type at 640x360: PSNR (tone-mapped, peak 1, mean over frames) against the FP64 ideal oracle (no
-    binary16 storage) and its change from the FP32 path; and, against the FP64 oracle with the same
-    binary16 storage (which isolates the arithmetic), the 99.9th percentile of the HDR output's relative
-    error |gpu - ref| / max(|ref|, 1e-3) and the largest absolute error of the tone-mapped output (the HDR
-    maximum is set by the few pixels where a threshold decision flips between FP32 and FP64)"""
+def psnr_check(precisions, w=640, h=360, frames=8, seed=1):
+    """Error of each MLP precision's output against the FP64 oracle over a sequence of the boxes scene:
+    PSNR (tone-mapped, peak 1, mean over frames) against the FP64 ideal oracle (no binary16 storage) and its
+    change from the FP32 path; and, against the FP64 oracle with the same binary16 storage (which isolates
+    the arithmetic), the 99.9th percentile and the maximum of the HDR output's relative error
+    |gpu - ref| / max(|ref|, 1e-3) and the largest absolute error of the tone-mapped output (the maxima are
+    set by the few pixels where a threshold decision flips between FP32 and FP64, or where TAA's inverse
+    tone map saturates)"""
     from oracle.ho import Oracle
     from tests.parity import psnr, tonemap, upload_frame
     import hush_amd
     from hush_amd.scene import SceneGenerator
-    gen = SceneGenerator("boxes", w, h, seed=1)
+    gen = SceneGenerator("boxes", w, h, seed=seed)
     gs = [gen.render(f) for f in range(frames)]
     o = Oracle(w, h, hush_amd.default_weights(), mode="fp64")
     refs = [o.denoise(g) for g in gs]
+    o.close()
     oq = Oracle(w, h, hush_amd.default_weights(), mode="fp64q")
     refq = [oq.denoise(g)[..., :3] for g in gs]
+    oq.close()
     out = {}
     for p in precisions:
         prec, comp = (p[:-len("+compensated")], True) if p.endswith("+compensated") else (p, False)
@@ -104,14 +107,16 @@ def psnr_check(precisions, w=640, h=360, frames=8):
             surfs = upload_frame(g, surfs)
             o_ = hs.denoise(surfs).download().astype(np.float64)[..., :3]
             vals.append(psnr(o_.astype(np.float32), r))
-            rels.append((np.abs(o_ - rq) / np.maximum(np.abs(rq), 1e-3)).ravel())
+            rels.append((np.abs(o_ - rq) / np.maximum(np.abs(rq), 1e-3)).ravel().astype(np.float32))
             tmax = max(tmax, float(np.abs(tonemap(o_) - tonemap(rq)).max()))
         hs.destroy()
         rel = np.concatenate(rels)
         out[p] = {"psnr_db": round(float(np.mean(vals)), 3), "p999_rel_hdr_vs_fp64_storage": round(float(np.quantile(rel, 0.999)), 5),
+                  "max_rel_hdr_vs_fp64_storage": round(float(rel.max()), 5),
                   "max_abs_tonemapped_vs_fp64_storage": round(tmax, 5)}
     for p in precisions:
         out[p]["psnr_delta_vs_fp32_db"] = round(out[p]["psnr_db"] - out["fp32"]["psnr_db"], 3)
+    out["frames"] = frames
     return out
 
 
@@ -324,6 +329,8 @@ def main():
                     help="after the warm-up frames, time only the MLP stage (hush_mlp_forward over the frame's half-res "
                          "feature rows, every precision, --steps launches each) and print it as one JSON line; for "
                          "profiling the MLP kernels on their own")
+    ap.add_argument("--uhd-error-frames", type=int, default=8,
+                    help="frames of the 3840x2160 error check against the FP64 oracle (CPU-bound: ~4 s per frame)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run device-to-device copies of this size (the runtime's copy kernel calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
@@ -426,6 +433,12 @@ def main():
         print(f"[bench] 2160p {extras['uhd_2160p']}", file=sys.stderr, flush=True)
         extras["error_vs_fp64_oracle_640x360"] = psnr_check(["fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma", "fp32+compensated",
                                                               "fp16-mfma+compensated"])
+        # the same at 3840x2160 on a bounded sample (the first 8 frames of the 2160p sequence's scene type):
+        # the FP64 oracle takes ~2 s per 2160p frame and mode on the host cores
+        t0 = time.perf_counter()
+        extras["error_vs_fp64_oracle_3840x2160"] = psnr_check(["fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma"], 3840, 2160,
+                                                               frames=args.uhd_error_frames, seed=2)
+        print(f"[bench] 2160p error check {time.perf_counter() - t0:.0f}s", file=sys.stderr, flush=True)
         print(f"[bench] extras {extras}", file=sys.stderr, flush=True)
 
     if args.copy_probe_mb:
