@@ -81,6 +81,7 @@ struct TaaArgs {
 };
 
 hipError_t launch_temporal(const TemporalArgs &a, bool fast, hipStream_t s);
+hipError_t launch_mlp_tex(int mode, TexU F0, TexU F1, TexU F2, int W2, int H2, float *out, const void *wq, hipStream_t s);
 hipError_t launch_math_selftest(int which, uint32_t base, uint32_t blocks, unsigned long long *bad, uint32_t *first,
                                 hipStream_t s);
 hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s);

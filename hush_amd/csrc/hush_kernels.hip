@@ -902,6 +902,39 @@ __global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
   }
 }
 
+// The MLP's 12 outputs (before the activations, bias included) of every half-res texel, read from F0..F2
+// in place with the binary16 / bfloat16 MFMA forms: the training forward's MFMA option.  Texels in raster
+// order, 64 per wave in four 16-texel tiles; lane (kb, tl) of tile g loads the B fragments of texel
+// 64w + 16g + tl as k_infer_mfma does.  Sky texels are computed too (the caller masks them).
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_mlp_tex(TexU F0, TexU F1, TexU F2, int W2, int H2, float *__restrict__ out,
+                                                 const void *__restrict__ wq) {
+  const MfmaPack *__restrict__ P = (const MfmaPack *)wq;
+  const int tid = threadIdx.x, lane = tid & 63, kb = lane >> 4, tl = lane & 15;
+  const long base = (long)blockIdx.x * NT + (tid & ~63), n = (long)W2 * H2;
+  const TexU &Fk = kb < 2 ? F0 : F1;
+  uint2 x0[4], x1[4];
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const long r = base + 16 * g + tl;
+    const bool live = r < n;
+    const int x = live ? (int)(r % W2) : 0, y = live ? (int)(r / W2) : 0;
+    const uint2 *rowk = (const uint2 *)(Fk.p + (size_t)y * Fk.pitch + x);
+    const uint2 *row2 = (const uint2 *)(F2.p + (size_t)y * F2.pitch + x);
+    x0[g] = live ? rowk[kb & 1] : make_uint2(0, 0);
+    x1[g] = (live && kb == 0) ? row2[0] : make_uint2(0, 0);
+  }
+  f32x4 d[4];
+  mlp_mfma_x4<MODE>(P, x0, x1, d, lane);
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const long r = base + 16 * g + tl;
+    if (r < n && kb < 3)
+      *(float4 *)(out + r * 12 + 4 * kb) = make_float4(d[g][0] + P->b3[4 * kb], d[g][1] + P->b3[4 * kb + 1],
+                                                       d[g][2] + P->b3[4 * kb + 2], d[g][3] + P->b3[4 * kb + 3]);
+  }
+}
+
 // The MLP alone over n feature rows (20 binary16 each, row-major, 40 B) -> n x 12 fp32 outputs.  Used to
 // time the MLP stage in isolation and by training tools.  The MFMA forms read their B fragments and write
 // their D fragments straight from / to global memory (no LDS): lane l of group g serves row
@@ -1439,6 +1472,14 @@ hipError_t launch_pack_mlp(const float *w, MfmaPack *p, float s1, float s2, floa
   return hipGetLastError();
 }
 
+hipError_t launch_mlp_tex(int mode, TexU F0, TexU F1, TexU F2, int W2, int H2, float *out, const void *wq, hipStream_t s) {
+  const long n = (long)W2 * H2;
+  const dim3 g((unsigned)((n + NT - 1) / NT));
+  if (mode == MLP_FP16_MFMA) hipLaunchKernelGGL(k_mlp_tex<MLP_FP16_MFMA>, g, dim3(NT), 0, s, F0, F1, F2, W2, H2, out, wq);
+  else if (mode == MLP_BF16_MFMA) hipLaunchKernelGGL(k_mlp_tex<MLP_BF16_MFMA>, g, dim3(NT), 0, s, F0, F1, F2, W2, H2, out, wq);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
 hipError_t launch_mlp(int mode, const uint32_t *feat, float *out, int n, const float *w, const void *wq, hipStream_t s) {
   dim3 g((n + NT - 1) / NT);
   switch (mode) {

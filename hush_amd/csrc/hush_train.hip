@@ -98,6 +98,19 @@ __device__ __forceinline__ void mlp_fwd(const T *w, const T *f, T *h1, T *h2, T 
   }
 }
 
+// activations (hush.js:357-366): par = (alpha, gamma, beta, kappa, s0..s3, c0..c3) from the 12 outputs
+template <class T>
+__device__ __forceinline__ void acts(const T *r, T *pp) {
+  pp[0] = T(1.0 / 32.0) + T(31.0 / 32.0) * sigm(r[0]);
+  pp[1] = T(0.5) + T(7.5) * sigm(r[1]);
+  pp[2] = sigm(r[2]);
+  pp[3] = sigm(r[3]);
+  for (int k = 0; k < 4; k++) {
+    pp[4 + k] = T(0.25) * texp(tmin(tmax(r[4 + k], T(-5)), T(5)));
+    pp[8 + k] = texp(tmin(tmax(r[8 + k], T(-4)), T(4)));
+  }
+}
+
 // MLP and activations per half-res texel (hush.js:348-366): o (12) and par = (alpha, gamma, beta, kappa,
 // s0..s3, c0..c3); sky blocks give zeros
 template <class T>
@@ -114,14 +127,22 @@ __global__ __launch_bounds__(NTH) void tk_infer(TexU F0, TexU F1, TexU F2, TexF 
   features(F0, F1, F2, x, y, f);
   mlp_fwd(w, f, h1, h2, r);
   for (int k = 0; k < 12; k++) oo[k] = r[k];
-  pp[0] = T(1.0 / 32.0) + T(31.0 / 32.0) * sigm(r[0]);
-  pp[1] = T(0.5) + T(7.5) * sigm(r[1]);
-  pp[2] = sigm(r[2]);
-  pp[3] = sigm(r[3]);
-  for (int k = 0; k < 4; k++) {
-    pp[4 + k] = T(0.25) * texp(tmin(tmax(r[4 + k], T(-5)), T(5)));
-    pp[8 + k] = texp(tmin(tmax(r[8 + k], T(-4)), T(4)));
+  acts(r, pp);
+}
+
+// the activations alone, on MLP outputs already in o (the MFMA forward option); sky blocks give zeros
+template <class T>
+__global__ __launch_bounds__(NTH) void tk_act(TexF GH, int W2, int H2, T *o, T *par) {
+  const int i = tid1();
+  if (i >= W2 * H2) return;
+  T *oo = o + 12 * i, *pp = par + 12 * i;
+  if (guide<T>(GH, i % W2, i / W2).z < T(0)) {
+    for (int k = 0; k < 12; k++) { oo[k] = T(0); pp[k] = T(0); }
+    return;
   }
+  T r[12];
+  for (int k = 0; k < 12; k++) r[k] = oo[k];
+  acts(r, pp);
 }
 
 // ------------------------------------------------------------------------------------------- a-trous
@@ -897,6 +918,8 @@ struct hush_train_ctx {
   bool has_prev = false;                // Fprev holds the committed previous frame
   int prev_mode = 0;                    // this frame: 0 no history, 1 alpha/gamma from the stored par0, 2 recomputed
   bool a0_fixed = false;
+  int fwd_prec = HUSH_MLP_FP32;         // forward MLP: FP32, or an MFMA form over packed weights (wq)
+  void *wq = nullptr;
   hush_inputs in{};
   bool prepared = false, forwarded = false, reset = true;
   char err[512] = {0};
@@ -959,6 +982,7 @@ extern "C" void hush_train_destroy(hush_train_t *t) {
   if (!t) return;
   (void)hipSetDevice(t->h->device);
   if (t->arena) (void)hipFree(t->arena);
+  if (t->wq) (void)hipFree(t->wq);
   delete t;
 }
 
@@ -1026,14 +1050,35 @@ template <class T>
 static hipError_t forward_t(hush_train_t *t, const T *w, T *out, hipStream_t s) {
   hush_t *h = t->h;
   const long N = (long)t->W * t->H, N2 = (long)t->W2 * t->H2;
+  // MFMA forward option (FP32 contexts only): the weights are packed for the matrix cores on every forward,
+  // the MLP runs over the feature surfaces in place, the activations follow; the backward stays FP32
+  const bool mf = sizeof(T) == sizeof(float) && t->fwd_prec != HUSH_MLP_FP32;
+  if (mf) {
+    hipError_t e = launch_pack_mlp((const float *)w, (MfmaPack *)t->wq, 1.0f, 1.0f, 1.0f, s);
+    if (e != hipSuccess) return e;
+  }
+  auto mlp = [&](TexU F0, TexU F1, TexU F2, TexF GH, void *o, void *par) -> hipError_t {
+    if (!mf) {
+      hipLaunchKernelGGL(tk_infer<T>, grid1(N2), dim3(NTH), 0, s, F0, F1, F2, GH, t->W2, t->H2, w, (T *)o, (T *)par);
+      return hipGetLastError();
+    }
+    hipError_t e = launch_mlp_tex(t->fwd_prec, F0, F1, F2, t->W2, t->H2, (float *)o, t->wq, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(tk_act<T>, grid1(N2), dim3(NTH), 0, s, GH, t->W2, t->H2, (T *)o, (T *)par);
+    return hipGetLastError();
+  };
   if (!t->a0_fixed) {
     // the previous frame's alpha / gamma with these weights (one step back through the temporal blend)
-    if (t->prev_mode == 2)
-      hipLaunchKernelGGL(tk_infer<T>, grid1(N2), dim3(NTH), 0, s, fprev_tex(t, 0), fprev_tex(t, 1), fprev_tex(t, 2), TexF{t->GHprev, t->W2}, t->W2,
-                         t->H2, w, (T *)t->oprev, (T *)t->pprev);
+    if (t->prev_mode == 2) {
+      hipError_t e = mlp(fprev_tex(t, 0), fprev_tex(t, 1), fprev_tex(t, 2), TexF{t->GHprev, t->W2}, t->oprev, t->pprev);
+      if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(tk_blend<T>, grid1(N), dim3(NTH), 0, s, (const float *)t->aux, (const T *)t->pprev, (int)N, (T *)t->a0);
   }
-  hipLaunchKernelGGL(tk_infer<T>, grid1(N2), dim3(NTH), 0, s, h->F[0], h->F[1], h->F[2], h->gh, t->W2, t->H2, w, (T *)t->o, (T *)t->par);
+  {
+    hipError_t e = mlp(h->F[0], h->F[1], h->F[2], h->gh, t->o, t->par);
+    if (e != hipSuccess) return e;
+  }
   // mid1 (R 1, step 1, prefilter, source = block means), mid2 (step 2), mid3 (step 4) (hush.js:366, 753-754)
   hipLaunchKernelGGL((tk_filt<T, 1, 1, true, 0, false, false>), grid1(N2), dim3(NTH), 0, s, filt_args<T>(t, 1, nullptr, nullptr));
   hipLaunchKernelGGL((tk_filt<T, 1, 2, false, 1, false, false>), grid1(N2), dim3(NTH), 0, s, filt_args<T>(t, 2, nullptr, nullptr));
@@ -1055,6 +1100,21 @@ static hipError_t forward_t(hush_train_t *t, const T *w, T *out, hipStream_t s) 
   ta.W = t->W; ta.H = t->H; ta.reset = t->reset ? 1 : 0; ta.out = out;
   hipLaunchKernelGGL((tk_taa<T, false>), grid1(N), dim3(NTH), 0, s, ta);
   return hipGetLastError();
+}
+
+extern "C" int32_t hush_train_set_forward_precision(hush_train_t *t, int32_t precision) {
+  if (!t) return tfail(nullptr, HUSH_ERR_ARGUMENT, "hush_train_set_forward_precision: null context");
+  if (precision != HUSH_MLP_FP32 && precision != HUSH_MLP_FP16_MFMA && precision != HUSH_MLP_BF16_MFMA)
+    return tfail(t, HUSH_ERR_ARGUMENT, "hush_train_set_forward_precision: the training forward takes fp32, fp16-mfma or bf16-mfma");
+  if (precision != HUSH_MLP_FP32 && t->fp64)
+    return tfail(t, HUSH_ERR_UNSUPPORTED, "hush_train_set_forward_precision: the MFMA forward needs an FP32 context");
+  if (precision != HUSH_MLP_FP32 && !t->wq) {
+    hipError_t e = hipSetDevice(t->h->device);
+    if (e == hipSuccess) e = hipMalloc(&t->wq, sizeof(MfmaPack));
+    if (e != hipSuccess) { t->wq = nullptr; return tfail(t, HUSH_ERR_DEVICE, "hush_train_set_forward_precision: hipMalloc", e); }
+  }
+  t->fwd_prec = precision;
+  return HUSH_OK;
 }
 
 extern "C" int32_t hush_train_forward(hush_train_t *t, const void *weights, void *out, void *stream) {
@@ -1194,5 +1254,11 @@ extern "C" int32_t hush_train_set_a0_fixed(hush_train_t *t, int32_t fixed) {
 extern "C" int32_t hush_train_a0(hush_train_t *t, void **ptr) {
   if (!t || !ptr) return tfail(t, HUSH_ERR_ARGUMENT, "hush_train_a0: null argument");
   *ptr = t->a0;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_train_mlp_out(hush_train_t *t, void **ptr) {
+  if (!t || !ptr) return tfail(t, HUSH_ERR_ARGUMENT, "hush_train_mlp_out: null argument");
+  *ptr = t->o;
   return HUSH_OK;
 }

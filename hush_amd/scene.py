@@ -100,6 +100,14 @@ class GpuSceneGenerator:
     def is_cut(self, frame: int) -> bool:
         return self.cpu.is_cut(frame)
 
+    def set_seed(self, seed: int) -> None:
+        """Switches to the scene of another seed (same kind and size), keeping the GPU renderer."""
+        old = self.cpu
+        self.cpu = SceneGenerator(self.kind, self.width, self.height, seed)
+        old.close()
+        self._n = self.cpu._L.hs_num_prims(self.cpu._s)
+        self._seed = self.cpu._L.hs_seed(self.cpu._s)
+
     def setup(self, frame: int, crop=None):
         """the host half: (cams, prims, off, lst) numpy arrays of hs_frame_setup"""
         x0, y0, w, h = crop if crop is not None else (0, 0, self.width, self.height)

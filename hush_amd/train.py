@@ -38,6 +38,8 @@ def _load():
     L.hush_train_commit.argtypes = [P, P, P]; L.hush_train_commit.restype = i32
     L.hush_train_a0.argtypes = [P, ctypes.POINTER(P)]; L.hush_train_a0.restype = i32
     L.hush_train_set_a0_fixed.argtypes = [P, i32]; L.hush_train_set_a0_fixed.restype = i32
+    L.hush_train_set_forward_precision.argtypes = [P, i32]; L.hush_train_set_forward_precision.restype = i32
+    L.hush_train_mlp_out.argtypes = [P, ctypes.POINTER(P)]; L.hush_train_mlp_out.restype = i32
     L._train_declared = True
     return L
 
@@ -52,9 +54,11 @@ def _ptr(t):
 
 
 class DiffFrame:
-    """The differentiable frame attached to a Hush context; fp64=True for gradient checks."""
+    """The differentiable frame attached to a Hush context; fp64=True for gradient checks.  `forward`
+    selects the MLP arithmetic of the forward ("fp32", or "fp16-mfma" / "bf16-mfma" in an FP32 context:
+    mixed-precision training with an FP32 backward)."""
 
-    def __init__(self, hush: Hush, fp64: bool = False):
+    def __init__(self, hush: Hush, fp64: bool = False, forward: str = "fp32"):
         import torch
         self._L = _load()
         self.hush = hush
@@ -67,6 +71,9 @@ class DiffFrame:
         self._t = p.value
         self.width, self.height = hush.width, hush.height
         self._keep = None
+        if forward != "fp32":
+            from . import PRECISIONS
+            self._check(self._L.hush_train_set_forward_precision(self._t, PRECISIONS[forward]))
 
     def _check(self, r):
         if r != 0:
@@ -116,6 +123,16 @@ class DiffFrame:
         t = torch.empty((self.height, self.width, 4), dtype=self.dtype, device="cuda")
         _memcpy_d2d(t.data_ptr(), p.value, t.numel() * t.element_size())
         return t
+
+    def mlp_outputs(self):
+        """the last forward's MLP outputs before the activations, (H/2, W/2, 12) numpy copy"""
+        import torch
+        p = ctypes.c_void_p()
+        self._check(self._L.hush_train_mlp_out(self._t, ctypes.byref(p)))
+        h2, w2 = (self.height + 1) // 2, (self.width + 1) // 2
+        t = torch.empty((h2, w2, 12), dtype=self.dtype, device="cuda")
+        _memcpy_d2d(t.data_ptr(), p.value, t.numel() * t.element_size())
+        return t.cpu().numpy()
 
     def set_a0(self, a0) -> None:
         p = ctypes.c_void_p()
@@ -226,57 +243,98 @@ def target_of(g: dict, dtype):
 
 
 def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256), scene_size=(640, 360), lr: float = 1e-3,
-          lam: float = 0.5, seed: int = 0, init: str = "random", log=print, gpu_scenes: bool = True):
-    """Adam over seeded procedural sequences ('train' scenes: 300 primitives, 30% moving, camera cuts).
-    Each step: `sequences` crops x `frames` frames; the gradient of every frame accumulates (the history
-    enters each frame as an input).  With gpu_scenes the crops and their clean targets are rendered on the
-    GPU straight into device tensors.  Returns (weights tensor, [(step, loss)])."""
+          lam: float = 0.5, seed: int = 0, init: str = "random", log=print, gpu_scenes: bool = True, forward: str = "fp32",
+          ckpt: str | None = None, ckpt_every: int = 0, resume: str | None = None, time_limit: float | None = None):
+    """Adam over seeded procedural sequences ('train' scenes: 300 primitives, 30% moving, camera cuts every 48
+    frames, which reset the history).  Each step: `sequences` crops of `scene_size` frames x `frames` frames;
+    the gradient of every frame accumulates (the history enters each frame as an input).  `forward` selects
+    the MLP arithmetic of the forward ("fp32", "fp16-mfma", "bf16-mfma"; the backward is FP32).  One denoiser
+    and training context per sequence slot is reused across steps (reset at every sequence start); with
+    gpu_scenes the crops and their clean targets are rendered on the GPU straight into device tensors.
+    `ckpt` (+ `ckpt_every`) saves weights, Adam state and step; `resume` continues from such a file;
+    `time_limit` (seconds) stops early after a whole step.  Returns (weights tensor, [(step, loss)])."""
     import torch
     from .scene import GpuSceneGenerator, SceneGenerator
     w0 = random_weights(seed) if init == "random" else weights_to_float(default_weights())
     weights = torch.tensor(w0, dtype=torch.float32, device="cuda", requires_grad=True)
     opt = torch.optim.Adam([weights], lr=lr)
+    hist, start = [], 0
+    if resume:
+        st = torch.load(resume, map_location="cuda", weights_only=True)
+        with torch.no_grad():
+            weights.copy_(st["weights"])
+        opt.load_state_dict(st["opt"])
+        hist = [tuple(x) for x in st["hist"].tolist()]
+        start = int(st["step"]) + 1
+
+    def save(step):
+        torch.save({"weights": weights.detach(), "opt": opt.state_dict(), "step": step,
+                    "hist": torch.tensor(hist, dtype=torch.float64)}, ckpt)
+
     cw, ch = crop
-    hist = []
+    slots = []
+    for s in range(sequences):
+        hs = Hush(0, {"width": cw, "height": ch})      # temporal / coarse passes only: no MLP weights used
+        fr = DiffFrame(hs, forward=forward)
+        gen = GpuSceneGenerator("train", scene_size[0], scene_size[1], 0) if gpu_scenes else None
+        ins = {k: torch.empty((ch, cw, 4), dtype=torch.float16, device="cuda") for k in GpuSceneGenerator.KEYS}
+        tgt = [torch.empty((ch, cw, 4), dtype=torch.float32, device="cuda") for _ in range(2)]
+        slots.append((hs, fr, gen, ins, tgt))
     t0 = time.perf_counter()
-    for step in range(steps):
+    for step in range(start, steps):
         opt.zero_grad(set_to_none=False)
         total = 0.0
         for s in range(sequences):
+            hs, fr, gen, ins, tgts = slots[s]
             rs = np.random.default_rng([seed, step, s])
             gseed = int(rs.integers(1 << 30))
-            gen = (GpuSceneGenerator if gpu_scenes else SceneGenerator)("train", scene_size[0], scene_size[1], gseed)
+            if gpu_scenes:
+                gen.set_seed(gseed)
+            else:
+                gen = SceneGenerator("train", scene_size[0], scene_size[1], gseed)
             x0 = int(rs.integers(0, scene_size[0] - cw + 1))
             y0 = int(rs.integers(0, scene_size[1] - ch + 1))
             f0 = int(rs.integers(0, 96))
-            hs = Hush(0, {"width": cw, "height": ch})      # temporal / coarse passes only: no MLP weights used
-            fr = DiffFrame(hs)
+            hs.reset()
             prev_out = prev_tgt = None
             for f in range(frames):
                 if gpu_scenes:
-                    ins = {k: torch.empty((ch, cw, 4), dtype=torch.float16, device="cuda") for k in GpuSceneGenerator.KEYS}
-                    tgt = torch.empty((ch, cw, 4), dtype=torch.float32, device="cuda")
+                    tgt = tgts[f & 1]
                     gen.render(f0 + f, ins, crop=(x0, y0, cw, ch), clean=tgt)
+                    fins = ins
                 else:
                     g = gen.render(f0 + f, crop=(x0, y0, cw, ch), clean=True)
-                    ins, tgt = upload(g), target_of(g, torch.float32)
+                    fins, tgt = upload(g), target_of(g, torch.float32)
                 if gen.is_cut(f0 + f):
                     hs.reset()
-                fr.prepare(ins)
+                    prev_out = prev_tgt = None
+                fr.prepare(fins)
                 out = frame_apply(weights, fr)
                 loss = loss_apply(out, tgt, prev_out, prev_tgt, lam if prev_out is not None else 0.0, fr)
                 (loss / (sequences * frames)).backward()
-                total += float(loss)
+                total += float(loss.detach())
                 fr.commit(out.detach())
                 prev_out, prev_tgt = out.detach(), tgt
-            fr.destroy()
-            hs.destroy()
-            if gpu_scenes:
+            if not gpu_scenes:
                 gen.close()
         opt.step()
         hist.append((step, total / (sequences * frames)))
-        if log and (step % max(1, steps // 20) == 0 or step == steps - 1):
+        last = step == steps - 1
+        if log and (step % max(1, steps // 20) == 0 or last):
             log(f"[train] step {step} loss {hist[-1][1]:.5f} ({time.perf_counter() - t0:.1f}s)")
+        if ckpt and ((ckpt_every and step % ckpt_every == ckpt_every - 1) or last):
+            save(step)
+        if time_limit is not None and time.perf_counter() - t0 > time_limit and not last:
+            if log:
+                log(f"[train] time limit: stopped after step {step}")
+            if ckpt:
+                save(step)
+            break
+    for hs, fr, gen, _, _ in slots:
+        fr.destroy()
+        hs.destroy()
+        if gen is not None:
+            gen.close()
     return weights.detach(), hist
 
 
@@ -350,16 +408,31 @@ def main(argv=None):
     ap.add_argument("--init", choices=["random", "builtin"], default="random")
     ap.add_argument("--out", default=None, help="write the trained weights (812 binary16, hush-weights.bin layout)")
     ap.add_argument("--eval-frames", type=int, default=24)
+    ap.add_argument("--scene-size", type=int, nargs=2, default=(640, 360), help="size of the scenes the crops are taken from")
+    ap.add_argument("--forward", choices=["fp32", "fp16-mfma", "bf16-mfma"], default="fp32",
+                    help="MLP arithmetic of the forward (the backward is FP32)")
+    ap.add_argument("--ckpt", default=None, help="checkpoint file (weights, Adam state, step, loss history)")
+    ap.add_argument("--ckpt-every", type=int, default=0)
+    ap.add_argument("--resume", default=None, help="continue from a checkpoint file")
+    ap.add_argument("--time-limit", type=float, default=None, help="stop after the step that passes this many seconds")
+    ap.add_argument("--no-eval", action="store_true", help="skip the validation (e.g. when the run will be resumed)")
+    ap.add_argument("--eval-size", type=int, nargs=2, default=(640, 360), help="size of the held-out validation sequences")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    w, hist = train(args.steps, args.sequences, args.frames, tuple(args.crop), lr=args.lr, seed=args.seed, init=args.init, log=log)
+    w, hist = train(args.steps, args.sequences, args.frames, tuple(args.crop), scene_size=tuple(args.scene_size), lr=args.lr,
+                    seed=args.seed, init=args.init, log=log, forward=args.forward, ckpt=args.ckpt, ckpt_every=args.ckpt_every,
+                    resume=args.resume, time_limit=args.time_limit)
     blob = to_f16_blob(w.cpu().numpy())
     if args.out:
         blob.tofile(args.out)
-    sets = {"trained": blob, "builtin": default_weights(), "random_init": to_f16_blob(random_weights(args.seed))}
-    res = evaluate(sets, frames=args.eval_frames)
-    print(json.dumps({"steps": args.steps, "sequences": args.sequences, "frames": args.frames, "crop": list(args.crop),
-                      "loss_first": hist[0][1], "loss_last": hist[-1][1], "validation": res}), flush=True)
+    res = None
+    if not args.no_eval:
+        sets = {"trained": blob, "builtin": default_weights(), "random_init": to_f16_blob(random_weights(args.seed))}
+        res = evaluate(sets, size=tuple(args.eval_size), frames=args.eval_frames)
+    print(json.dumps({"steps_done": hist[-1][0] + 1 if hist else 0, "steps": args.steps, "sequences": args.sequences,
+                      "frames": args.frames, "crop": list(args.crop), "scene_size": list(args.scene_size),
+                      "forward": args.forward, "loss_first": hist[0][1], "loss_last": hist[-1][1],
+                      "validation": res}), flush=True)
 
 
 if __name__ == "__main__":

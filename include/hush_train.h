@@ -65,8 +65,19 @@ int32_t hush_train_backward(hush_train_t *t, const void *weights, const void *gr
    counter advances and the reset flag clears */
 int32_t hush_train_commit(hush_train_t *t, const void *out, void *stream);
 
+/* MLP arithmetic of later forwards (enum hush_precision): HUSH_MLP_FP32 (default), HUSH_MLP_FP16_MFMA or
+   HUSH_MLP_BF16_MFMA (FP32 contexts only): the MFMA forms of hush.h run the MLP of this frame (and of the
+   previous frame, for the temporal blend) on weights packed from the FP32 weights at every forward; the
+   rest of the forward and the whole backward stay FP32, the MLP's weight gradient recomputing its hidden
+   layers in FP32 (mixed-precision training: low-precision forward, FP32 master weights and gradients) */
+int32_t hush_train_set_forward_precision(hush_train_t *t, int32_t precision);
+
 /* A0 in the element type (W x H x 4): after a forward, the re-blended temporal output */
 int32_t hush_train_a0(hush_train_t *t, void **ptr);
+
+/* the MLP's 12 outputs before the activations (W/2 x H/2 x 12, element type, row-major; zero for sky
+   blocks): after a forward, those of this frame */
+int32_t hush_train_mlp_out(hush_train_t *t, void **ptr);
 
 /* fixed = 1: later forwards take A0 as it is in that buffer (writable, for input gradient checks) instead
    of re-blending it, and the backward stops at A0; 0 (default): re-blend */

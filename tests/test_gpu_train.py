@@ -8,6 +8,10 @@
   have kinks that a finite step can straddle).
 * The FP32 backward agrees with the FP64 one to 1e-3 (relative to the largest gradient).
 * Adam steps on seeded sequences lower the loss.
+* The MFMA forward option (binary16 / bfloat16 MLP on the matrix cores, FP32 backward): the MLP outputs
+  equal the MLP stage's MFMA kernels on the same features, the frame's output stays within the precision's
+  error of the FP32 forward, and the backward of its state agrees with the FP32 backward (cosine >= 0.99
+  on the pixels whose outputs agree to 1%); Adam with it lowers the loss too.
 """
 import numpy as np
 import pytest
@@ -165,3 +169,83 @@ def test_reblend_reproduces_the_temporal_output():
     assert np.array_equal(Q(True).st(a0), ref), float(np.mean(Q(True).st(a0) != ref))
     fr.destroy()
     hs.destroy()
+
+
+@pytest.mark.parametrize("forward,out_tol", [("fp16-mfma", 2e-2), ("bf16-mfma", 1e-1)])
+def test_mfma_forward_matches_fp32_forward(forward, out_tol):
+    """one frame after three frames of history, prepared identically in two contexts, one with the FP32
+    forward and one with the MFMA forward: output error p99 within out_tol relative to max(|ref|, 1e-2) and
+    the loss within 2%.  The backward of the MFMA forward's state, fed the FP32 forward's loss gradient on the
+    pixels whose outputs agree to 1%, matches the FP32 backward (cosine >= 0.99, norms within 10%).  The
+    few pixels left out sit near TAA's inverse-tone-map singularity (1 / max(1 - lum, 1e-4)), where a
+    1e-3 change of the MLP outputs moves the output by tens of percent and flips the L1 loss's signs: with
+    the bf16 forward, 2 of 9216 pixels of this frame, enough to turn the raw weight gradient around.  The
+    fp16 forward keeps them: its raw gradient (own loss gradient, every pixel) agrees too."""
+    import torch
+    from hush_amd import PRECISIONS
+    ctx = []
+    for f in ("fp32", forward):
+        hs, fr, w, tgt, prev = _setup(False, kind="boxes", size=96, seed=9)
+        assert fr._L.hush_train_set_forward_precision(fr._t, PRECISIONS[f]) == 0
+        out = fr.forward(w)
+        l, g = fr.loss(out, tgt, prev[0], prev[1], 0.5)
+        ctx.append((hs, fr, w, out, float(l), g))
+    (hs1, fr1, w, o1, l1, g1), (hs2, fr2, _, o2, l2, g2) = ctx
+    rel = (o2 - o1).abs() / torch.clamp(o1.abs(), min=1e-2)
+    p99 = float(torch.quantile(rel.double().flatten(), 0.99))
+    stable = ((o2 - o1).abs().amax(-1) / (1 + o1.abs().amax(-1)) <= 1e-2)[..., None].to(g1.dtype)
+    gm = (g1 * stable).contiguous()
+    a = fr1.backward(w, gm).double().cpu().numpy()
+    b = fr2.backward(w, gm).double().cpu().numpy()
+    raw1 = fr1.backward(w, g1).double().cpu().numpy()
+    raw2 = fr2.backward(w, g2).double().cpu().numpy()
+    for hs, fr, *_ in ctx:
+        fr.destroy()
+        hs.destroy()
+    cos = lambda x, y: float(np.dot(x, y) / (np.linalg.norm(x) * np.linalg.norm(y)))
+    c, ratio, craw = cos(a, b), float(np.linalg.norm(b) / np.linalg.norm(a)), cos(raw1, raw2)
+    print(forward, "output rel err p99", p99, "loss", l1, l2, "pixels outside 1%:", int((stable == 0).sum()),
+          "| backward cosine", c, "norm ratio", ratio, "| raw gradient cosine", craw)
+    assert p99 <= out_tol
+    assert abs(l2 - l1) <= 2e-2 * abs(l1)
+    assert c >= 0.99 and abs(ratio - 1.0) <= 0.1
+    if forward == "fp16-mfma":
+        assert craw >= 0.99
+
+
+def test_mfma_forward_outputs_equal_the_mlp_stage():
+    """the MFMA forward's MLP outputs (before the activations) are the MLP stage's (hush_mlp_forward) on the
+    same feature rows, bit for bit: the same MFMA tile code over the same packed weights"""
+    import torch
+    import hush_amd
+    from hush_amd import PRECISIONS
+    from hush_amd.hip import DeviceBuffer, lib, check, hipMemcpyHostToDevice, hipMemcpyDeviceToHost
+    hs, fr, w, tgt, prev = _setup(False, kind="boxes", size=96, seed=9)
+    assert fr._L.hush_train_set_forward_precision(fr._t, PRECISIONS["fp16-mfma"]) == 0
+    fr.forward(w)
+    torch.cuda.synchronize()
+    o = fr.mlp_outputs()
+    F = np.concatenate([hs.surface(f"F{k}").download().view(np.float16) for k in range(3)], -1)
+    sky = hs.surface("F3").download().view(np.float16)[..., 2].astype(np.float32) < 0
+    rows = np.ascontiguousarray(F.reshape(-1, 24)[:, :20])
+    n = rows.shape[0]
+    fin, fout = DeviceBuffer(rows.nbytes), DeviceBuffer(n * 48)
+    check(lib().hipMemcpy(fin.ptr, rows.ctypes.data, rows.nbytes, hipMemcpyHostToDevice), "h2d")
+    hs.mlp_forward(fin, fout, n, "fp16-mfma")
+    ref = np.empty((n, 12), np.float32)
+    check(lib().hipMemcpy(ref.ctypes.data, fout.ptr, ref.nbytes, hipMemcpyDeviceToHost), "d2h")
+    fr.destroy()
+    hs.destroy()
+    live = ~sky.reshape(-1)
+    assert live.sum() > 0
+    assert np.array_equal(o.reshape(-1, 12)[live], ref[live])
+    assert np.all(o.reshape(-1, 12)[~live] == 0)
+
+
+def test_adam_lowers_the_loss_with_mfma_forward():
+    from hush_amd import train as T
+    w, hist = T.train(steps=12, sequences=2, frames=4, crop=(64, 64), scene_size=(160, 96), lr=3e-3, seed=1, log=None,
+                      forward="fp16-mfma")
+    losses = [l for _, l in hist]
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-3:]) < np.mean(losses[:3]), losses
