@@ -311,6 +311,46 @@ def cpu_baseline(frames_g, w, h, n_frames):
             "sample": f"{n_frames} frames of the same 1920x1080 boxes sequence, FP32 mirror oracle (C, OpenMP)"}
 
 
+def c0_leg(hip, stream, reps=16):
+    """The reference's CPU-runnable case: 640x360 spheres (16 spheres on a ground plane, orbiting camera),
+    an 8-frame sequence, FP32, built-in weights.  GPU ms/frame over `reps` passes of the sequence (reset at
+    each start), and the FP32-mirror oracle's ms/frame on the host cores over the same 8 frames."""
+    import hush_amd
+    from hush_amd.scene import SceneGenerator
+    from oracle.ho import Oracle
+    W, H, F = 640, 360, 8
+    gen = SceneGenerator("spheres", W, H, seed=0)
+    gs = [gen.render(f) for f in range(F)]
+    dev = [{k: hip.Surface(W, H).upload(g[k].view(np.float16)) for k in INPUT_KEYS} for g in gs]
+    hs = hush_amd.Hush(0, {"width": W, "height": H})
+    for d in dev:
+        hs.encode(d, stream=stream.handle)
+    e0, e1 = hip.Event(), hip.Event()
+    stream.synchronize()
+    e0.record(stream.handle)
+    for r in range(reps):
+        hs.reset()
+        for d in dev:
+            hs.encode(d, stream=stream.handle)
+    e1.record(stream.handle)
+    e1.synchronize()
+    gpu_ms = e0.elapsed_ms(e1) / (reps * F)
+    hs.destroy()
+    o = Oracle(W, H, hush_amd.default_weights())
+    o.denoise(gs[0])
+    o.reset()
+    o.set_state(0, True)
+    t = time.perf_counter()
+    for g in gs:
+        o.denoise(g)
+    cpu_ms = (time.perf_counter() - t) / F * 1e3
+    for d in dev:
+        for v in d.values():
+            v._buf.free()
+    return {"gpu_ms_per_frame": round(gpu_ms, 5), "cpu_oracle_ms_per_frame": round(cpu_ms, 3),
+            "cpu_cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "frames": F, "gpu_passes": reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,7 +447,23 @@ def main():
                 "traffic_source": traffic_src, "algorithmic_bytes": ab["temporal"],
                 "algorithmic_bytes_rule": "80*N + 72*N2 (N = W*H, N2 = ceil(W/2)*ceil(H/2))"}
 
-    extras = {}
+    # per-frame durations: an event after every frame of a separate 64-frame loop, median and spread
+    n_m = min(args.steps, 64)
+    evs = [hip.Event() for _ in range(n_m + 1)]
+    stream.synchronize()
+    evs[0].record(stream.handle)
+    for i in range(n_m):
+        f = i % F
+        if f == 0:
+            hs.reset()
+        hs.encode(dev[f], stream=stream.handle)
+        evs[i + 1].record(stream.handle)
+    evs[-1].synchronize()
+    per_frame = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(n_m)])
+    frame_stats = {"median_ms": round(float(np.median(per_frame)), 5), "p10_ms": round(float(np.quantile(per_frame, 0.1)), 5),
+                   "p90_ms": round(float(np.quantile(per_frame, 0.9)), 5), "frames": n_m}
+
+    extras = {f"frame_time_{W}x{H}": frame_stats}
     if not args.no_extras:
         extras["mlp_stage_1080p"] = mlp_stage(hs, hip, W, H)
         print(f"[bench] MLP stage: {extras['mlp_stage_1080p']}", file=sys.stderr, flush=True)
@@ -429,6 +485,8 @@ def main():
         extras["train_fp32"] = train_leg(dev, W, H)
         extras["scene_generator_1080p"] = scene_leg(W, H)
         print(f"[bench] train {extras['train_fp32']}", file=sys.stderr, flush=True)
+        extras["c0_640x360_spheres"] = c0_leg(hip, stream)
+        print(f"[bench] 640x360 {extras['c0_640x360_spheres']}", file=sys.stderr, flush=True)
         extras["uhd_2160p"] = uhd_leg(hip, stream)
         print(f"[bench] 2160p {extras['uhd_2160p']}", file=sys.stderr, flush=True)
         extras["error_vs_fp64_oracle_640x360"] = psnr_check(["fp32", "fp16-mfma", "bf16-mfma", "fp8-mfma", "fp32+compensated",
