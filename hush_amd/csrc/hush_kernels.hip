@@ -285,9 +285,10 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   }
   __syncthreads();
 
-  // block phase (hush.js:226-282, 301-306): wave 0 computes the 16x4 half-res texels of this tile
-  if (tid >= 64) return;
-  const int bxl = tid & 15, byl = tid >> 4;
+  // block phase (hush.js:226-282, 301-306): one wave computes the 16x4 half-res texels of this tile; which
+  // wave rotates with the workgroup index, so that the phase's extra work spreads over the CU's SIMDs
+  if ((tid >> 6) != (int)(blockIdx.x & 3)) return;
+  const int bxl = tid & 15, byl = (tid & 63) >> 4;
   const int X = (x0 >> 1) + bxl, Y = (y0 >> 1) + byl;
   if (X >= a.W2 || Y >= a.H2) return;
   float nv = 0.0f;
