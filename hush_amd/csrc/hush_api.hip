@@ -131,11 +131,14 @@ extern "C" int32_t hush_resize(hush_t *h, int32_t width, int32_t height) {
   if (width == h->W && height == h->H) return HUSH_OK;
   hipError_t e = hipSetDevice(h->device);
   if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
-  (void)hipDeviceSynchronize();
-  free_surfaces(h);
-  h->W = h->H = 0;
+  // the new surfaces first: if that fails the context keeps its size, surfaces and history
+  void *old = h->arena;
   int r = alloc_surfaces(h, width, height);
   if (r != HUSH_OK) return r;
+  if (old) {
+    (void)hipDeviceSynchronize();      // work still in flight on the old surfaces
+    (void)hipFree(old);
+  }
   h->frame = 0;
   h->reset = true;
   return HUSH_OK;
@@ -181,12 +184,12 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
       return HUSH_ERR_DEVICE;
     }
   }
-  for (int i = 0; i < 9; i++) (void)hipEventCreate(&h->ev[i]);
-  e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);    // default priority
+  for (int i = 0; i < 9 && e == hipSuccess; i++) e = hipEventCreate(&h->ev[i]);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);    // default priority
   if (e == hipSuccess) e = hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
-    hip_fail(nullptr, e, "side stream");
+    hip_fail(nullptr, e, "timing events and side stream");
     hush_destroy(h);
     return HUSH_ERR_DEVICE;
   }
