@@ -108,13 +108,13 @@ __device__ __forceinline__ int2 tile_origin(int w, int h) {
 template <bool FAST> struct Math;
 template <> struct Math<false> {
   static __device__ __forceinline__ float div(float a, float b) { return a / b; }
-  // a / b for a denominator shared by several quotients, y = rcp(b) = RN(1/b): two FMA corrections
-  // (Markstein) give the correctly rounded quotient, bit-identical to a / b for finite a and finite non-zero b
-  // (5 VALU instructions instead of 11)
+  // a / b for a denominator shared by several quotients, y = RN(1/b): q0 = RN(a y) is within an ulp of a / b,
+  // the residual a - b q0 is exact in an FMA, and one correction RN(q0 + r y) is the correctly rounded
+  // quotient (Markstein's theorem for a correctly rounded reciprocal), bit-identical to a / b for finite a
+  // and finite non-zero b with a normal quotient (3 VALU instructions instead of 11)
   static __device__ __forceinline__ float divs(float a, float b, float y) {
     const float q0 = a * y;
-    const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
-    return __builtin_fmaf(__builtin_fmaf(-b, q1, a), y, q1);
+    return __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
   }
   static __device__ __forceinline__ float rcp(float b) { return 1.0f / b; }
   static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
