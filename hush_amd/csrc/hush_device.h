@@ -110,11 +110,12 @@ template <> struct Math<false> {
   static __device__ __forceinline__ float div(float a, float b) { return a / b; }
   // a / b for a denominator shared by several quotients, y = RN(1/b): q0 = RN(a y) is within an ulp of a / b,
   // the residual a - b q0 is exact in an FMA, and one correction RN(q0 + r y) is the correctly rounded
-  // quotient (Markstein's theorem for a correctly rounded reciprocal), bit-identical to a / b for finite a
-  // and finite non-zero b with a normal quotient (3 VALU instructions instead of 11)
+  // quotient (Markstein's theorem for a correctly rounded reciprocal).  For a = +-0 the correction adds two
+  // zeros and can lose the quotient's sign, which q0 always carries: hence the copysign.  Bit-identical to
+  // a / b for finite a and finite non-zero b with a normal or zero quotient (4 VALU instructions, not 11)
   static __device__ __forceinline__ float divs(float a, float b, float y) {
     const float q0 = a * y;
-    return __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
+    return __builtin_copysignf(__builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0), q0);
   }
   static __device__ __forceinline__ float rcp(float b) { return 1.0f / b; }
   static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }

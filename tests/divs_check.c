@@ -1,9 +1,9 @@
 /* The shared-reciprocal division of the kernels (Math<false>::divs in hush_amd/csrc/hush_device.h): with
    y = RN(1/b), q0 = RN(a y) and one FMA correction q = RN(q0 + RN(a - b q0) y) (the residual a - b q0 is
-   exact in an FMA), the result must be the correctly rounded a / b (Markstein's one-step correction).
-   Checks, against IEEE division:
+   exact in an FMA), the result must be the correctly rounded a / b (Markstein's one-step correction); the
+   sign of a zero quotient is restored from q0.  Checks, against IEEE division (bit for bit, so -0 != +0):
      * random operands over wide exponent ranges (argv[1] pairs);
-     * every quotient of small integers;
+     * every quotient of small integers, and +-0 over divisors of both signs;
      * every numerator significand (2^23) at three exponents for divisors with the hard significands (all
        ones, all ones minus 1 and 2, one ulp above 1, exactly 1, 1.5, 1.5 minus one ulp) at three exponents,
        and for argv[2] random divisors (one numerator exponent each).
@@ -16,7 +16,7 @@
 
 static float divs(float a, float b, float y) {
   float q0 = a * y;
-  return fmaf(fmaf(-b, q0, a), y, q0);
+  return copysignf(fmaf(fmaf(-b, q0, a), y, q0), q0);
 }
 static uint64_t st = 88172645463325252ull;
 static uint64_t rnd(void) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return st; }
@@ -46,6 +46,10 @@ int main(int argc, char **argv) {
       float a = (float)i, b = (float)j, q = divs(a, b, 1.0f / b), r = a / b;
       bad += memcmp(&q, &r, 4) != 0;
     }
+  for (int i = 0; i < 100000; i++) {
+    const float z = (i & 1) ? -0.0f : 0.0f, b = rf(-40, 40), q = divs(z, b, 1.0f / b), r = z / b;
+    bad += memcmp(&q, &r, 4) != 0;
+  }
   const uint32_t hard[] = {0x7fffff, 0x7ffffe, 0x7ffffd, 0x000001, 0x000000, 0x400000, 0x3fffff};
   for (int s = 0; s < 7; s++)
     for (int be = -3; be <= 3; be += 3)
