@@ -79,3 +79,48 @@ def test_node_svgf_sequence_matches_oracle(tmp_path):
         assert st["frac_rel_gt_1e-3"] <= 1e-3 and st["max_rel"] <= 1e-2, (f, st)
     timing = json.loads((tmp_path / "timing.json").read_text())
     assert [p["label"] for p in timing] == ["temporal", "variance"] + ["atrous"] * 5 + ["resolve"]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_options_match_oracle_and_python(tmp_path):
+    """the JavaScript class's options through the N-API addon: an exposure change and reset() mid-sequence and
+    weights as a base64 string (hidden width 8) against the oracle (tolerance of the end-to-end parity test);
+    the FP16-MFMA, shader-f16 and compensated forms bit-identical to the same library driven from Python
+    (the f16 form's own oracle agreement, within one binary16 ulp per MLP output, is checked at the MLP)"""
+    import base64
+    import hush_amd
+    from hush_amd.scene import SceneGenerator
+    from hush_amd.train import random_weights, to_f16_blob
+    from oracle.ho import Oracle
+    from tests.parity import compare, upload_frame
+    w, h, frames = 128, 72, 5
+    gen = SceneGenerator("boxes", w, h, seed=17)
+    gs = [gen.render(f) for f in range(frames)]
+    for f, g in enumerate(gs):
+        for k in ("radiance", "normalDepth", "motion", "material", "emission"):
+            g[k].tofile(tmp_path / f"f{f}_{k}.bin")
+    w8 = to_f16_blob(random_weights(5, 8))
+    (tmp_path / "weights.b64").write_text(base64.b64encode(w8.tobytes()).decode())
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "options_sequence.js"), str(tmp_path), str(w), str(h),
+                        str(frames)], capture_output=True, text=True, timeout=180)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "options_sequence: ok" in r.stdout, r.stdout + r.stderr
+    out = lambda c, f: np.fromfile(tmp_path / f"out{c}{f}.bin", np.float16).reshape(h, w, 4)
+    refs = {"A": Oracle(w, h, hush_amd.default_weights()), "B": Oracle(w, h, w8)}
+    for c, o in refs.items():
+        for f, g in enumerate(gs):
+            if c == "A" and f == 2:
+                o.set_exposure(2.5)
+            if c == "A" and f == 3:
+                o.reset()
+            st = compare(out(c, f).astype(np.float32)[..., :3], o.denoise(g)[..., :3])
+            assert st["frac_rel_gt_1e-3"] <= 1e-3 and st["max_rel"] <= 1e-2, (c, f, st)
+    for c, opts in (("C", {"precision": "fp16-mfma"}), ("D", {"f16": True}), ("E", {"compensated": True})):
+        hs = hush_amd.Hush(0, dict({"width": w, "height": h}, **opts))
+        surfs = None
+        for f, g in enumerate(gs):
+            surfs = upload_frame(g, surfs)
+            py = hs.denoise(surfs).download()
+            assert np.array_equal(py.view(np.uint16), out(c, f).view(np.uint16)), (c, f)
+        hs.destroy()
