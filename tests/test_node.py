@@ -86,8 +86,11 @@ def test_node_svgf_sequence_matches_oracle(tmp_path):
 def test_node_options_match_oracle_and_python(tmp_path):
     """the JavaScript class's options through the N-API addon: an exposure change and reset() mid-sequence and
     weights as a base64 string (hidden width 8) against the oracle (tolerance of the end-to-end parity test);
-    the FP16-MFMA, shader-f16 and compensated forms bit-identical to the same library driven from Python
-    (the f16 form's own oracle agreement, within one binary16 ulp per MLP output, is checked at the MLP)"""
+    the FP16-MFMA, shader-f16 and compensated forms bit-identical to the same library driven from Python.
+    (The f16 MLP equals the oracle's bit for bit, tests/test_gpu_mlp.py; end to end on this scene one pixel
+    of one frame moves 7% from single-ulp expf / logf differences between the GPU's math library and the
+    host's, compounding through the history, which is beyond the end-to-end tolerance, so the option's
+    plumbing is checked against the Python path.)"""
     import base64
     import hush_amd
     from hush_amd.scene import SceneGenerator
