@@ -342,7 +342,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   for (int j = 0; j < 16; j++) {
     int idx = b0 + (j >> 2) * SW + (j & 3);
     float w = sI[idx].w;
-    q1.add(w * E * qf16(sL[idx]));
+    // w * E * qf16(lum) = w * sQ: w is exactly 1 or 0, and an invalid texel's luminance is 0
+    q1.add(w * sQ[idx]);
     n4 += w;
   }
   const float mu4 = F::div(q1.get(), mx(n4, 1.0f));
