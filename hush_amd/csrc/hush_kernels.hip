@@ -96,6 +96,62 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     pqr = a.ppar0.p[hy * a.ppar0.pitch + hx];
   }
 
+  // reprojection (hush.js:144-187) ahead of the staging: it needs no LDS, and consuming the gathered taps
+  // here keeps only its results live across the staging barrier and the 3x3 statistics
+  const float3 n = make_float3(nd.x, nd.y, nd.z);
+  uint32_t mbits = 0;
+    // reprojection (hush.js:144-187)
+    bool has = false;
+    float hcx = 0.0f, hcy = 0.0f, hcz = 0.0f;
+    float4 hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float v = 0.0f, alpha = 1.0f, gamma = 1.0f;
+    if (use_hist) {
+      const int W = a.W, H = a.H;
+      const float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+      float4 c4[4], m4[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) { c4[i] = unpack4(c4r[i]); m4[i] = unpack4(m4r[i]); }
+      const float2 pq = unpack2(pqr.x);
+      float sumW = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if (wts[i] > 0.0f && inside(addw(bix, i & 1), addw(biy, i >> 1), W, H) && check(k4[i], n, mv.w, mv.z)) {
+          mbits |= 1u << i;
+          sumW += wts[i];
+          hcx += wts[i] * c4[i].x; hcy += wts[i] * c4[i].y; hcz += wts[i] * c4[i].z;
+          hm.x += wts[i] * m4[i].x; hm.y += wts[i] * m4[i].y; hm.z += wts[i] * m4[i].z; hm.w += wts[i] * m4[i].w;
+        }
+      }
+      if (sumW >= 0.01f) {
+        const float rs = F::rcpn(sumW);
+        hcx = F::divs(hcx, sumW, rs); hcy = F::divs(hcy, sumW, rs); hcz = F::divs(hcz, sumW, rs);
+        hm.x = F::divs(hm.x, sumW, rs); hm.y = F::divs(hm.y, sumW, rs); hm.z = F::divs(hm.z, sumW, rs); hm.w = F::divs(hm.w, sumW, rs);
+        v = sumW; has = true;
+      } else {
+        float n3 = 0.0f;
+        hcx = hcy = hcz = 0.0f;
+        hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int j = 0; j < 9; j++) {
+          int qx = addw(cx, j % 3 - 1), qy = addw(cy, j / 3 - 1);
+          if (inside(qx, qy, W, H) && check(ldf(a.pkd, qx, qy), n, mv.w, mv.z)) {
+            n3 += 1.0f;
+            float4 cc = ld(a.pcol, qx, qy), mm = ld(a.pmom, qx, qy);
+            hcx += cc.x; hcy += cc.y; hcz += cc.z;
+            hm.x += mm.x; hm.y += mm.y; hm.z += mm.z; hm.w += mm.w;
+          }
+        }
+        if (n3 > 0.0f) {
+          const float rn = F::rcpn(n3);
+          hcx = F::divs(hcx, n3, rn); hcy = F::divs(hcy, n3, rn); hcz = F::divs(hcz, n3, rn);
+          hm.x = F::divs(hm.x, n3, rn); hm.y = F::divs(hm.y, n3, rn); hm.z = F::divs(hm.z, n3, rn); hm.w = F::divs(hm.w, n3, rn);
+          v = 0.1f; has = true;
+          mbits |= 0x10;
+        }
+      }
+      alpha = pq.x; gamma = pq.y;
+    }
+    has = has && hm.w > 0.0f;
+    if (has) mbits |= 0x20;
   // tile staging (hush.js:287-294): each thread its own pixel, then 84 threads the 1-texel apron
   {
     float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -130,7 +186,6 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   float *P = &sP[0][tid];
   bool p_written = false;
   if (in_img) {
-    uint32_t mbits = 0;
     if (nd.w < 0.0f) {                                   // sky (hush.js:115-119)
       st(a.outA, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
       st(a.outMom, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
@@ -141,7 +196,6 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       mbits = 0x80;
       if (AUX) a.aux[((size_t)py * a.W + px) * TEMPORAL_AUX] = -1.0f;
     } else {
-      const float3 n = make_float3(nd.x, nd.y, nd.z);
       const int c0 = (ly + 1) * SW + lx + 1;
       const float4 ce = sI[c0];
       const float l = sL[c0];
@@ -174,58 +228,6 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       const float sdx = F::sqrtn(mx(F::divs(svx, cnt, rcnt), 1e-12f)), sdy = F::sqrtn(mx(F::divs(svy, cnt, rcnt), 1e-12f)),
                   sdz = F::sqrtn(mx(F::divs(svz, cnt, rcnt), 1e-12f));
       svar = F::divs(svar, cnt, rcnt);
-      // reprojection (hush.js:144-187)
-      bool has = false;
-      float hcx = 0.0f, hcy = 0.0f, hcz = 0.0f;
-      float4 hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      float v = 0.0f, alpha = 1.0f, gamma = 1.0f;
-      if (use_hist) {
-        const int W = a.W, H = a.H;
-        const float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
-        float4 c4[4], m4[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) { c4[i] = unpack4(c4r[i]); m4[i] = unpack4(m4r[i]); }
-        const float2 pq = unpack2(pqr.x);
-        float sumW = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          if (wts[i] > 0.0f && inside(addw(bix, i & 1), addw(biy, i >> 1), W, H) && check(k4[i], n, mv.w, mv.z)) {
-            mbits |= 1u << i;
-            sumW += wts[i];
-            hcx += wts[i] * c4[i].x; hcy += wts[i] * c4[i].y; hcz += wts[i] * c4[i].z;
-            hm.x += wts[i] * m4[i].x; hm.y += wts[i] * m4[i].y; hm.z += wts[i] * m4[i].z; hm.w += wts[i] * m4[i].w;
-          }
-        }
-        if (sumW >= 0.01f) {
-          const float rs = F::rcpn(sumW);
-          hcx = F::divs(hcx, sumW, rs); hcy = F::divs(hcy, sumW, rs); hcz = F::divs(hcz, sumW, rs);
-          hm.x = F::divs(hm.x, sumW, rs); hm.y = F::divs(hm.y, sumW, rs); hm.z = F::divs(hm.z, sumW, rs); hm.w = F::divs(hm.w, sumW, rs);
-          v = sumW; has = true;
-        } else {
-          float n3 = 0.0f;
-          hcx = hcy = hcz = 0.0f;
-          hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-          for (int j = 0; j < 9; j++) {
-            int qx = addw(cx, j % 3 - 1), qy = addw(cy, j / 3 - 1);
-            if (inside(qx, qy, W, H) && check(ldf(a.pkd, qx, qy), n, mv.w, mv.z)) {
-              n3 += 1.0f;
-              float4 cc = ld(a.pcol, qx, qy), mm = ld(a.pmom, qx, qy);
-              hcx += cc.x; hcy += cc.y; hcz += cc.z;
-              hm.x += mm.x; hm.y += mm.y; hm.z += mm.z; hm.w += mm.w;
-            }
-          }
-          if (n3 > 0.0f) {
-            const float rn = F::rcpn(n3);
-            hcx = F::divs(hcx, n3, rn); hcy = F::divs(hcy, n3, rn); hcz = F::divs(hcz, n3, rn);
-            hm.x = F::divs(hm.x, n3, rn); hm.y = F::divs(hm.y, n3, rn); hm.z = F::divs(hm.z, n3, rn); hm.w = F::divs(hm.w, n3, rn);
-            v = 0.1f; has = true;
-            mbits |= 0x10;
-          }
-        }
-        alpha = pq.x; gamma = pq.y;
-      }
-      has = has && hm.w > 0.0f;
-      if (has) mbits |= 0x20;
       // temporal blend (hush.js:189-211)
       float cr = ce.x, cg = ce.y, cb = ce.z;
       float nmx = l, nmy = l * l, nfast = l, len1 = 1.0f, lag = 0.0f;
