@@ -29,8 +29,10 @@ __device__ __forceinline__ bool check(float4 kd, float3 n, float zprev, float ob
   return dot3(make_float3(kd.x, kd.y, kd.z), n) >= 0.9f;
 }
 
-// per-pixel feature inputs kept in LDS for the block phase (structure-of-arrays, conflict-free)
-enum { P0X, P0Y, P0Z, P0W, P1X, P1Y, P1Z, P1W, P2Z, P2W, NSX, NSY, NSZ, P3X, P3Y, P3Z, P3W, P4X, P4Y, P4Z, P4W, NP };
+// per-pixel feature inputs kept in LDS for the block phase (structure-of-arrays, conflict-free).  P0.y
+// (E qf16(lum)) is the staged sQ of the pixel and is read from there; the material P3 is kept as its two
+// packed binary16 words (its exact input bits): 18 rows instead of 21, fewer LDS writes and reads.
+enum { P0X, P0Z, P0W, P1X, P1Y, P1Z, P1W, P2Z, P2W, NSX, NSY, NSZ, P3A, P3B, P4X, P4Y, P4Z, P4W, NP };
 
 // Temporal statistics (moments, variances, cancellation-prone differences such as m2 - m1^2) are
 // ill-conditioned, so this pass is evaluated with IEEE arithmetic under both math policies.
@@ -69,7 +71,12 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
 
   // this pixel's G-buffer (also its own staged texel)
   float4 nd = make_float4(0.0f, 0.0f, 0.0f, -1.0f), mv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mr = mv, rd = mv;
-  if (in_img) { nd = ld(a.nd, px, py); mv = ld(a.mv, px, py); mr = ld(a.mat, px, py); rd = ld(a.rad, px, py); }
+  uint2 mraw = make_uint2(0u, 0u);
+  if (in_img) {
+    nd = ld(a.nd, px, py); mv = ld(a.mv, px, py); rd = ld(a.rad, px, py);
+    mraw = a.mat.p[(size_t)py * a.mat.pitch + px];
+    mr = unpack4(mraw);
+  }
   const bool use_hist = in_img && !a.reset && nd.w >= 0.0f;
 
   // reprojection fetches (hush.js:144-170): the four bilinear taps of the decoded key, colour and moments
@@ -269,13 +276,13 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       float aqx = qf16(A.x), aqy = qf16(A.y), aqz = qf16(A.z), aqw = qf16(A.w);
       float m1 = E * qf16(nmx);
       float m2 = E * E * qf16(nmy);
-      P[P0X * NT] = E * lum(aqx, aqy, aqz); P[P0Y * NT] = E * qf16(l); P[P0Z * NT] = mx(m2 - m1 * m1, 0.0f); P[P0W * NT] = len1;
+      P[P0X * NT] = E * lum(aqx, aqy, aqz); P[P0Z * NT] = mx(m2 - m1 * m1, 0.0f); P[P0W * NT] = len1;
       P[P1X * NT] = qf16(v); P[P1Y * NT] = qf16(lag); P[P1Z * NT] = E * qf16(nfast); P[P1W * NT] = m1;
       float ocx = qf16(oc.x), ocy = qf16(oc.y);
       P[P2Z * NT] = qf16(nd.w); P[P2W * NT] = qf16(F::sqrtn(mv.x * mv.x + mv.y * mv.y));
       float3 ns = (ocx == ok.x && ocy == ok.y) ? dec : oct_dec_m<FAST>(ocx, ocy);
       P[NSX * NT] = ns.x; P[NSY * NT] = ns.y; P[NSZ * NT] = ns.z;
-      P[P3X * NT] = mr.x; P[P3Y * NT] = mr.y; P[P3Z * NT] = mr.z; P[P3W * NT] = mr.w;
+      P[P3A * NT] = __uint_as_float(mraw.x); P[P3B * NT] = __uint_as_float(mraw.y);
       P[P4X * NT] = aqx; P[P4Y * NT] = aqy; P[P4Z * NT] = aqz; P[P4W * NT] = aqw;
       p_written = true;
     }
@@ -306,9 +313,14 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     nv += 1.0f;
     la[j] = sP[P0X][t]; zs[j] = sP[P2Z][t];
     nsx[j] = sP[NSX][t]; nsy[j] = sP[NSY][t]; nsz[j] = sP[NSZ][t];
+    {
+      const int tl = (2 * byl + (j >> 1) + 1) * SW + 2 * bxl + (j & 1) + 1;    // the pixel's staged tile index
+      const float2 m01 = unpack2(__float_as_uint(sP[P3A][t])), m23 = unpack2(__float_as_uint(sP[P3B][t]));
+      const float p0[4] = {sP[P0X][t], sQ[tl], sP[P0Z][t], sP[P0W][t]}, p3[4] = {m01.x, m01.y, m23.x, m23.y};
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      qS0[c].add(sP[P0X + c][t]); qS1[c].add(sP[P1X + c][t]); qS3[c].add(sP[P3X + c][t]); qAC[c].add(sP[P4X + c][t]);
+      for (int c = 0; c < 4; c++) {
+        qS0[c].add(p0[c]); qS1[c].add(sP[P1X + c][t]); qS3[c].add(p3[c]); qAC[c].add(sP[P4X + c][t]);
+      }
     }
     qNx.add(nsx[j]); qNy.add(nsy[j]); qNz.add(nsz[j]);
     qMV.add(sP[P2W][t]); qZ.add(sP[P2Z][t]);
