@@ -127,7 +127,8 @@ static int alloc_surfaces(hush_t *h, int W, int H) {
 
 extern "C" int32_t hush_resize(hush_t *h, int32_t width, int32_t height) {
   if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_resize: null context");
-  if (!(width > 0 && height > 0)) return fail(h, HUSH_ERR_SIZE, "Hush: bad size %dx%d", width, height);
+  if (!(width > 0 && height > 0 && width <= HUSH_MAX_SIDE && height <= HUSH_MAX_SIDE))
+    return fail(h, HUSH_ERR_SIZE, "Hush: bad size %dx%d", width, height);
   if (width == h->W && height == h->H) return HUSH_OK;
   hipError_t e = hipSetDevice(h->device);
   if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
@@ -151,7 +152,8 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
   if (!weights) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_create: null weights");
   int hid = hush_hidden_of(count);
   if (!hid) return fail(nullptr, HUSH_ERR_WEIGHTS, "hush weights: %zu f16 values fit no MLP size", count);
-  if (!(width > 0 && height > 0)) return fail(nullptr, HUSH_ERR_SIZE, "Hush: bad size %dx%d", width, height);
+  if (!(width > 0 && height > 0 && width <= HUSH_MAX_SIDE && height <= HUSH_MAX_SIDE))
+    return fail(nullptr, HUSH_ERR_SIZE, "Hush: bad size %dx%d", width, height);
   if (precision < HUSH_MLP_FP32 || precision > HUSH_MLP_FP8_MFMA) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_create: unknown precision %d", precision);
   if (!precision_built(precision, hid))
     return fail(nullptr, HUSH_ERR_UNSUPPORTED, "hush_create: precision %d needs hidden width 16 (this blob: %d)", precision, hid);
@@ -352,7 +354,7 @@ hipError_t hush_run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream_t
   }
 }
 
-static int check_inputs(hush_t *h, const hush_inputs *in) {
+int hush_check_inputs(hush_t *h, const hush_inputs *in) {
   const hush_surface *s[5] = {&in->radiance, &in->normal_depth, &in->motion, &in->material, &in->emission};
   for (int k = 0; k < 5; k++) {
     if (!s[k]->ptr) return fail(h, HUSH_ERR_INPUT, "Hush.denoise: missing input '%s'", kInputName[k]);
@@ -364,6 +366,9 @@ static int check_inputs(hush_t *h, const hush_inputs *in) {
     if (s[k]->pitch % 8 != 0 || s[k]->pitch < (size_t)h->W * 8 || ((uintptr_t)s[k]->ptr & 7) != 0)
       return fail(h, HUSH_ERR_INPUT, "Hush.denoise: '%s' has pitch %zu bytes; rows need 8-byte alignment and >= %d bytes",
                   kInputName[k], s[k]->pitch, h->W * 8);
+    if (!hush_pitch_ok(s[k]->pitch / 8, h->H))
+      return fail(h, HUSH_ERR_INPUT, "Hush.denoise: '%s' has pitch %zu bytes; pitch x height must stay below 2^31 texels",
+                  kInputName[k], s[k]->pitch);
   }
   return HUSH_OK;
 }
@@ -405,7 +410,7 @@ extern "C" int32_t hush_encode(hush_t *h, const hush_inputs *in, void *stream, h
 extern "C" int32_t hush_denoise(hush_t *h, const hush_inputs *in, void *stream, hush_surface *out) {
   if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_denoise: null context");
   if (!in) return fail(h, HUSH_ERR_INPUT, "Hush.denoise: missing input '%s'", kInputName[0]);
-  int r = check_inputs(h, in);
+  int r = hush_check_inputs(h, in);
   if (r != HUSH_OK) return r;
   return hush_encode(h, in, stream, out);
 }
@@ -469,7 +474,7 @@ extern "C" int32_t hush_debug_set_state(hush_t *h, int64_t frame, int32_t reset)
 extern "C" int32_t hush_debug_run_pass(hush_t *h, int32_t pass, const hush_inputs *in, void *stream) {
   if (!h || !in) return fail(h, HUSH_ERR_ARGUMENT, "hush_debug_run_pass: null argument");
   if (pass < 0 || pass > 7) return fail(h, HUSH_ERR_ARGUMENT, "hush_debug_run_pass: unknown pass %d", pass);
-  int r = check_inputs(h, in);
+  int r = hush_check_inputs(h, in);
   if (r != HUSH_OK) return r;
   hipError_t e = hipSetDevice(h->device);
   if (e == hipSuccess) e = hush_run_pass(h, pass, in, (hipStream_t)stream);
@@ -495,7 +500,7 @@ extern "C" int32_t hush_debug_math_selftest(int32_t which, uint64_t *mismatches,
   for (uint32_t k = 0; k < 16 && e == hipSuccess; k++)
     e = launch_math_selftest(which, k << 28, 1u << 20, &dev->bad, &dev->first, nullptr);
   if (e == hipSuccess) e = hipMemcpy(&host, dev, sizeof(Counters), hipMemcpyDeviceToHost);
-  hipFree(dev);
+  (void)hipFree(dev);
   if (e != hipSuccess) return hip_fail(nullptr, e, "hush_debug_math_selftest");
   *mismatches = host.bad;
   *first = host.first;

@@ -50,3 +50,5 @@ __attribute__((visibility("hidden"))) int hush_fail(hush_t *h, int code, const c
 __attribute__((visibility("hidden"))) int hush_hip_fail(hush_t *h, hipError_t e, const char *what);
 // one pass of the frame at the context's current state (0 temporal .. 7 taa)
 __attribute__((visibility("hidden"))) hipError_t hush_run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream_t s);
+// the five G-buffer inputs against the context's size, format, alignment and pitch bounds (message on h)
+__attribute__((visibility("hidden"))) int hush_check_inputs(hush_t *h, const hush_inputs *in);

@@ -55,6 +55,11 @@ __device__ __forceinline__ float qf16(float x) {
 // pk (hush.js:101): pack2x16float(qf16(a), qf16(b))
 __device__ __forceinline__ uint32_t pk(float a, float b) { return pack_rtz(qf16(a), qf16(b)); }
 
+// Host-side bound on a caller's surface (pitch in texels): the kernels take the pitch as an int and index
+// some surfaces with 32-bit products.
+inline bool hush_pitch_ok(size_t pitch_texels, int height) {
+  return pitch_texels < ((size_t)1 << 23) && pitch_texels * (size_t)height < ((size_t)1 << 31);
+}
 __device__ __forceinline__ float4 ld(const Tex &t, int x, int y) { return unpack4(t.p[(size_t)y * t.pitch + x]); }
 __device__ __forceinline__ void st(const Tex &t, int x, int y, float4 v) { t.p[(size_t)y * t.pitch + x] = pack4(v); }
 __device__ __forceinline__ uint4 ldu(const TexU &t, int x, int y) { return t.p[(size_t)y * t.pitch + x]; }

@@ -996,6 +996,7 @@ extern "C" int32_t hush_train_prepare(hush_train_t *t, const hush_inputs *in, vo
   int r = check_size(t);
   if (r != HUSH_OK) return r;
   hush_t *h = t->h;
+  if (hush_check_inputs(h, in) != HUSH_OK) return tfail(t, HUSH_ERR_INPUT, hush_last_error(h));
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(h->device);
   const int passes[3] = {0, 4, 5};

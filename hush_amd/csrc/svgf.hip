@@ -478,6 +478,9 @@ static int check_inputs(svgf_t *s, const svgf_inputs *in) {
     if (v[k]->pitch % 8 != 0 || v[k]->pitch < (size_t)s->W * 8 || ((uintptr_t)v[k]->ptr & 7) != 0)
       return fail(s, HUSH_ERR_INPUT, "SVGF.encode: '%s' has pitch %zu bytes; rows need 8-byte alignment and >= %d bytes", kInputName[k],
                   v[k]->pitch, s->W * 8);
+    if (!hush_pitch_ok(v[k]->pitch / 8, s->H))
+      return fail(s, HUSH_ERR_INPUT, "SVGF.encode: '%s' has pitch %zu bytes; pitch x height must stay below 2^31 texels",
+                  kInputName[k], v[k]->pitch);
   }
   return HUSH_OK;
 }

@@ -36,7 +36,7 @@ enum hush_status {
   HUSH_OK = 0,
   HUSH_ERR_ARGUMENT = -1,     /* null pointer or invalid enum */
   HUSH_ERR_WEIGHTS = -2,      /* weight count fits no MLP width (hush.js:743) */
-  HUSH_ERR_SIZE = -3,         /* non-positive size (hush.js:765) */
+  HUSH_ERR_SIZE = -3,         /* non-positive size (hush.js:765), or a side above HUSH_MAX_SIDE */
   HUSH_ERR_INPUT = -4,        /* missing input or size mismatch (hush.js:816, 818) */
   HUSH_ERR_DEVICE = -5,       /* HIP runtime error (allocation, launch) */
   HUSH_ERR_UNSUPPORTED = -6   /* precision / width combination not built */
@@ -54,6 +54,10 @@ enum hush_precision {
 };
 /* The MFMA forms need hidden width 16 (the built-in 20-16-16-12 network); other widths report
    HUSH_ERR_UNSUPPORTED for them. */
+
+/* Largest width or height (WebGPU's maxTextureDimension2D is 8192 by default); the kernels' 32-bit texel
+   and workgroup indices hold within this bound. */
+#define HUSH_MAX_SIDE 16384
 
 enum hush_format { HUSH_RGBA16F = 0, HUSH_RGBA32U = 1, HUSH_R8U = 2, HUSH_RGBA32F = 3 };
 

@@ -157,5 +157,13 @@ def test_input_errors_match_reference():
     missing = {k: v for k, v in good.items() if k != "material"}
     with pytest.raises(hush_amd.HushError, match=r"^Hush\.denoise: missing input 'material'$"):
         hs.denoise(missing)
+    # a pitch whose rows would run past 2^31 texels (the kernels index texels in 32 bits): refused before
+    # anything is launched, so the view's memory is never read
+    r = good["radiance"]
+    wide = dict(good, radiance=Surface(64, 48, ptr=r.ptr, pitch=(1 << 23) * 8, owner=r))
+    with pytest.raises(hush_amd.HushError, match=r"^Hush\.denoise: 'radiance' has pitch 67108864 bytes; pitch x height"):
+        hs.denoise(wide)
     hs.denoise(good)                                   # the context is still usable
     hs.destroy()
+    with pytest.raises(hush_amd.HushError, match=r"bad size 16385x8"):
+        hush_amd.Hush(0, {"width": 16385, "height": 8})

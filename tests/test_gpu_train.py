@@ -249,3 +249,21 @@ def test_adam_lowers_the_loss_with_mfma_forward():
     losses = [l for _, l in hist]
     assert np.isfinite(losses).all()
     assert np.mean(losses[-3:]) < np.mean(losses[:3]), losses
+
+
+def test_prepare_refuses_bad_inputs():
+    """prepare validates the G-buffer like denoise (size, presence, pitch) before anything is launched"""
+    import hush_amd
+    from hush_amd import train as T
+    from hush_amd.hip import Surface
+    from tests.parity import INPUT_KEYS
+    hs = hush_amd.Hush(0, {"width": 64, "height": 48})
+    fr = T.DiffFrame(hs)
+    good = {k: Surface(64, 48) for k in INPUT_KEYS}
+    with pytest.raises(hush_amd.HushError, match=r"^Hush\.denoise: 'motion' is 32x48, the denoiser 64x48 \(call resize\)$"):
+        fr.prepare(dict(good, motion=Surface(32, 48)))
+    with pytest.raises(hush_amd.HushError, match=r"^Hush\.denoise: missing input 'material'$"):
+        fr.prepare({k: v for k, v in good.items() if k != "material"})
+    fr.prepare(good)                                   # still usable
+    fr.destroy()
+    hs.destroy()
