@@ -187,11 +187,8 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
     }
   }
   for (int i = 0; i < 9 && e == hipSuccess; i++) e = hipEventCreate(&h->ev[i]);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);    // default priority
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
-    hip_fail(nullptr, e, "timing events and side stream");
+    hip_fail(nullptr, e, "timing events");
     hush_destroy(h);
     return HUSH_ERR_DEVICE;
   }
@@ -215,9 +212,6 @@ extern "C" void hush_destroy(hush_t *h) {
   if (h->d_wscale) (void)hipFree(h->d_wscale);
   for (int i = 0; i < 9; i++)
     if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
-  if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
-  if (h->join_ev) (void)hipEventDestroy(h->join_ev);
-  if (h->side) (void)hipStreamDestroy(h->side);
   delete h;
 }
 
@@ -293,62 +287,81 @@ static hush_surface surf_of(const hush_t *h, int id) {
   return s;
 }
 
-// one pass of the frame at the context's current state (hush.js:858-870)
-hipError_t hush_run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream_t s) {
+// kernel arguments of each pass at the context's current state
+static TemporalArgs temporal_args(const hush_t *h, const hush_inputs *in) {
   const int cur = (int)(h->frame & 1), prev = cur ^ 1;
   const Tex *t = h->tex;
+  TemporalArgs a;
+  a.rad = in_tex(in->radiance); a.nd = in_tex(in->normal_depth); a.mv = in_tex(in->motion); a.mat = in_tex(in->material);
+  a.pkd = h->kd[prev]; a.pcol = t[S_HIST0 + prev]; a.pmom = t[S_MOM0 + prev]; a.ppar0 = t[S_PAR0_0 + prev];
+  a.outA = t[S_A0]; a.outMom = t[S_MOM0 + cur]; a.outKey = t[S_KEY0 + cur]; a.outKD = h->kd[cur];
+  a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.F3 = h->F[3]; a.outGH = h->gh;
+  a.mask = h->mask_on ? h->mask : nullptr; a.mask_pitch = h->mask_pitch; a.aux = h->train_aux; a.compensated = h->compensated ? 1 : 0;
+  a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.reset = h->reset ? 1 : 0; a.E = h->exposure;
+  return a;
+}
+static InferArgs infer_args(const hush_t *h) {
+  const int cur = (int)(h->frame & 1);
+  const Tex *t = h->tex;
+  InferArgs a;
+  a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.GH = h->gh;
+  a.out0 = t[S_PAR0_0 + cur]; a.out1 = t[S_PAR1]; a.out2 = t[S_PAR2]; a.outMid = t[S_BH0];
+  a.W2 = h->W2; a.H2 = h->H2; a.hidden = h->hidden; a.w = h->d_w; a.wq = h->d_wq; a.wscale = h->d_wscale;
+  return a;
+}
+static AtrousArgs mid_args(const hush_t *h, int pass) {      // pass 2 (mid2) or 3 (mid3)
+  const Tex *t = h->tex;
+  AtrousArgs a{};
+  a.inA = t[pass == 2 ? S_BH0 : S_BH1]; a.guide = h->gh; a.out = t[pass == 2 ? S_BH1 : S_BH0];
+  a.par1 = t[S_PAR1]; a.par2 = t[S_PAR2]; a.w = h->W2; a.h = h->H2; a.s = 0.0f;
+  return a;
+}
+static Coarse1Args coarse1_args(const hush_t *h) {
+  const Tex *t = h->tex;
+  Coarse1Args a;
+  a.F2 = h->F[2]; a.GH = h->gh; a.outC = t[S_C1]; a.outG = t[S_CG]; a.outGD = h->cgd;
+  a.W4 = h->W4; a.H4 = h->H4; a.W2 = h->W2; a.H2 = h->H2; a.s = 0.25f;   // COARSE_S (hush.js:26)
+  return a;
+}
+static AtrousArgs coarse2_args(const hush_t *h) {
+  const Tex *t = h->tex;
+  AtrousArgs a{};
+  a.inA = t[S_C1]; a.guide = h->cgd; a.out = t[S_C0]; a.w = h->W4; a.h = h->H4; a.s = 0.25f;
+  return a;
+}
+static FineArgs fine_args(const hush_t *h, const hush_inputs *in) {
+  const int cur = (int)(h->frame & 1);
+  const Tex *t = h->tex;
+  FineArgs a;
+  a.A0 = t[S_A0]; a.kd = h->kd[cur]; a.par0 = t[S_PAR0_0 + cur]; a.par1 = t[S_PAR1]; a.par2 = t[S_PAR2];
+  a.F2 = h->F[2]; a.GH = h->gh; a.mid = t[S_BH0]; a.coarse = t[S_C0]; a.cgd = h->cgd;
+  a.mat = in_tex(in->material); a.emi = in_tex(in->emission);
+  a.outHist = t[S_HIST0 + cur]; a.outX = t[S_X];
+  a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.W4 = h->W4; a.H4 = h->H4;
+  return a;
+}
+static TaaArgs taa_args(const hush_t *h, const hush_inputs *in) {
+  const int cur = (int)(h->frame & 1), prev = cur ^ 1;
+  const Tex *t = h->tex;
+  TaaArgs a;
+  a.X = t[S_X]; a.mv = in_tex(in->motion); a.prev = t[S_TAA0 + prev]; a.out = t[S_TAA0 + cur];
+  a.W = h->W; a.H = h->H; a.reset = h->reset ? 1 : 0;
+  a.rW = 1.0f / (float)h->W; a.rH = 1.0f / (float)h->H;
+  return a;
+}
+
+// one pass of the frame at the context's current state (hush.js:858-870)
+hipError_t hush_run_pass(hush_t *h, int pass, const hush_inputs *in, hipStream_t s) {
+  const bool fast = !h->ieee;
   switch (pass) {
-    case 0: {
-      TemporalArgs a;
-      a.rad = in_tex(in->radiance); a.nd = in_tex(in->normal_depth); a.mv = in_tex(in->motion); a.mat = in_tex(in->material);
-      a.pkd = h->kd[prev]; a.pcol = t[S_HIST0 + prev]; a.pmom = t[S_MOM0 + prev]; a.ppar0 = t[S_PAR0_0 + prev];
-      a.outA = t[S_A0]; a.outMom = t[S_MOM0 + cur]; a.outKey = t[S_KEY0 + cur]; a.outKD = h->kd[cur];
-      a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.F3 = h->F[3]; a.outGH = h->gh;
-      a.mask = h->mask_on ? h->mask : nullptr; a.mask_pitch = h->mask_pitch; a.aux = h->train_aux; a.compensated = h->compensated ? 1 : 0;
-      a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.reset = h->reset ? 1 : 0; a.E = h->exposure;
-      return launch_temporal(a, !h->ieee, s);
-    }
-    case 1: {
-      InferArgs a;
-      a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.GH = h->gh;
-      a.out0 = t[S_PAR0_0 + cur]; a.out1 = t[S_PAR1]; a.out2 = t[S_PAR2]; a.outMid = t[S_BH0];
-      a.W2 = h->W2; a.H2 = h->H2; a.hidden = h->hidden; a.w = h->d_w; a.wq = h->d_wq; a.wscale = h->d_wscale;
-      return launch_infer(a, h->precision, !h->ieee, s);
-    }
-    case 2:
-    case 3: {
-      AtrousArgs a{};
-      a.inA = t[pass == 2 ? S_BH0 : S_BH1]; a.guide = h->gh; a.out = t[pass == 2 ? S_BH1 : S_BH0];
-      a.par1 = t[S_PAR1]; a.par2 = t[S_PAR2]; a.w = h->W2; a.h = h->H2; a.s = 0.0f;
-      return pass == 2 ? launch_mid2(a, !h->ieee, s) : launch_mid3(a, !h->ieee, s);
-    }
-    case 4: {
-      Coarse1Args a;
-      a.F2 = h->F[2]; a.GH = h->gh; a.outC = t[S_C1]; a.outG = t[S_CG]; a.outGD = h->cgd;
-      a.W4 = h->W4; a.H4 = h->H4; a.W2 = h->W2; a.H2 = h->H2; a.s = 0.25f;   // COARSE_S (hush.js:26)
-      return launch_coarse1(a, !h->ieee, s);
-    }
-    case 5: {
-      AtrousArgs a{};
-      a.inA = t[S_C1]; a.guide = h->cgd; a.out = t[S_C0]; a.w = h->W4; a.h = h->H4; a.s = 0.25f;
-      return launch_coarse2(a, !h->ieee, s);
-    }
-    case 6: {
-      FineArgs a;
-      a.A0 = t[S_A0]; a.kd = h->kd[cur]; a.par0 = t[S_PAR0_0 + cur]; a.par1 = t[S_PAR1]; a.par2 = t[S_PAR2];
-      a.F2 = h->F[2]; a.GH = h->gh; a.mid = t[S_BH0]; a.coarse = t[S_C0]; a.cgd = h->cgd;
-      a.mat = in_tex(in->material); a.emi = in_tex(in->emission);
-      a.outHist = t[S_HIST0 + cur]; a.outX = t[S_X];
-      a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.W4 = h->W4; a.H4 = h->H4;
-      return launch_fine_final(a, !h->ieee, s);
-    }
-    case 7: {
-      TaaArgs a;
-      a.X = t[S_X]; a.mv = in_tex(in->motion); a.prev = t[S_TAA0 + prev]; a.out = t[S_TAA0 + cur];
-      a.W = h->W; a.H = h->H; a.reset = h->reset ? 1 : 0;
-      a.rW = 1.0f / (float)h->W; a.rH = 1.0f / (float)h->H;
-      return launch_taa(a, !h->ieee, s);
-    }
+    case 0: return launch_temporal(temporal_args(h, in), fast, s);
+    case 1: return launch_infer(infer_args(h), h->precision, fast, s);
+    case 2: return launch_mid2(mid_args(h, 2), fast, s);
+    case 3: return launch_mid3(mid_args(h, 3), fast, s);
+    case 4: return launch_coarse1(coarse1_args(h), fast, s);
+    case 5: return launch_coarse2(coarse2_args(h), fast, s);
+    case 6: return launch_fine_final(fine_args(h, in), fast, s);
+    case 7: return launch_taa(taa_args(h, in), fast, s);
     default:
       return hipErrorInvalidValue;
   }
@@ -387,16 +400,13 @@ extern "C" int32_t hush_encode(hush_t *h, const hush_inputs *in, void *stream, h
     (void)hipEventRecord(h->ev[8], s);
     h->ev_ready = true;
   } else {
-    // temporal; then coarse1 -> coarse2 on the side stream beside infer -> mid2 -> mid3; fine + taa after both
-    // (the next frame's coarse passes start after its temporal pass, which follows this frame's fine pass)
-    static const int main_passes[3] = {1, 2, 3};
+    // six launches on the caller's stream: temporal; coarse1 beside infer + mid1; mid2; coarse2 beside mid3;
+    // fine + final; TAA (the coarse level needs only the temporal pass's outputs)
+    const bool fast = !h->ieee;
     e = hush_run_pass(h, 0, in, s);
-    if (e == hipSuccess) e = hipEventRecord(h->fork_ev, s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->fork_ev, 0);
-    for (int p = 4; p <= 5 && e == hipSuccess; p++) e = hush_run_pass(h, p, in, h->side);
-    if (e == hipSuccess) e = hipEventRecord(h->join_ev, h->side);
-    for (int k = 0; k < 3 && e == hipSuccess; k++) e = hush_run_pass(h, main_passes[k], in, s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(s, h->join_ev, 0);
+    if (e == hipSuccess) e = launch_coarse1_infer(coarse1_args(h), infer_args(h), h->precision, fast, s);
+    if (e == hipSuccess) e = hush_run_pass(h, 2, in, s);
+    if (e == hipSuccess) e = launch_coarse2_mid3(coarse2_args(h), mid_args(h, 3), fast, s);
     for (int p = 6; p <= 7 && e == hipSuccess; p++) e = hush_run_pass(h, p, in, s);
     if (e != hipSuccess) return hip_fail(h, e, "hush_encode");
   }

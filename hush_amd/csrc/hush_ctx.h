@@ -33,15 +33,11 @@ struct hush_ctx {
   int mask_pitch = 0;
   bool mask_on = false;
   float *train_aux = nullptr;      // set by the differentiable frame around its temporal pass
-  bool ieee = true;
-  bool compensated = false;        // compensated (Neumaier) moment sums in the temporal statistics                // IEEE math policy (the oracle's operation-exact mirror); false = FAST
-  bool timing = false;
+  bool ieee = true;                 // IEEE math policy (the oracle's operation-exact mirror); false = FAST
+  bool compensated = false;         // compensated (Neumaier) moment sums in the temporal statistics
+  bool timing = false;              // timed frames: the eight passes one after another, events between them
   bool ev_ready = false;
   hipEvent_t ev[9] = {};
-  // the coarse level (passes 4, 5) depends only on the temporal pass: it runs on `side`, concurrently with
-  // infer / mid2 / mid3, forked and joined with events (untimed frames)
-  hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   char err[512] = {0};
 };
 

@@ -92,18 +92,21 @@ __device__ __forceinline__ float mx(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float lum(float r, float g, float b) { return r * 0.2126f + g * 0.7152f + b * 0.0722f; }
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
-// Tile origin of this workgroup in a 1-D grid of ceil(w/TW) * ceil(h/TH) workgroups.  The dispatcher hands
+// Tile origin of workgroup n in a 1-D grid of ceil(w/TW) * ceil(h/TH) workgroups.  The dispatcher hands
 // workgroup n to XCD n % 8, and each of the 8 XCDs has its own L2; remapping n gives every XCD one
 // contiguous band of tiles in raster order, so neighbouring tiles (which share apron texels) meet in the
-// same L2 instead of fetching the apron again from HBM.
+// same L2 instead of fetching the apron again from HBM.  A launch that runs two passes passes n relative
+// to its pass's first workgroup, which starts at a multiple of 8 (n keeps its XCD).
 template <int TW, int TH>
-__device__ __forceinline__ int2 tile_origin(int w, int h) {
+__device__ __forceinline__ int2 tile_origin(int w, int h, int n) {
   const int gx = (w + TW - 1) / TW, gy = (h + TH - 1) / TH;
-  const int T = gx * gy, n = blockIdx.x;
+  const int T = gx * gy;
   const int q = T >> 3, r = T & 7, x = n & 7, i = n >> 3;
   const int L = x * q + (x < r ? x : r) + i;
   return make_int2((L % gx) * TW, (L / gx) * TH);
 }
+template <int TW, int TH>
+__device__ __forceinline__ int2 tile_origin(int w, int h) { return tile_origin<TW, TH>(w, h, (int)blockIdx.x); }
 
 // Math policies.  IEEE: correctly rounded division and square root, ocml expf / logf -- the operation-exact
 // mirror of the oracle.  FAST: hardware reciprocal, square root, exp2 and log2 (about 1-2 ulp in fp32),

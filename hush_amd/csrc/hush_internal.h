@@ -89,6 +89,9 @@ hipError_t launch_mid2(const AtrousArgs &a, bool fast, hipStream_t s);
 hipError_t launch_mid3(const AtrousArgs &a, bool fast, hipStream_t s);
 hipError_t launch_coarse1(const Coarse1Args &a, bool fast, hipStream_t s);
 hipError_t launch_coarse2(const AtrousArgs &a, bool fast, hipStream_t s);
+// coarse1 beside infer + mid1, coarse2 beside mid3: one launch each, same results as the separate passes
+hipError_t launch_coarse1_infer(const Coarse1Args &c, const InferArgs &a, int mode, bool fast, hipStream_t s);
+hipError_t launch_coarse2_mid3(const AtrousArgs &c, const AtrousArgs &m, bool fast, hipStream_t s);
 hipError_t launch_fine_final(const FineArgs &a, bool fast, hipStream_t s);
 hipError_t launch_taa(const TaaArgs &a, bool fast, hipStream_t s);
 hipError_t launch_pack_mlp(const float *w, MfmaPack *p, float s1, float s2, float s3, hipStream_t s);

@@ -554,12 +554,16 @@ __device__ __forceinline__ void unpack_features(uint4 f0, uint4 f1, uint4 f2, fl
   }
 }
 
+// Each pass's tile body takes its workgroup index n within the pass and an LDS block of the size its
+// *_lds() gives (in float4 units), so that one launch can run two independent passes side by side.
+constexpr int infer_lds() { return 2 * (TW + 2) * (TH + 2); }
+
 template <int HID, int MODE, bool FAST>   // MODE 0: fp32, 1: shader-f16 arithmetic
-__global__ __launch_bounds__(NT) void k_infer(InferArgs a) {
+__device__ __forceinline__ void infer_tile(const InferArgs &a, int n, float4 *lds) {
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
-  __shared__ float4 sA[SN], sG[SN];
+  float4 *sA = lds, *sG = lds + SN;
   const int tid = threadIdx.x;
-  const int2 org = tile_origin<TW, TH>(a.W2, a.H2);
+  const int2 org = tile_origin<TW, TH>(a.W2, a.H2, n);
   const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.W2 - 1), qy = clampi(y0 - AP + i / SW, 0, a.H2 - 1);
@@ -584,6 +588,11 @@ __global__ __launch_bounds__(NT) void k_infer(InferArgs a) {
   if (MODE == 0) mlp_fp32<HID>(a.w, a.hidden, f, o);
   else mlp_f16(a.w, a.hidden, f, o);
   infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
+}
+template <int HID, int MODE, bool FAST>
+__global__ __launch_bounds__(NT) void k_infer(InferArgs a) {
+  __shared__ float4 lds[infer_lds()];
+  infer_tile<HID, MODE, FAST>(a, (int)blockIdx.x, lds);
 }
 
 // ============================================================================ MFMA forms of the MLP
@@ -840,12 +849,18 @@ __device__ __forceinline__ void mlp_mfma_x4(const MfmaPack *__restrict__ P, cons
 // 16-texel tiles straight from F0 / F1 / F2 (tile g = texels 16(g&1) .. +15 of tile row 2v + (g>>1)), the
 // four MFMA chains run interleaved, and the D fragments (+ bias) go through LDS at a 48-byte texel stride
 // (16-byte stores and loads, conflict free) so each lane ends with the 12 outputs of its own texel.
+// LDS: the staged tile, then per wave the 64 x 3 output float4s (fp16 / bf16) or the fp8 form's 64 x 16
+// activation words and 64 x 16 outputs
+template <int MODE>
+constexpr int infer_mfma_lds() {
+  return infer_lds() + ((MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) ? 4 * 64 * 3 : 2 * 4 * 64 * 16 / 4);
+}
 template <int MODE, bool FAST>
-__global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
+__device__ __forceinline__ void infer_mfma_tile(const InferArgs &a, int n, float4 *lds) {
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
-  __shared__ float4 sA[SN], sG[SN];
+  float4 *sA = lds, *sG = lds + SN;
   const int tid = threadIdx.x;
-  const int2 org = tile_origin<TW, TH>(a.W2, a.H2);
+  const int2 org = tile_origin<TW, TH>(a.W2, a.H2, n);
   const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.W2 - 1), qy = clampi(y0 - AP + i / SW, 0, a.H2 - 1);
@@ -859,7 +874,7 @@ __global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
   const int wv = tid >> 6, lane = tid & 63, kb = lane >> 4, tl = lane & 15;
   const MfmaPack *__restrict__ P = (const MfmaPack *)a.wq;
   if constexpr (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
-    __shared__ float4 sO[4][64 * 3];
+    float4 (*sO)[64 * 3] = (float4 (*)[64 * 3])(lds + 2 * SN);
     uint2 fx0[4], fx1[4];
     const TexU &Fk = kb < 2 ? a.F0 : a.F1;
 #pragma unroll
@@ -896,8 +911,8 @@ __global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
     }
     infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
   } else {
-    __shared__ uint32_t sX[4][64 * 16];
-    __shared__ float sOq[4][64 * 16];
+    uint32_t (*sX)[64 * 16] = (uint32_t (*)[64 * 16])(lds + 2 * SN);
+    float (*sOq)[64 * 16] = (float (*)[64 * 16])(lds + 2 * SN + 4 * 64 * 16 / 4);
     const bool in_img = inside(X, Y, a.W2, a.H2);
     uint32_t u[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (in_img) {
@@ -916,6 +931,11 @@ __global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
     }
     infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
   }
+}
+template <int MODE, bool FAST>
+__global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
+  __shared__ float4 lds[infer_mfma_lds<MODE>()];
+  infer_mfma_tile<MODE, FAST>(a, (int)blockIdx.x, lds);
 }
 
 // The MLP's 12 outputs (before the activations, bias included) of every half-res texel, read from F0..F2
@@ -1093,13 +1113,18 @@ __global__ __launch_bounds__(NT) void k_mlp(const uint32_t *__restrict__ feat, f
 // and learned s/c from par1/par2; coarse2 (R2 step 2, prefilter) at quarter res with the CG guide.
 
 // GUIDE_PACKED: guide w = binary16 (Z, count) (GH); else w = depth (CGD)
+template <int R, int STEP>
+constexpr int atrous_lds() {
+  constexpr int AP = R * STEP > 1 ? R * STEP : 1;
+  return 2 * (TW + 2 * AP) * (TH + 2 * AP);
+}
 template <int R, int STEP, bool PRE, bool GUIDE_PACKED, int COMP, bool FAST>
-__global__ __launch_bounds__(NT) void k_atrous(AtrousArgs a) {
+__device__ __forceinline__ void atrous_tile(const AtrousArgs &a, int n, float4 *lds) {
   constexpr int AP = R * STEP > 1 ? R * STEP : 1;
   constexpr int SW = TW + 2 * AP, SH = TH + 2 * AP, SN = SW * SH;
-  __shared__ float4 sA[SN], sG[SN];
+  float4 *sA = lds, *sG = lds + SN;
   const int tid = threadIdx.x;
-  const int2 org = tile_origin<TW, TH>(a.w, a.h);
+  const int2 org = tile_origin<TW, TH>(a.w, a.h, n);
   const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.w - 1), qy = clampi(y0 - AP + i / SW, 0, a.h - 1);
@@ -1121,6 +1146,11 @@ __global__ __launch_bounds__(NT) void k_atrous(AtrousArgs a) {
     c = COMP == 0 ? p2.x : COMP == 1 ? p2.y : COMP == 2 ? p2.z : p2.w;
   }
   st(a.out, X, Y, filt<R, STEP, PRE, SW, FAST>(sA, sG, tx, ty, X, Y, a.w, a.h, s, c));
+}
+template <int R, int STEP, bool PRE, bool GUIDE_PACKED, int COMP, bool FAST>
+__global__ __launch_bounds__(NT) void k_atrous(AtrousArgs a) {
+  __shared__ float4 lds[atrous_lds<R, STEP>()];
+  atrous_tile<R, STEP, PRE, GUIDE_PACKED, COMP, FAST>(a, (int)blockIdx.x, lds);
 }
 
 // ============================================================================ pass 5: coarse1
@@ -1156,12 +1186,13 @@ __device__ __forceinline__ void down(const TexU &F2, const TexF &GH, int QX, int
   g = make_float4(qf16(o.x), qf16(o.y), qf16(F::divs(z, N, rN)), 0.0f);
 }
 
+constexpr int coarse1_lds() { return 2 * (TW + 4) * (TH + 4); }
 template <bool FAST>
-__global__ __launch_bounds__(NT) void k_coarse1(Coarse1Args a) {
+__device__ __forceinline__ void coarse1_tile(const Coarse1Args &a, int n, float4 *lds) {
   constexpr int AP = 2, SW = TW + 2 * AP, SH = TH + 2 * AP, SN = SW * SH;
-  __shared__ float4 sA[SN], sG[SN];
+  float4 *sA = lds, *sG = lds + SN;
   const int tid = threadIdx.x;
-  const int2 org = tile_origin<TW, TH>(a.W4, a.H4);
+  const int2 org = tile_origin<TW, TH>(a.W4, a.H4, n);
   const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int lqx = x0 - AP + i % SW, lqy = y0 - AP + i / SW;
@@ -1185,6 +1216,51 @@ __global__ __launch_bounds__(NT) void k_coarse1(Coarse1Args a) {
   const int tx = lx + AP, ty = ly + AP;
   if (sG[ty * SW + tx].w < 0.0f) { st(a.outC, X, Y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
   st(a.outC, X, Y, filt<2, 1, true, SW, FAST>(sA, sG, tx, ty, X, Y, a.W4, a.H4, a.s, 1.0f));
+}
+template <bool FAST>
+__global__ __launch_bounds__(NT) void k_coarse1(Coarse1Args a) {
+  __shared__ float4 lds[coarse1_lds()];
+  coarse1_tile<FAST>(a, (int)blockIdx.x, lds);
+}
+
+// ============================================================================ two passes in one launch
+// The coarse passes depend only on their own inputs: coarse1 on the temporal pass (like infer + mid1) and
+// coarse2 on coarse1 (mid3 on mid2).  Running each beside a pass of the main chain in one launch fills the
+// small coarse grids into that launch instead of a second stream, whose cross-stream waits leave the GPU
+// idle for several microseconds each.  Workgroups [0, nA) run the coarse pass's tiles, [nApad, ...) the
+// other pass's; nApad = nA rounded up to a multiple of 8 keeps every tile on the XCD its own launch would
+// give it.  The LDS block is the larger of the two passes' blocks.
+constexpr int imax(int a, int b) { return a > b ? a : b; }
+
+template <int HID, int MODE, bool FAST>
+__global__ __launch_bounds__(NT) void k_coarse1_infer(Coarse1Args c, InferArgs a, int nA, int nApad) {
+  __shared__ float4 lds[imax(coarse1_lds(), infer_lds())];
+  const int b = (int)blockIdx.x;
+  if (b < nApad) {
+    if (b < nA) coarse1_tile<FAST>(c, b, lds);
+    return;
+  }
+  infer_tile<HID, MODE, FAST>(a, b - nApad, lds);
+}
+template <int MODE, bool FAST>
+__global__ __launch_bounds__(NT) void k_coarse1_infer_mfma(Coarse1Args c, InferArgs a, int nA, int nApad) {
+  __shared__ float4 lds[imax(coarse1_lds(), infer_mfma_lds<MODE>())];
+  const int b = (int)blockIdx.x;
+  if (b < nApad) {
+    if (b < nA) coarse1_tile<FAST>(c, b, lds);
+    return;
+  }
+  infer_mfma_tile<MODE, FAST>(a, b - nApad, lds);
+}
+template <bool FAST>
+__global__ __launch_bounds__(NT) void k_coarse2_mid3(AtrousArgs c, AtrousArgs m, int nA, int nApad) {
+  __shared__ float4 lds[imax(atrous_lds<2, 2>(), atrous_lds<1, 4>())];
+  const int b = (int)blockIdx.x;
+  if (b < nApad) {
+    if (b < nA) atrous_tile<2, 2, true, false, -1, FAST>(c, b, lds);
+    return;
+  }
+  atrous_tile<1, 4, false, true, 3, FAST>(m, b - nApad, lds);
 }
 
 // ============================================================================ pass 7: fine + final
@@ -1523,6 +1599,46 @@ hipError_t launch_coarse1(const Coarse1Args &a, bool fast, hipStream_t s) {
 }
 hipError_t launch_coarse2(const AtrousArgs &a, bool fast, hipStream_t s) {
   HUSH_LAUNCH(K_COARSE2, grid_for(a.w, a.h), a);
+  return hipGetLastError();
+}
+#define HUSH_LAUNCH_N(KERNEL, GRID, ...)                                                 \
+  do {                                                                                   \
+    if (fast) hipLaunchKernelGGL(KERNEL(true), GRID, dim3(NT), 0, s, __VA_ARGS__);      \
+    else hipLaunchKernelGGL(KERNEL(false), GRID, dim3(NT), 0, s, __VA_ARGS__);          \
+  } while (0)
+#define K_C1_INFER16(F) k_coarse1_infer<16, 0, F>
+#define K_C1_INFER(F) k_coarse1_infer<0, 0, F>
+#define K_C1_INFER_F16(F) k_coarse1_infer<0, 1, F>
+#define K_C1_INFER_H(F) k_coarse1_infer_mfma<MLP_FP16_MFMA, F>
+#define K_C1_INFER_B(F) k_coarse1_infer_mfma<MLP_BF16_MFMA, F>
+#define K_C1_INFER_Q(F) k_coarse1_infer_mfma<MLP_FP8_MFMA, F>
+#define K_C2_MID3(F) k_coarse2_mid3<F>
+
+hipError_t launch_coarse1_infer(const Coarse1Args &c, const InferArgs &a, int mode, bool fast, hipStream_t s) {
+  const int nA = (int)grid_for(c.W4, c.H4).x, nApad = (nA + 7) & ~7;
+  const dim3 g((unsigned)nApad + grid_for(a.W2, a.H2).x);
+  if (mode == MLP_FP32) {
+    if (a.hidden == 16) HUSH_LAUNCH_N(K_C1_INFER16, g, c, a, nA, nApad);
+    else HUSH_LAUNCH_N(K_C1_INFER, g, c, a, nA, nApad);
+  } else if (mode == MLP_F16) {
+    HUSH_LAUNCH_N(K_C1_INFER_F16, g, c, a, nA, nApad);
+  } else if (a.hidden != 16 || !a.wq) {
+    return hipErrorInvalidValue;
+  } else if (mode == MLP_FP16_MFMA) {
+    HUSH_LAUNCH_N(K_C1_INFER_H, g, c, a, nA, nApad);
+  } else if (mode == MLP_BF16_MFMA) {
+    HUSH_LAUNCH_N(K_C1_INFER_B, g, c, a, nA, nApad);
+  } else if (mode == MLP_FP8_MFMA) {
+    HUSH_LAUNCH_N(K_C1_INFER_Q, g, c, a, nA, nApad);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_coarse2_mid3(const AtrousArgs &c, const AtrousArgs &m, bool fast, hipStream_t s) {
+  const int nA = (int)grid_for(c.w, c.h).x, nApad = (nA + 7) & ~7;
+  const dim3 g((unsigned)nApad + grid_for(m.w, m.h).x);
+  HUSH_LAUNCH_N(K_C2_MID3, g, c, m, nA, nApad);
   return hipGetLastError();
 }
 hipError_t launch_fine_final(const FineArgs &a, bool fast, hipStream_t s) {

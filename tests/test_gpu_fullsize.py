@@ -8,8 +8,8 @@ for a frame-by-frame comparison:
   every pixel from the second frame on (bit-exact);
 * a frame after reset() is bit-identical to the first frame of a fresh context;
 * the same sequence run twice gives bit-identical outputs (no order-dependent reductions, no races);
-* the overlapped schedule (coarse passes on a second stream) gives bit-identical outputs to the serial
-  one (per-pass timing enabled)."""
+* the default schedule (coarse1 and coarse2 in the launches of infer + mid1 and mid3) gives bit-identical
+  outputs to the eight separate passes of a timed frame."""
 import numpy as np
 import pytest
 
