@@ -8,6 +8,8 @@
 // The arithmetic follows the reference's operation order with no FMA contraction (build flag
 // -ffp-contract=off); the MLP's multiply-adds are fused explicitly (__builtin_fmaf).
 #include "hush_device.h"
+#include <type_traits>
+
 #include "hush_internal.h"
 
 namespace hush {
@@ -409,9 +411,11 @@ __host__ __device__ constexpr double tap_dist(int d2) {
        : d2 == 8 ? 2.8284271247461903 : 0.0;
 }
 
-template <int R, int STEP, bool PRE, int SW, bool FAST>
+// CHECK = false: the caller guarantees that every tap lies inside the image (an interior tile).  sL (when
+// LUM): lum() of each staged texel's rgb, the value the taps would compute.
+template <int R, int STEP, bool PRE, int SW, bool FAST, bool CHECK = true, bool LUM = false>
 __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int tx, int ty, int qx, int qy, int w, int h,
-                                       float s, float cc) {
+                                       float s, float cc, const float *sL = nullptr) {
   using F = Math<FAST>;
   const int c = ty * SW + tx;
   const float4 gp = sG[c], center = sA[c];
@@ -423,7 +427,7 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
   } else {
     gv = center.w;
   }
-  const float lc = lum(center.x, center.y, center.z);
+  const float lc = LUM ? sL[c] : lum(center.x, center.y, center.z);
   const float sl = F::div(s, F::sqrtn(mx(gv, 1e-12f)) + 1e-6f);
   const float zl = sG[c - 1].w, zr = sG[c + 1].w, zu = sG[c - SW].w, zd = sG[c + SW].w;
   const float gx = mn(zr > 0.0f ? fabsf(zr - gp.w) : 1e9f, zl > 0.0f ? fabsf(zl - gp.w) : 1e9f);
@@ -444,11 +448,12 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
       if (dx == 0 && dy == 0) continue;
       const int o = dy * STEP * SW + dx * STEP;
       const float4 gq = sG[c + o];
-      if (inside(qx + dx * STEP, qy + dy * STEP, w, h) && gq.w >= 0.0f) {
+      if ((!CHECK || inside(qx + dx * STEP, qy + dy * STEP, w, h)) && gq.w >= 0.0f) {
         const float4 cq = sA[c + o];
+        const float lq = LUM ? sL[c + o] : lum(cq.x, cq.y, cq.z);
         const float ww = (float)(Kern<R>::h[dx + R] * Kern<R>::h[dy + R])
                        * pow128(mx(dot3(make_float3(gq.x, gq.y, gq.z), make_float3(gp.x, gp.y, gp.z)), 0.0f))
-                       * F::expn(-fabsf(gq.w - gp.w) * iz[dx * dx + dy * dy] - sl * fabsf(lum(cq.x, cq.y, cq.z) - lc));
+                       * F::expn(-fabsf(gq.w - gp.w) * iz[dx * dx + dy * dy] - sl * fabsf(lq - lc));
         sw += ww;
         scx += ww * cq.x; scy += ww * cq.y; scz += ww * cq.z;
         sv += ww * ww * cq.w;
@@ -1293,6 +1298,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   using F = Math<FAST>;
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
   __shared__ float4 sA[SN], sG[SN];
+  __shared__ float sL[SN];                // lum(A0 rgb)
   __shared__ float4 sHG[HWN], sHM[HWN];   // half-res: decoded guide (normal, depth); mid rgb + F2.z low
   __shared__ float2 sHA[HWN];             // half-res: F2.z high, F2.w low
   __shared__ float4 sQG[QWN], sQC[QWN];   // quarter-res: decoded guide; coarse
@@ -1303,7 +1309,9 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   const int hx0 = (x0 >> 1) - 1, hy0 = (y0 >> 1) - 1, qx0 = (x0 >> 2) - 1, qy0 = (y0 >> 2) - 1;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
-    sA[i] = ld(a.A0, qx, qy);
+    const float4 a0 = ld(a.A0, qx, qy);
+    sA[i] = a0;
+    sL[i] = lum(a0.x, a0.y, a0.z);
     const float4 kd = ldf(a.kd, qx, qy);
     sG[i] = make_float4(kd.x, kd.y, kd.z, pair_of(kd.w).x);
   }
@@ -1340,8 +1348,15 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
     st(a.outX, px, py, make_float4(emi.x, emi.y, emi.z, 0.0f));
     return;
   }
+  // interior tiles (every tap of the fine filter and of both upsampling windows inside the image: all but
+  // the border tiles) run the pixel phase without the per-tap bounds tests
+  const bool interior = x0 >= 1 && y0 >= 1 && x0 + TW < a.W && y0 + TH < a.H && hx0 >= 0 && hy0 >= 0 &&
+                        hx0 + HWW <= a.W2 && hy0 + HWH <= a.H2 && qx0 >= 0 && qy0 >= 0 && qx0 + QWW <= a.W4 &&
+                        qy0 + QWH <= a.H4;
+  auto pixel = [&](auto in_tile) {
+  constexpr bool CHK = !decltype(in_tile)::value;
   const float4 pp = sPar[(ly >> 1) * (TW / 2) + (lx >> 1)];       // par1.x, par2.x, par0.z, par0.w
-  const float4 A1 = filt<1, 1, true, SW, FAST>(sA, sG, tx, ty, px, py, a.W, a.H, pp.x, pp.y);
+  const float4 A1 = filt<1, 1, true, SW, FAST, CHK, true>(sA, sG, tx, ty, px, py, a.W, a.H, pp.x, pp.y, sL);
   st(a.outHist, px, py, A1);
   const float fx = qf16(A1.x), fy = qf16(A1.y), fz = qf16(A1.z);
   const float3 n = make_float3(gp.x, gp.y, gp.z);
@@ -1354,7 +1369,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     int qx = (int)hbx + (j & 1), qy = (int)hby + (j >> 1);
-    if (qx < 0 || qy < 0 || qx >= a.W2 || qy >= a.H2) continue;
+    if (CHK && (qx < 0 || qy < 0 || qx >= a.W2 || qy >= a.H2)) continue;
     const int wi = (qy - hy0) * HWW + (qx - hx0);
     const float4 gh = sHG[wi];
     if (gh.w < 0.0f) continue;
@@ -1379,7 +1394,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     int qx = (int)cbx + (j & 1), qy = (int)cby + (j >> 1);
-    if (qx < 0 || qy < 0 || qx >= a.W4 || qy >= a.H4) continue;
+    if (CHK && (qx < 0 || qy < 0 || qx >= a.W4 || qy >= a.H4)) continue;
     const int wi = (qy - qy0) * QWW + (qx - qx0);
     const float4 g = sQG[wi];
     if (g.w < 0.0f) continue;
@@ -1397,6 +1412,9 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   iy = iy + (upy - iy) * pp.z;
   iz = iz + (upz - iz) * pp.z;
   st(a.outX, px, py, make_float4(ix * alb.x + emi.x, iy * alb.y + emi.y, iz * alb.z + emi.z, 0.0f));
+  };
+  if (interior) pixel(std::true_type{});
+  else pixel(std::false_type{});
 }
 
 // ============================================================================ pass 8: TAA
