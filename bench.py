@@ -158,6 +158,41 @@ def uhd_leg(hip, stream, steps=32, warmup=4, frames=32):
     return res
 
 
+def post_bytes(w, h, tw, th):
+    """algorithmic bytes of one Post.encode (src/demo/post.js:179-209): every texture a pass binds once per
+    texel, reads and writes (rgba16float 8 B, the 8-bit target 4 B); the additive upsample reads and
+    writes its destination"""
+    n = [max(1, w >> i) * max(1, h >> i) for i in range(7)]
+    composite = 16 * n[0]
+    down = sum(8 * n[i - 1] + 8 * n[i] for i in range(1, 7))
+    up = sum(8 * n[i + 1] + 16 * n[i] for i in range(1, 6))
+    final = 8 * n[0] + 8 * n[1] + 4 * tw * th
+    return composite + down + up + final
+
+
+def post_leg(dev, stream, hip, W, H, steps):
+    """the display post (src/demo/post.js) of two resident HDR images into a W x H bgra8 target"""
+    import hush_amd
+    from hush_amd.hip import Surface
+    post = hush_amd.Post(0, "bgra8unorm")
+    tgt = Surface(W, H, np.uint8, 4)
+    left, right = dev[0]["radiance"], dev[1]["radiance"]
+    for _ in range(4):
+        post.encode(left, right, split=0.5, target=tgt, stream=stream.handle)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream.handle)
+    for _ in range(steps):
+        post.encode(left, right, split=0.5, target=tgt, stream=stream.handle)
+    e1.record(stream.handle)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    post.destroy()
+    b = post_bytes(W, H, W, H)
+    return {"ms_per_frame": round(ms, 5), "kernels": 13, "algorithmic_MB": round(b / 1e6, 2),
+            "GB/s": round(b / (ms * 1e-3) / 1e9, 1), "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
+
+
 def svgf_leg(dev, stream, hip, W, H, steps, warmup):
     """the SVGF baseline (src/denoise/svgf.js) over the same resident frames: ms/frame and per-pass ms"""
     import hush_amd
@@ -369,6 +404,9 @@ def main():
                     help="after the warm-up frames, time only the MLP stage (hush_mlp_forward over the frame's half-res "
                          "feature rows, every precision, --steps launches each) and print it as one JSON line; for "
                          "profiling the MLP kernels on their own")
+    ap.add_argument("--post-only", action="store_true",
+                    help="time only the display post (Post.encode of two resident seeded HDR images into a "
+                         "width x height bgra8 target, --steps encodes) and print it as one JSON line")
     ap.add_argument("--uhd-error-frames", type=int, default=8,
                     help="frames of the 3840x2160 error check against the FP64 oracle (CPU-bound: ~4 s per frame)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -385,6 +423,13 @@ def main():
     from hush_amd.scene import SceneGenerator
 
     W, H, F = args.width, args.height, args.frames
+    if args.post_only:
+        rng = np.random.default_rng(0)
+        imgs = [{"radiance": hip.Surface(W, H).upload(rng.lognormal(-1.0, 1.0, (H, W, 4)).astype(np.float16))}
+                for _ in range(2)]
+        print(json.dumps({"post": post_leg(imgs, hip.Stream(), hip, W, H, args.steps), "width": W, "height": H}),
+              flush=True)
+        return
     t0 = time.perf_counter()
     gen = SceneGenerator("boxes", W, H, seed=0)
     frames_g = [gen.render(f) for f in range(F)]
@@ -482,6 +527,8 @@ def main():
         extras["fp16_mfma_ms_per_frame"] = round(e0.elapsed_ms(e1) / args.steps, 5)
         hm.destroy()
         extras["svgf_1080p"] = svgf_leg(dev, stream, hip, W, H, min(args.steps, 64), args.warmup)
+        extras["post_1080p"] = post_leg(dev, stream, hip, W, H, 100)
+        print(f"[bench] post {extras['post_1080p']}", file=sys.stderr, flush=True)
         extras["train_fp32"] = train_leg(dev, W, H)
         extras["scene_generator_1080p"] = scene_leg(W, H)
         print(f"[bench] train {extras['train_fp32']}", file=sys.stderr, flush=True)

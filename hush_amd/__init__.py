@@ -296,6 +296,8 @@ class Hush:
 
 
 from .svgf import SVGF, SVGFError, SVGFTimer, SVGF_INPUTS  # noqa: E402
+from .post import Post, PostError  # noqa: E402
 
 __all__ = ["Hush", "HushError", "HushTimer", "hiddenOf", "hidden_of", "default_weights", "decode_weights", "INPUTS",
-           "PASS_LABELS", "N_FEAT", "N_OUT", "Surface", "SVGF", "SVGFError", "SVGFTimer", "SVGF_INPUTS"]
+           "PASS_LABELS", "N_FEAT", "N_OUT", "Surface", "SVGF", "SVGFError", "SVGFTimer", "SVGF_INPUTS", "Post",
+           "PostError"]

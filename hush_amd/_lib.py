@@ -29,6 +29,9 @@ EXPORTS = [
     "hush_train_set_forward_precision", "hush_train_mlp_out",
     # include/hush_scene.h
     "hush_scene_create", "hush_scene_destroy", "hush_scene_last_error", "hush_scene_render",
+    # include/post.h
+    "post_default_options", "post_create", "post_destroy", "post_resize", "post_encode", "post_last_error",
+    "post_debug_surface",
 ]
 
 
@@ -59,6 +62,17 @@ class svgf_inputs(ctypes.Structure):
     _fields_ = [("radiance", hush_surface), ("normal_depth", hush_surface), ("motion", hush_surface),
                 ("material", hush_surface), ("emission", hush_surface), ("prev_normal_depth", hush_surface),
                 ("prev_motion", hush_surface)]
+
+
+class post_options(ctypes.Structure):
+    _fields_ = [("bloom", ctypes.c_float), ("vignette", ctypes.c_float)]
+
+
+class post_args(ctypes.Structure):
+    _fields_ = [("left", hush_surface), ("right", hush_surface), ("mid", hush_surface), ("split", ctypes.c_float),
+                ("split2", ctypes.c_float), ("same", ctypes.c_int32), ("exposure", ctypes.c_float),
+                ("line", ctypes.c_float), ("target", ctypes.c_void_p), ("target_pitch", ctypes.c_size_t),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("format", ctypes.c_int32)]
 
 
 _lib = None
@@ -127,5 +141,13 @@ def load() -> ctypes.CDLL:
     L.hush_scene_destroy.argtypes = [P]; L.hush_scene_destroy.restype = None
     L.hush_scene_last_error.argtypes = [P]; L.hush_scene_last_error.restype = ctypes.c_char_p
     L.hush_scene_render.argtypes = [P, ctypes.POINTER(hush_scene_frame), I, P, sz, P]; L.hush_scene_render.restype = i32
+    PO = ctypes.POINTER(post_options)
+    L.post_default_options.argtypes = [PO]; L.post_default_options.restype = None
+    L.post_create.argtypes = [i32, PO, ctypes.POINTER(P)]; L.post_create.restype = i32
+    L.post_destroy.argtypes = [P]; L.post_destroy.restype = None
+    L.post_resize.argtypes = [P, i32, i32]; L.post_resize.restype = i32
+    L.post_encode.argtypes = [P, ctypes.POINTER(post_args), P]; L.post_encode.restype = i32
+    L.post_last_error.argtypes = [P]; L.post_last_error.restype = ctypes.c_char_p
+    L.post_debug_surface.argtypes = [P, i32, S]; L.post_debug_surface.restype = i32
     _lib = L
     return L

@@ -23,7 +23,7 @@ HIP_FLAGS = [
     "-Wno-pass-failed", "-Wno-unused-command-line-argument",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
-HIP_SOURCES = ["hush_kernels.hip", "hush_api.hip", "svgf.hip", "hush_train.hip", "scene_gpu.hip"]
+HIP_SOURCES = ["hush_kernels.hip", "hush_api.hip", "svgf.hip", "hush_train.hip", "scene_gpu.hip", "post.hip"]
 
 
 def run(cmd, cwd=None):

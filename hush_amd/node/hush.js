@@ -67,6 +67,8 @@ function initDevice(ordinal = 0) {
     limits: { maxStorageTexturesPerShaderStage: 8 },
     features: new Set(['shader-f16']),
     createTexture(width, height) { return new DeviceTexture(device, width, height); },
+    // an 8-bit colour target for Post (the canvas texture of the reference's demo)
+    createCanvasTexture(width, height, format) { return new CanvasTexture(device, width, height, format); },
   };
   return device;
 }
@@ -187,4 +189,67 @@ class SVGF {
   }
 }
 
-module.exports = { Hush, SVGF, hiddenOf, initDevice, DeviceTexture, decodeWeights, defaultWeights, N_FEAT, N_OUT };
+const POST_FORMATS = { rgba8unorm: 0, bgra8unorm: 1 };
+
+// An 8-bit colour target (the canvas texture the reference's demo renders into): width x height 4-byte
+// texels in `format` byte order.
+class CanvasTexture {
+  constructor(device, width, height, format = 'bgra8unorm') {
+    if (!(format in POST_FORMATS)) throw new Error(`Post: unsupported target format '${format}'`);
+    this.device = device;
+    this.width = width;
+    this.height = height;
+    this.format = format;
+    this._s = native.alloc8(device.ordinal, width, height);
+  }
+  get surface() { return this._s; }
+  download() { return native.download8(this._s); }
+  destroy() {
+    if (this._s) native.free(this._s);
+    this._s = null;
+  }
+}
+
+// The display post of the reference's demo (src/demo/post.js:131-210): split-screen composite of two (or
+// three) linear HDR images, bloom, vignette, AgX tone map, sRGB, divider lines.
+// new Post(device, format, { bloom = 0.03, vignette = 0.35 }); encode({ left, right, mid = right,
+// split = 0.5, split2 = split, same = false, exposure = 1, target, width, height, line = 1.5 }, stream)
+// with DeviceTextures for the images and a CanvasTexture (or {ptr, pitch, width, height}) as the target.
+class Post {
+  constructor(device, format = 'bgra8unorm', opts = {}) {
+    if (!(format in POST_FORMATS)) throw new Error(`Post: unsupported target format '${format}'`);
+    const bloom = opts.bloom === undefined ? 0.03 : opts.bloom;
+    const vignette = opts.vignette === undefined ? 0.35 : opts.vignette;
+    this.device = device;
+    this.format = format;
+    this.bloom = bloom;
+    this.vignette = vignette;
+    this._p = native.postCreate(device.ordinal, { bloom, vignette });
+  }
+
+  // HDR and bloom surfaces for images of this size (no-op if unchanged); encode() calls it (post.js:149-158)
+  resize(width, height) { native.postResize(this._p, width, height); }
+
+  destroy() {
+    if (this._p) native.postDestroy(this._p);
+    this._p = null;
+  }
+
+  encode(args, stream) {
+    const s = (t) => t && (t.surface || t);
+    const split = args.split === undefined ? 0.5 : args.split;
+    const target = args.target;
+    native.postEncode(this._p, {
+      left: s(args.left), right: s(args.right), mid: s(args.mid || args.right), split,
+      split2: args.split2 === undefined ? split : args.split2, same: !!args.same,
+      exposure: args.exposure === undefined ? 1 : args.exposure, line: args.line === undefined ? 1.5 : args.line,
+      target: s(target), width: args.width === undefined ? target && target.width : args.width,
+      height: args.height === undefined ? target && target.height : args.height, format: POST_FORMATS[this.format],
+    }, stream || 0n);
+    return target;
+  }
+}
+
+module.exports = {
+  Hush, SVGF, Post, hiddenOf, initDevice, DeviceTexture, CanvasTexture, decodeWeights, defaultWeights, N_FEAT, N_OUT,
+};

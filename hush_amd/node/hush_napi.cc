@@ -1,4 +1,5 @@
-// Node.js N-API addon over the Hush C ABI (include/hush.h) and the SVGF baseline's (include/svgf.h).
+// Node.js N-API addon over the Hush C ABI (include/hush.h), the SVGF baseline's (include/svgf.h) and the
+// display post's (include/post.h).
 //
 // The JavaScript `Hush` class (hush_amd/node/hush.js) keeps the reference's API
 // (src/denoise/hush.js:725-875); this addon is the thin native layer under it: context calls map 1:1 onto
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "hush.h"
+#include "post.h"
 #include "svgf.h"
 
 namespace {
@@ -352,6 +354,39 @@ napi_value Download(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+// ---- 8-bit colour targets of the display post (4 bytes per texel)
+
+// alloc8(device, width, height) -> {ptr, pitch, width, height}
+napi_value Alloc8(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return nullptr;
+  int dev = (int)num(env, a[0]), w = (int)num(env, a[1]), hh = (int)num(env, a[2]);
+  if (w <= 0 || hh <= 0) return throw_err(env, "hush addon: bad surface size");
+  if (hipSetDevice(dev) != hipSuccess) return throw_err(env, "hush addon: hipSetDevice failed");
+  hush_surface s{};
+  s.pitch = ((size_t)w * 4 + 255) / 256 * 256;
+  s.width = w; s.height = hh; s.format = -1;
+  if (hipMalloc(&s.ptr, s.pitch * hh) != hipSuccess) return throw_err(env, "hush addon: hipMalloc failed");
+  return make_surface(env, s);
+}
+
+// download8(surface) -> Uint8Array of width*height*4 bytes (synchronises the device)
+napi_value Download8(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!get_args(env, info, 1, a)) return nullptr;
+  hush_surface s;
+  read_surface(env, a[0], &s);
+  size_t n = (size_t)s.width * s.height * 4;
+  napi_value ab, arr;
+  void *data = nullptr;
+  NAPI_CALL(env, napi_create_arraybuffer(env, n, &data, &ab));
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy2D(data, (size_t)s.width * 4, s.ptr, s.pitch, (size_t)s.width * 4, s.height, hipMemcpyDeviceToHost) != hipSuccess)
+    return throw_err(env, "hush addon: hipMemcpy2D download failed");
+  NAPI_CALL(env, napi_create_typedarray(env, napi_uint8_array, n, ab, 0, &arr));
+  return arr;
+}
+
 napi_value DeviceCount(napi_env env, napi_callback_info) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
@@ -551,6 +586,142 @@ napi_value SvPassTimes(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+// ---- display post (include/post.h; the reference's Post class, src/demo/post.js:131-210)
+
+struct PoHolder {
+  post_t *p;
+};
+
+post_t *po_of(napi_env env, napi_value v) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) return nullptr;
+  return static_cast<PoHolder *>(p)->p;
+}
+
+napi_value po_throw(napi_env env, post_t *p, int code) {
+  napi_value err, msg, c;
+  const char *m = post_last_error(p);
+  napi_create_string_utf8(env, m ? m : "post error", NAPI_AUTO_LENGTH, &msg);
+  napi_create_error(env, nullptr, msg, &err);
+  napi_create_int32(env, code, &c);
+  napi_set_named_property(env, err, "code", c);
+  napi_throw(env, err);
+  return nullptr;
+}
+
+void finalize_po(napi_env, void *data, void *) {
+  PoHolder *hd = static_cast<PoHolder *>(data);
+  if (!hd) return;
+  if (hd->p) post_destroy(hd->p);
+  delete hd;
+}
+
+double named_num(napi_env env, napi_value o, const char *k, double dflt) {
+  bool has = false;
+  napi_has_named_property(env, o, k, &has);
+  if (!has) return dflt;
+  napi_value v;
+  napi_get_named_property(env, o, k, &v);
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_boolean) {
+    bool b = false;
+    napi_get_value_bool(env, v, &b);
+    return b ? 1.0 : 0.0;
+  }
+  return num(env, v, dflt);
+}
+
+// postCreate(device, opts {bloom, vignette})
+napi_value PoCreate(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return nullptr;
+  post_options o;
+  post_default_options(&o);
+  napi_valuetype t;
+  napi_typeof(env, a[1], &t);
+  if (t == napi_object) {
+    o.bloom = (float)named_num(env, a[1], "bloom", o.bloom);
+    o.vignette = (float)named_num(env, a[1], "vignette", o.vignette);
+  }
+  post_t *p = nullptr;
+  int r = post_create((int)num(env, a[0]), &o, &p);
+  if (r != HUSH_OK) return po_throw(env, nullptr, r);
+  napi_value ext;
+  PoHolder *hd = new PoHolder{p};
+  if (napi_create_external(env, hd, finalize_po, nullptr, &ext) != napi_ok) {
+    post_destroy(p);
+    delete hd;
+    return throw_err(env, "hush addon: napi_create_external failed");
+  }
+  return ext;
+}
+
+napi_value PoDestroy(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!get_args(env, info, 1, a)) return nullptr;
+  void *p = nullptr;
+  if (napi_get_value_external(env, a[0], &p) == napi_ok && p) {
+    PoHolder *hd = static_cast<PoHolder *>(p);
+    if (hd->p) post_destroy(hd->p);
+    hd->p = nullptr;
+  }
+  return nullptr;
+}
+
+napi_value PoResize(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return nullptr;
+  post_t *p = po_of(env, a[0]);
+  int r = post_resize(p, (int)num(env, a[1]), (int)num(env, a[2]));
+  if (r != HUSH_OK) return po_throw(env, p, r);
+  return nullptr;
+}
+
+// postEncode(p, {left, right, mid, split, split2, same, exposure, line, target, width, height, format}, stream)
+napi_value PoEncode(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return nullptr;
+  post_t *p = po_of(env, a[0]);
+  post_args pa{};
+  struct { const char *k; hush_surface *s; } imgs[3] = {{"left", &pa.left}, {"right", &pa.right}, {"mid", &pa.mid}};
+  for (auto &e : imgs) {
+    napi_value v;
+    bool has = false;
+    napi_has_named_property(env, a[1], e.k, &has);
+    if (has) napi_get_named_property(env, a[1], e.k, &v);
+    else napi_get_undefined(env, &v);
+    if (!read_surface(env, v, e.s)) return throw_err(env, "hush addon: bad surface object");
+  }
+  pa.split = (float)named_num(env, a[1], "split", 0.5);
+  pa.split2 = (float)named_num(env, a[1], "split2", pa.split);
+  pa.same = named_num(env, a[1], "same", 0) != 0.0;
+  pa.exposure = (float)named_num(env, a[1], "exposure", 1.0);
+  pa.line = (float)named_num(env, a[1], "line", 1.5);
+  pa.format = (int32_t)named_num(env, a[1], "format", POST_BGRA8_UNORM);
+  hush_surface tgt;
+  {
+    napi_value v;
+    bool has = false;
+    napi_has_named_property(env, a[1], "target", &has);
+    if (has) napi_get_named_property(env, a[1], "target", &v);
+    else napi_get_undefined(env, &v);
+    if (!read_surface(env, v, &tgt)) return throw_err(env, "hush addon: bad target object");
+  }
+  pa.target = tgt.ptr;
+  pa.target_pitch = tgt.pitch;
+  pa.width = (int32_t)named_num(env, a[1], "width", tgt.width);
+  pa.height = (int32_t)named_num(env, a[1], "height", tgt.height);
+  uint64_t stream = 0;
+  bool lossless = false;
+  napi_valuetype t;
+  napi_typeof(env, a[2], &t);
+  if (t == napi_bigint) napi_get_value_bigint_uint64(env, a[2], &stream, &lossless);
+  int r = post_encode(p, &pa, (void *)(uintptr_t)stream);
+  if (r != HUSH_OK) return po_throw(env, p, r);
+  return nullptr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback fn; } fns[] = {
       {"hiddenOf", HiddenOf}, {"create", Create}, {"destroy", Destroy}, {"resize", Resize}, {"reset", Reset},
@@ -559,6 +730,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"download", Download}, {"deviceCount", DeviceCount},
       {"svgfCreate", SvCreate}, {"svgfDestroy", SvDestroy}, {"svgfResize", SvResize}, {"svgfReset", SvReset},
       {"svgfInfo", SvInfo}, {"svgfEncode", SvEncode}, {"svgfOutput", SvOutput}, {"svgfPassTimes", SvPassTimes},
+      {"alloc8", Alloc8}, {"download8", Download8}, {"postCreate", PoCreate}, {"postDestroy", PoDestroy},
+      {"postResize", PoResize}, {"postEncode", PoEncode},
   };
   for (auto &f : fns) {
     napi_value v;

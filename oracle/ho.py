@@ -261,3 +261,74 @@ class SvgfOracle:
 
     def run_pass(self, p: int) -> None:
         self._pass(self._c, int(p))
+
+
+def post_encode(left, right, mid=None, split=0.5, split2=None, same=False, exposure=1.0, bloom=0.03, vignette=0.35,
+                line=1.5, width=None, height=None, bgra=False):
+    """CPU restatement of Post.encode (src/demo/post.js:179-209).  left / right / mid: (h, w, 4) float16
+    arrays.  Returns (target (height, width, 4) uint8, hdr (h, w, 4) float32, [level 1..6 arrays])."""
+    L = load()
+    f = L.po32_encode
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 2 + [ctypes.c_float] * 2 + [ctypes.c_int] + \
+        [ctypes.c_float] * 4 + [ctypes.c_int] * 3 + [ctypes.c_void_p] * 3
+    h, w = left.shape[:2]
+    width = w if width is None else width
+    height = h if height is None else height
+    split2 = split if split2 is None else split2
+    imgs = [np.ascontiguousarray(a, dtype=np.float16) for a in (left, right)]
+    m = None if mid is None else np.ascontiguousarray(mid, dtype=np.float16)
+    target = np.zeros((height, width, 4), np.uint8)
+    hdr = np.zeros((h, w, 4), np.float32)
+    sizes = [(max(1, h >> i), max(1, w >> i)) for i in range(1, 7)]
+    lv = np.zeros(sum(a * b * 4 for a, b in sizes), np.float32)
+    r = f(imgs[0].ctypes.data, imgs[1].ctypes.data, None if m is None else m.ctypes.data, w, h, split, split2,
+          int(bool(same)), exposure, bloom, vignette, line, width, height, int(bool(bgra)), target.ctypes.data,
+          hdr.ctypes.data, lv.ctypes.data)
+    assert r == 0
+    levels, o = [], 0
+    for a, b in sizes:
+        levels.append(lv[o:o + a * b * 4].reshape(a, b, 4))
+        o += a * b * 4
+    return target, hdr, levels
+
+
+def post_agx(c):
+    """The AgX transform with its look (post.js:99-119) of one linear colour -> display-linear [0, 1]^3."""
+    L = load()
+    f = L.po32_agx
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    i = np.ascontiguousarray(c, dtype=np.float32)
+    o = np.zeros(3, np.float32)
+    f(i.ctypes.data, o.ctypes.data)
+    return o
+
+
+def post_srgb(c):
+    L = load()
+    f = L.po32_srgb
+    f.restype = ctypes.c_float
+    f.argtypes = [ctypes.c_float]
+    return float(f(c))
+
+
+def post_sample(img, u, v):
+    """The linear clamp-to-edge sampler of the post oracle at normalised (u, v) of img (h, w, 4)."""
+    L = load()
+    f = L.po32_sample
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    o = np.zeros(3, np.float32)
+    f(a.ctypes.data, a.shape[1], a.shape[0], u, v, o.ctypes.data)
+    return o
+
+
+def post_pass(which, src, w, h, dst=None):
+    """One post pass alone: 0 Karis downsample, 1 downsample, 2 additive upsample into dst; returns dst."""
+    L = load()
+    f = L.po32_pass
+    f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    a = np.ascontiguousarray(src, dtype=np.float32)
+    d = np.zeros((h, w, 4), np.float32) if dst is None else np.ascontiguousarray(dst, dtype=np.float32).copy()
+    f(which, a.ctypes.data, a.shape[1], a.shape[0], d.ctypes.data, w, h)
+    return d

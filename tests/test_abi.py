@@ -19,7 +19,7 @@ def declared_symbols():
     for name in sorted(os.listdir(inc)):
         if name.endswith(".h"):
             text = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, name)).read(), flags=re.S)
-            syms |= set(re.findall(r"\b((?:hush|svgf)_[a-z_0-9]+)\s*\(", text))
+            syms |= set(re.findall(r"\b((?:hush|svgf|post)_[a-z_0-9]+)\s*\(", text))
     return sorted(syms)
 
 

@@ -127,3 +127,22 @@ def test_node_options_match_oracle_and_python(tmp_path):
             py = hs.denoise(surfs).download()
             assert np.array_equal(py.view(np.uint16), out(c, f).view(np.uint16)), (c, f)
         hs.destroy()
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_post_matches_oracle(tmp_path):
+    from oracle import ho
+    w, h = 120, 72
+    rng = np.random.default_rng(5)
+    left = rng.lognormal(-1.0, 1.0, (h, w, 4)).astype(np.float16)
+    right = rng.lognormal(-0.5, 1.0, (h, w, 4)).astype(np.float16)
+    left.tofile(tmp_path / "left.bin")
+    right.tofile(tmp_path / "right.bin")
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "post_sequence.js"), str(tmp_path), str(w), str(h)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.fromfile(tmp_path / "target.bin", np.uint8).reshape(h, w, 4)
+    ref, _, _ = ho.post_encode(left, right, split=0.4, exposure=1.25, bgra=True)
+    assert np.abs(out.astype(int) - ref.astype(int)).max() <= 1
