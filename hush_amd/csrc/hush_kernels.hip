@@ -211,10 +211,11 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       // 3x3 stats (hush.js:125-142)
       Sum<KAHAN> a1x, a1y, a1z;
       float cnt = 0.0f;
+      // t.w is exactly 1 or 0 and an invalid texel is staged as (0, 0, 0, 0), so t.x * t.w is t.x bit for bit
 #pragma unroll
       for (int j = 0; j < 9; j++) {
         float4 t = sI[c0 + (j / 3 - 1) * SW + (j % 3 - 1)];
-        a1x.add(t.x * t.w); a1y.add(t.y * t.w); a1z.add(t.z * t.w);
+        a1x.add(t.x); a1y.add(t.y); a1z.add(t.z);
         cnt += t.w;
       }
       const float s1x = a1x.get(), s1y = a1y.get(), s1z = a1z.get();
@@ -358,8 +359,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   for (int j = 0; j < 16; j++) {
     int idx = b0 + (j >> 2) * SW + (j & 3);
     float w = sI[idx].w;
-    // w * E * qf16(lum) = w * sQ: w is exactly 1 or 0, and an invalid texel's luminance is 0
-    q1.add(w * sQ[idx]);
+    // w * E * qf16(lum) = sQ: w is exactly 1 or 0, and an invalid texel's sQ is E * qf16(0) = 0
+    q1.add(sQ[idx]);
     n4 += w;
   }
   const float mu4 = F::div(q1.get(), mx(n4, 1.0f));
