@@ -512,20 +512,26 @@ def main():
     if not args.no_extras:
         extras["mlp_stage_1080p"] = mlp_stage(hs, hip, W, H)
         print(f"[bench] MLP stage: {extras['mlp_stage_1080p']}", file=sys.stderr, flush=True)
-        # the whole frame with the FP16-MFMA MLP
-        hm = hush_amd.Hush(0, {"width": W, "height": H, "precision": "fp16-mfma", "ieee_math": args.math == "ieee"})
-        for i in range(args.warmup):
-            hm.encode(dev[i % F], stream=stream.handle)
-        stream.synchronize()
-        e0.record(stream.handle)
-        for i in range(args.steps):
-            if i % F == 0:
-                hm.reset()
-            hm.encode(dev[i % F], stream=stream.handle)
-        e1.record(stream.handle)
-        e1.synchronize()
-        extras["fp16_mfma_ms_per_frame"] = round(e0.elapsed_ms(e1) / args.steps, 5)
-        hm.destroy()
+        # the whole frame with the FP16-MFMA MLP, and with the FAST math policy (the other policy than --math)
+        def frame_ms(opts):
+            hm = hush_amd.Hush(0, dict({"width": W, "height": H}, **opts))
+            for i in range(args.warmup):
+                hm.encode(dev[i % F], stream=stream.handle)
+            stream.synchronize()
+            e0.record(stream.handle)
+            for i in range(args.steps):
+                if i % F == 0:
+                    hm.reset()
+                hm.encode(dev[i % F], stream=stream.handle)
+            e1.record(stream.handle)
+            e1.synchronize()
+            hm.destroy()
+            return round(e0.elapsed_ms(e1) / args.steps, 5)
+
+        extras["fp16_mfma_ms_per_frame"] = frame_ms({"precision": "fp16-mfma", "ieee_math": args.math == "ieee"})
+        other = "fast" if args.math == "ieee" else "ieee"
+        extras[f"{other}_math_ms_per_frame"] = frame_ms({"precision": args.precision, "ieee_math": other == "ieee"})
+        extras[f"{other}_math_fp16_mfma_ms_per_frame"] = frame_ms({"precision": "fp16-mfma", "ieee_math": other == "ieee"})
         extras["svgf_1080p"] = svgf_leg(dev, stream, hip, W, H, min(args.steps, 64), args.warmup)
         extras["post_1080p"] = post_leg(dev, stream, hip, W, H, 100)
         print(f"[bench] post {extras['post_1080p']}", file=sys.stderr, flush=True)
