@@ -58,6 +58,13 @@ __device__ __forceinline__ int2 pixel_of(int W, int H) {
 }
 
 // pow(x, sigmaN) (svgf.js:137, 205); n > 0: x^n by binary exponentiation in FP64
+// x^128 (the default sigmaN) as sv_pow evaluates it: seven FP64 squarings, times 1
+__device__ __forceinline__ float sv_pow128(float x) {
+  double b = (double)x;
+#pragma unroll
+  for (int k = 0; k < 7; k++) b *= b;
+  return (float)(1.0 * b);
+}
 __device__ __forceinline__ float sv_pow(float x, int n, float y) {
   if (n > 0) {
     double b = (double)x, r = 1.0;
@@ -70,6 +77,16 @@ __device__ __forceinline__ float sv_pow(float x, int n, float y) {
     return (float)r;
   }
   return powf(x, y);
+}
+
+// a / b for a b shared by many quotients (rb = sv_rcp(b)): the shared-reciprocal quotient, bit-identical to the
+// IEEE one for finite a and a normal b below 2^126 (the quotient here is normal or zero); an infinite or NaN a
+// takes a * rb, which gives what a / b gives (+-inf, NaN); an infinite or NaN b has rb = 1 / b itself
+__device__ __forceinline__ float sv_rcp(float b) {
+  return __builtin_fabsf(b) < 0x1p126f ? Math<false>::rcpn(b) : 1.0f / b;
+}
+__device__ __forceinline__ float sv_div(float a, float b, float rb) {
+  return (__builtin_fabsf(a) < __builtin_inff() && __builtin_fabsf(b) < 0x1p126f) ? Math<false>::divs(a, b, rb) : a * rb;
 }
 
 // consistent (svgf.js:32-40)
@@ -152,6 +169,7 @@ __global__ __launch_bounds__(NT) void k_sv_temporal(SvTemporalArgs a) {
 
 // VARIANCE (svgf.js:114-148): temporal variance for long histories, a 7x7 depth / normal weighted
 // spatial estimate (scaled by 4 / history length) for histories shorter than 4 frames
+template <bool P128>                 // sigmaN = 128
 __global__ __launch_bounds__(NT) void k_sv_variance(SvVarianceArgs a) {
   const int2 p = pixel_of(a.W, a.H);
   if (p.x >= a.W || p.y >= a.H) return;
@@ -173,8 +191,9 @@ __global__ __launch_bounds__(NT) void k_sv_variance(SvVarianceArgs a) {
       const float4 ndq = ld(a.nd, qx, qy);
       if (ndq.w < 0.0f) continue;
       const float len = sqrtf((float)dx * (float)dx + (float)dy * (float)dy);
-      const float wz = expf(-fabsf(ndq.w - nd.w) / (0.02f * nd.w * len + 1e-3f));
-      const float wn = sv_pow(mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z), a.n_int, a.sigma_n);
+      const float wz = Math<false>::expn(-fabsf(ndq.w - nd.w) / (0.02f * nd.w * len + 1e-3f));
+      const float dn = mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z);
+      const float wn = P128 ? sv_pow128(dn) : sv_pow(dn, a.n_int, a.sigma_n);
       const float w = wz * wn;
       sw += w;
       const float4 cq = ld(a.col, qx, qy), mq = ld(a.mom, qx, qy);
@@ -190,7 +209,7 @@ __global__ __launch_bounds__(NT) void k_sv_variance(SvVarianceArgs a) {
 
 // ATROUS (svgf.js:157-216): one 5x5 B3-spline wavelet iteration with taps `step` texels apart, depth /
 // normal / luminance edge-stopping; iteration 0 also writes the colour feedback (the next frame's history)
-template <bool FIRST>
+template <bool FIRST, bool P128>     // P128: sigmaN = 128
 __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
   const int2 p = pixel_of(a.W, a.H);
   if (p.x >= a.W || p.y >= a.H) return;
@@ -210,7 +229,7 @@ __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
     for (int dx = -1; dx <= 1; dx++)
       gv += k3[dx + 1] * k3[dy + 1] * ld(a.in, clampi(p.x + dx, 0, a.W - 1), clampi(p.y + dy, 0, a.H - 1)).w;
   const float lp = lum(center.x, center.y, center.z);
-  const float denomL = a.sigma_l * sqrtf(mx(gv, 0.0f)) + 1e-6f;
+  const float denomL = a.sigma_l * sqrtf(mx(gv, 0.0f)) + 1e-6f, rL = sv_rcp(denomL);
   // zGrad (svgf.js:157-165)
   const float z = nd.w;
   const float zl = ld(a.nd, clampi(p.x - 1, 0, a.W - 1), p.y).w, zr = ld(a.nd, clampi(p.x + 1, 0, a.W - 1), p.y).w;
@@ -220,6 +239,13 @@ __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
   const float zg = mn(mx(gx, gy), z);
   const float h[5] = {1.0f / 16.0f, 1.0f / 4.0f, 3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
   const float zden = a.sigma_z * zg;
+  // the depth term's divisor zden * dist + 1e-3, one per tap distance (dx^2 + dy^2 = 1, 2, 4, 5, 8)
+  float zdiv[9], zrcp[9];
+#pragma unroll
+  for (int d2 = 1; d2 <= 8; d2++) {
+    zdiv[d2] = zden * (sqrtf((float)d2) * (float)a.step) + 1e-3f;
+    zrcp[d2] = sv_rcp(zdiv[d2]);
+  }
   float sw = h[2] * h[2];
   float s0 = center.x * sw, s1 = center.y * sw, s2 = center.z * sw;
   float sv = center.w * sw * sw;
@@ -236,11 +262,12 @@ __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
       const float4 ndq = ld(a.nd, qx, qy);
       if (ndq.w < 0.0f) continue;
       const float4 cq = ld(a.in, qx, qy);
-      const float dist = sqrtf((float)dx * (float)dx + (float)dy * (float)dy) * (float)step;
-      const float wz = fabsf(ndq.w - z) / (zden * dist + 1e-3f);
-      const float wn = sv_pow(mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z), a.n_int, a.sigma_n);
-      const float wl = fabsf(lum(cq.x, cq.y, cq.z) - lp) / denomL;
-      const float w = h[dx + 2] * h[dy + 2] * wn * expf(-wz - wl);
+      const int d2 = dx * dx + dy * dy;
+      const float wz = sv_div(fabsf(ndq.w - z), zdiv[d2], zrcp[d2]);
+      const float dn = mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z);
+      const float wn = P128 ? sv_pow128(dn) : sv_pow(dn, a.n_int, a.sigma_n);
+      const float wl = sv_div(fabsf(lum(cq.x, cq.y, cq.z) - lp), denomL, rL);
+      const float w = h[dx + 2] * h[dy + 2] * wn * Math<false>::expn(-wz - wl);
       sw += w;
       s0 += w * cq.x; s1 += w * cq.y; s2 += w * cq.z;
       sv += w * w * cq.w;
@@ -502,15 +529,22 @@ static hipError_t run_pass(svgf_t *s, int pass, const svgf_inputs *in, hipStream
     SvVarianceArgs a;
     a.col = t[SV_TMP]; a.mom = t[SV_HM0 + cur]; a.nd = in_tex(in->normal_depth); a.out = t[SV_AT0];
     a.W = W; a.H = H; a.n_int = n_int; a.sigma_n = s->o.sigma_n;
-    hipLaunchKernelGGL(k_sv_variance, g, b, 0, st, a);
+    if (a.n_int == 128) hipLaunchKernelGGL(k_sv_variance<true>, g, b, 0, st, a);
+    else hipLaunchKernelGGL(k_sv_variance<false>, g, b, 0, st, a);
   } else if (pass < 2 + s->o.iterations) {
     const int i = pass - 2, src = i & 1;
     SvAtrousArgs a;
     a.in = t[SV_AT0 + src]; a.nd = in_tex(in->normal_depth); a.out = t[SV_AT0 + (src ^ 1)]; a.fb = t[SV_HC0 + cur];
     a.W = W; a.H = H; a.step = 1 << i; a.n_int = n_int;
     a.sigma_l = s->o.sigma_l; a.sigma_n = s->o.sigma_n; a.sigma_z = s->o.sigma_z;
-    if (i == 0) hipLaunchKernelGGL(k_sv_atrous<true>, g, b, 0, st, a);
-    else hipLaunchKernelGGL(k_sv_atrous<false>, g, b, 0, st, a);
+    const bool p128 = a.n_int == 128;
+    if (i == 0) {
+      if (p128) hipLaunchKernelGGL((k_sv_atrous<true, true>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_sv_atrous<true, false>), g, b, 0, st, a);
+    } else {
+      if (p128) hipLaunchKernelGGL((k_sv_atrous<false, true>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_sv_atrous<false, false>), g, b, 0, st, a);
+    }
   } else if (pass == 2 + s->o.iterations) {
     SvResolveArgs a;
     a.illum = t[SV_AT0 + (s->o.iterations & 1)]; a.mat = in_tex(in->material); a.emi = in_tex(in->emission);
