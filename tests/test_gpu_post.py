@@ -96,3 +96,19 @@ def test_post_errors():
         hush_amd.Post(0, "rgba16float")
     post.encode(a, b)                                    # still usable
     post.destroy()
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (2, 3), (1, 5)])
+def test_post_tiny_images(h, w):
+    """images smaller than the pyramid: every level is 1 x 1 (max(1, w >> i)), the target larger than them"""
+    import hush_amd
+    from oracle import ho
+    imgs = _images(h, w, seed=h * 10 + w)
+    post = hush_amd.Post(0, "rgba8unorm")
+    t, hdr, levels, _ = _run(post, imgs, split=0.5, width=16, height=9)
+    rt, rhdr, rlevels = ho.post_encode(imgs[0], imgs[1], imgs[2], split=0.5, width=16, height=9)
+    assert np.array_equal(hdr.view(np.uint32), rhdr.view(np.uint32))
+    for a, b in zip(levels, rlevels):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.abs(t.astype(int) - rt.astype(int)).max() <= 1
+    post.destroy()
