@@ -120,7 +120,13 @@ template <> struct Math<false> {
   // the residual a - b q0 is exact in an FMA, and one correction RN(q0 + r y) is the correctly rounded
   // quotient (Markstein's theorem for a correctly rounded reciprocal).  For a = +-0 the correction adds two
   // zeros and can lose the quotient's sign, which q0 always carries: hence the copysign.  Bit-identical to
-  // a / b for finite a and finite non-zero b with a normal or zero quotient (4 VALU instructions, not 11)
+  // a / b for a = +-0 or 2^-103 <= |a| < inf, 2^-126 <= |b| < 2^126 and a normal or zero quotient (4 VALU
+  // instructions, not 11).  The residual's exactness sets the lower bound: b q0 is a multiple of
+  // ulp(b) ulp(q0), about 2^-46 |a|, so below |a| = 2^-103 the residual can fall under the 2^-149 grid and
+  // the correction can miss by an ulp.  At the kernels' call sites the numerators are zero or far above
+  // 2^-103 (sums and products of binary16 texels, which are multiples of 2^-24, and of their quotients by
+  // bounded factors), or variance sums whose quotient is clamped at 1e-12 or goes into a binary16
+  // store, where an ulp of a value below 2^-103 does not survive.
   static __device__ __forceinline__ float divs(float a, float b, float y) {
     const float q0 = a * y;
     return __builtin_copysignf(__builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0), q0);

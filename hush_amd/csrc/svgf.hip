@@ -79,9 +79,13 @@ __device__ __forceinline__ float sv_pow(float x, int n, float y) {
   return powf(x, y);
 }
 
-// a / b for a b shared by many quotients (rb = sv_rcp(b)): the shared-reciprocal quotient, bit-identical to the
-// IEEE one for finite a and a normal b below 2^126 (the quotient here is normal or zero); an infinite or NaN a
-// takes a * rb, which gives what a / b gives (+-inf, NaN); an infinite or NaN b has rb = 1 / b itself
+// a / b for a b shared by many quotients (rb = sv_rcp(b)): the shared-reciprocal quotient (Math<false>::divs),
+// bit-identical to the IEEE one for a = 0 or 2^-103 <= |a| < inf, 2^-126 <= |b| < 2^126 and a normal or zero
+// quotient; an infinite or NaN a takes a * rb, which gives what a / b gives (+-inf, NaN); an infinite or NaN b
+// has rb = 1 / b itself.  The filters' numerators are differences of binary16 depths (0 or >= 2^-24) and of
+// luminances of binary16 colours (0 or >= 2^-51); the variance pass's divisors lie in [1e-3, 5600] and the
+// a-trous divisors in [1e-6, 2^82] while sigmaL and sigmaZ lie in [0, 2^60] (the host launches the IEEE
+// quotient otherwise), so every finite quotient is normal or zero.
 __device__ __forceinline__ float sv_rcp(float b) {
   return __builtin_fabsf(b) < 0x1p126f ? Math<false>::rcpn(b) : 1.0f / b;
 }
@@ -180,7 +184,16 @@ __global__ __launch_bounds__(NT) void k_sv_variance(SvVarianceArgs a) {
     st(a.out, p.x, p.y, make_float4(c.x, c.y, c.z, mx(0.0f, m.y - m.x * m.x)));
     return;
   }
+  // the depth term's divisor 0.02 z len + 1e-3, one per squared tap distance len^2 = dx^2 + dy^2 (0 .. 18)
+  const float zs = 0.02f * nd.w;
+  float zdiv[19], zrcp[19];
+#pragma unroll
+  for (int d2 = 0; d2 <= 18; d2++) {
+    zdiv[d2] = zs * sqrtf((float)d2) + 1e-3f;
+    zrcp[d2] = sv_rcp(zdiv[d2]);
+  }
   float sw = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sm0 = 0.0f, sm1 = 0.0f;
+#pragma unroll
   for (int dy = -3; dy <= 3; dy++) {
     const int qy = p.y + dy;
     if (qy < 0 || qy >= a.H) continue;
@@ -190,8 +203,8 @@ __global__ __launch_bounds__(NT) void k_sv_variance(SvVarianceArgs a) {
       if (qx < 0 || qx >= a.W) continue;
       const float4 ndq = ld(a.nd, qx, qy);
       if (ndq.w < 0.0f) continue;
-      const float len = sqrtf((float)dx * (float)dx + (float)dy * (float)dy);
-      const float wz = Math<false>::expn(-fabsf(ndq.w - nd.w) / (0.02f * nd.w * len + 1e-3f));
+      const int d2 = dx * dx + dy * dy;
+      const float wz = Math<false>::expn(sv_div(-fabsf(ndq.w - nd.w), zdiv[d2], zrcp[d2]));
       const float dn = mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z);
       const float wn = P128 ? sv_pow128(dn) : sv_pow(dn, a.n_int, a.sigma_n);
       const float w = wz * wn;
@@ -209,8 +222,10 @@ __global__ __launch_bounds__(NT) void k_sv_variance(SvVarianceArgs a) {
 
 // ATROUS (svgf.js:157-216): one 5x5 B3-spline wavelet iteration with taps `step` texels apart, depth /
 // normal / luminance edge-stopping; iteration 0 also writes the colour feedback (the next frame's history)
-template <bool FIRST, bool P128>     // P128: sigmaN = 128
+// SHARED: the edge-stopping quotients as shared-reciprocal ones (sigmaL and sigmaZ in [0, 2^60]), else IEEE
+template <bool FIRST, bool P128, bool SHARED>     // P128: sigmaN = 128
 __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
+  auto quot = [](float x, float d, float rd) { return SHARED ? sv_div(x, d, rd) : x / d; };
   const int2 p = pixel_of(a.W, a.H);
   if (p.x >= a.W || p.y >= a.H) return;
   const float4 nd = ld(a.nd, p.x, p.y);
@@ -229,7 +244,7 @@ __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
     for (int dx = -1; dx <= 1; dx++)
       gv += k3[dx + 1] * k3[dy + 1] * ld(a.in, clampi(p.x + dx, 0, a.W - 1), clampi(p.y + dy, 0, a.H - 1)).w;
   const float lp = lum(center.x, center.y, center.z);
-  const float denomL = a.sigma_l * sqrtf(mx(gv, 0.0f)) + 1e-6f, rL = sv_rcp(denomL);
+  const float denomL = a.sigma_l * sqrtf(mx(gv, 0.0f)) + 1e-6f, rL = SHARED ? sv_rcp(denomL) : 0.0f;
   // zGrad (svgf.js:157-165)
   const float z = nd.w;
   const float zl = ld(a.nd, clampi(p.x - 1, 0, a.W - 1), p.y).w, zr = ld(a.nd, clampi(p.x + 1, 0, a.W - 1), p.y).w;
@@ -244,7 +259,7 @@ __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
 #pragma unroll
   for (int d2 = 1; d2 <= 8; d2++) {
     zdiv[d2] = zden * (sqrtf((float)d2) * (float)a.step) + 1e-3f;
-    zrcp[d2] = sv_rcp(zdiv[d2]);
+    zrcp[d2] = SHARED ? sv_rcp(zdiv[d2]) : 0.0f;
   }
   float sw = h[2] * h[2];
   float s0 = center.x * sw, s1 = center.y * sw, s2 = center.z * sw;
@@ -263,10 +278,10 @@ __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
       if (ndq.w < 0.0f) continue;
       const float4 cq = ld(a.in, qx, qy);
       const int d2 = dx * dx + dy * dy;
-      const float wz = sv_div(fabsf(ndq.w - z), zdiv[d2], zrcp[d2]);
+      const float wz = quot(fabsf(ndq.w - z), zdiv[d2], zrcp[d2]);
       const float dn = mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z);
       const float wn = P128 ? sv_pow128(dn) : sv_pow(dn, a.n_int, a.sigma_n);
-      const float wl = sv_div(fabsf(lum(cq.x, cq.y, cq.z) - lp), denomL, rL);
+      const float wl = quot(fabsf(lum(cq.x, cq.y, cq.z) - lp), denomL, rL);
       const float w = h[dx + 2] * h[dy + 2] * wn * Math<false>::expn(-wz - wl);
       sw += w;
       s0 += w * cq.x; s1 += w * cq.y; s2 += w * cq.z;
@@ -513,6 +528,17 @@ static int check_inputs(svgf_t *s, const svgf_inputs *in) {
 }
 
 // one pass at the current frame / reset state (svgf.js:359-374)
+template <bool FIRST>
+static void launch_atrous(bool p128, bool shared, dim3 g, dim3 b, hipStream_t st, const SvAtrousArgs &a) {
+  if (p128) {
+    if (shared) hipLaunchKernelGGL((k_sv_atrous<FIRST, true, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_sv_atrous<FIRST, true, false>), g, b, 0, st, a);
+  } else {
+    if (shared) hipLaunchKernelGGL((k_sv_atrous<FIRST, false, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_sv_atrous<FIRST, false, false>), g, b, 0, st, a);
+  }
+}
+
 static hipError_t run_pass(svgf_t *s, int pass, const svgf_inputs *in, hipStream_t st) {
   const int cur = (int)(s->frame & 1), prev = cur ^ 1, W = s->W, H = s->H;
   const Tex *t = s->tex;
@@ -538,13 +564,10 @@ static hipError_t run_pass(svgf_t *s, int pass, const svgf_inputs *in, hipStream
     a.W = W; a.H = H; a.step = 1 << i; a.n_int = n_int;
     a.sigma_l = s->o.sigma_l; a.sigma_n = s->o.sigma_n; a.sigma_z = s->o.sigma_z;
     const bool p128 = a.n_int == 128;
-    if (i == 0) {
-      if (p128) hipLaunchKernelGGL((k_sv_atrous<true, true>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_sv_atrous<true, false>), g, b, 0, st, a);
-    } else {
-      if (p128) hipLaunchKernelGGL((k_sv_atrous<false, true>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_sv_atrous<false, false>), g, b, 0, st, a);
-    }
+    // the shared-reciprocal quotients' domain holds for sigmaL, sigmaZ in [0, 2^60] (NaN fails the test)
+    const bool shared = a.sigma_l >= 0.0f && a.sigma_l <= 0x1p60f && a.sigma_z >= 0.0f && a.sigma_z <= 0x1p60f;
+    if (i == 0) launch_atrous<true>(p128, shared, g, b, st, a);
+    else launch_atrous<false>(p128, shared, g, b, st, a);
   } else if (pass == 2 + s->o.iterations) {
     SvResolveArgs a;
     a.illum = t[SV_AT0 + (s->o.iterations & 1)]; a.mat = in_tex(in->material); a.emi = in_tex(in->emission);

@@ -81,6 +81,8 @@ def teacher_forced(kind, w, h, n, seed, **opts):
     ("boxes", 96, 64, 4, 11, {}),
     ("spheres", 77, 53, 3, 12, {"iterations": 3, "sigmaN": 64.5, "taa": False}),
     ("train", 128, 72, 3, 13, {"sigmaL": 2.0, "alphaColor": 0.1}),
+    # sigmaL, sigmaZ above 2^60: the a-trous passes take IEEE quotients in place of shared-reciprocal ones
+    ("boxes", 64, 48, 3, 14, {"sigmaL": 3e18, "sigmaZ": 3e18}),
 ])
 def test_teacher_forced_passes(kind, w, h, n, seed, opts):
     stats = teacher_forced(kind, w, h, n, seed, **opts)
