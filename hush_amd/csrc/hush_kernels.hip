@@ -1424,14 +1424,23 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
 
 // Karis tone map and its inverse, IEEE under both policies: itm(tm(x)) lands on x exactly for most texels
 // (the reset frame), where any negative error would drop a round-toward-zero store by a full binary16 ulp.
+// c / d for the three channels and the d they share: the shared-reciprocal quotients where their domain holds,
+// else the IEEE quotients (a branch that ordinary texels never take).  tm's colours are finite (binary16 texels, whose stores clamp to +-65504,
+// or their bilinear mix) and its divisor 1 + lum(c) is a multiple of 2^-99 no larger than 65505, so only
+// d = 0 (negative channels with luminance exactly -1, which tm maps to infinities) leaves the domain.  itm's
+// divisor is at least 1e-4 but can be +inf, and its colour can hold those infinities.
 __device__ __forceinline__ float3 tm(float3 c) {
   using M = Math<false>;
   const float d = 1.0f + lum(c.x, c.y, c.z), r = 1.0f / d;
+  if (__builtin_expect(d == 0.0f, 0)) return make_float3(c.x / d, c.y / d, c.z / d);
   return make_float3(M::divs(c.x, d, r), M::divs(c.y, d, r), M::divs(c.z, d, r));
 }
 __device__ __forceinline__ float3 itm(float3 c) {
   using M = Math<false>;
   const float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f), r = 1.0f / d;
+  if (__builtin_expect(!(__builtin_isfinite(d) && __builtin_isfinite(c.x) && __builtin_isfinite(c.y) &&
+                         __builtin_isfinite(c.z)), 0))
+    return make_float3(c.x / d, c.y / d, c.z / d);
   return make_float3(M::divs(c.x, d, r), M::divs(c.y, d, r), M::divs(c.z, d, r));
 }
 __device__ __forceinline__ float3 to_ycocg(float3 c) {

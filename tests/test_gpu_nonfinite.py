@@ -1,7 +1,8 @@
 """GPU regression for motion vectors that overflowed binary16 (+-inf) or are NaN (tests/nonfinite.py):
 every kernel that reprojects (Hush temporal and TAA, SVGF temporal and resolve, the differentiable frame)
 keeps its index arithmetic defined and in bounds, and matches the oracles as on ordinary frames.
-Tolerances as in test_gpu_parity.py / test_gpu_svgf.py."""
+Tolerances as in test_gpu_parity.py / test_gpu_svgf.py.  Also the TAA tone map's zero divisor: a colour
+whose luminance is exactly -1 tone-maps to infinities, which the stores clamp like the oracle's."""
 import numpy as np
 import pytest
 
@@ -86,4 +87,43 @@ def test_train_frame_with_nonfinite_motion():
         fr.commit(out)
         prev = (out, tgt)
     fr.destroy()
+    hs.destroy()
+
+
+def test_taa_tone_map_zero_divisor():
+    """(-7.75390625, 1.5419921875, -6.29296875) has fp32 luminance exactly -1, so tm() divides by 0: the
+    reset frame's TAA output is itm(tm(c)) = (-inf, +inf, -inf) / 1e-4, stored as -65504, 65504, -65504
+    (a NaN in place of an infinity would store -65504 in every channel)."""
+    import hush_amd
+    from hush_amd.scene import SceneGenerator
+    w, h = 64, 32
+    gen = SceneGenerator("boxes", w, h, 5)
+    hs = hush_amd.Hush(0, {"width": w, "height": h})
+    o = Oracle(w, h, hush_amd.default_weights())
+    g = gen.render(0)
+    surfs = upload_frame(g)
+    o.set_inputs(g)
+    for p in range(7):
+        hs.set_state(0, True)
+        o.set_state(0, True)
+        force_oracle(hs, o)
+        hs.run_pass(p, surfs)
+        o.run_pass(p)
+    x = hs.surface("X").download()
+    spots = [(3, 5), (10, 40), (31, 63)]
+    for (yy, xx) in spots:
+        x[yy, xx, :3] = np.array([-7.75390625, 1.5419921875, -6.29296875], np.float16)
+    hs.surface("X").upload(x)
+    hs.set_state(0, True)
+    o.set_state(0, True)
+    force_oracle(hs, o)
+    hs.run_pass(7, surfs)
+    o.run_pass(7)
+    gpu, ref = gpu_surface(hs, "taa0"), o.surface("taa0")
+    for (yy, xx) in spots:
+        assert list(ref[yy, xx, :3]) == [-65504.0, 65504.0, -65504.0], ref[yy, xx]
+    st = compare(gpu, ref)
+    assert np.array_equal(gpu[..., :3], ref[..., :3]) or (st["max_ulp"] <= 1.0 and st["exact"] >= 0.999), st
+    for (yy, xx) in spots:
+        assert np.array_equal(gpu[yy, xx, :3], ref[yy, xx, :3]), (gpu[yy, xx], ref[yy, xx])
     hs.destroy()
