@@ -118,16 +118,25 @@ template <> struct Math<false> {
   static __device__ __forceinline__ float div(float a, float b) { return a / b; }
   // a / b for a denominator shared by several quotients, y = RN(1/b): q0 = RN(a y) is within an ulp of a / b,
   // the residual a - b q0 is exact in an FMA, and one correction RN(q0 + r y) is the correctly rounded
-  // quotient (Markstein's theorem for a correctly rounded reciprocal).  For a = +-0 the correction adds two
-  // zeros and can lose the quotient's sign, which q0 always carries: hence the copysign.  Bit-identical to
-  // a / b for a = +-0 or 2^-103 <= |a| < inf, 2^-126 <= |b| < 2^126 and a normal or zero quotient (4 VALU
-  // instructions, not 11).  The residual's exactness sets the lower bound: b q0 is a multiple of
-  // ulp(b) ulp(q0), about 2^-46 |a|, so below |a| = 2^-103 the residual can fall under the 2^-149 grid and
-  // the correction can miss by an ulp.  At the kernels' call sites the numerators are zero or far above
-  // 2^-103 (sums and products of binary16 texels, which are multiples of 2^-24, and of their quotients by
-  // bounded factors), or variance sums whose quotient is clamped at 1e-12 or goes into a binary16
-  // store, where an ulp of a value below 2^-103 does not survive.
+  // quotient (Markstein's theorem for a correctly rounded reciprocal).  v_div_fixup_f32, the last step of the
+  // compiler's own IEEE division, then gives the result IEEE's sign (a zero quotient's sign can be lost when
+  // the correction adds two zeros) and IEEE's special cases: a or b zero, infinite or NaN (there the
+  // correction itself is NaN).  Bit-identical to a / b, in 4 VALU instructions instead of 11, for
+  // a = +-0, +-inf, NaN or 2^-102 <= |a| < inf and b = +-0, +-inf, NaN or 2^-126 <= |b| < 2^126, with
+  // 2^-126 <= |a / b| < 2^127 when both are finite and non-zero.  The residual's exactness sets the lower bound on a: b q0 is a
+  // multiple of ulp(b) ulp(q0) >= 2^-47 |a|, so below |a| = 2^-102 the residual can fall under the
+  // 2^-149 grid and the correction can miss by an ulp.  At the kernels' call sites the numerators are zero
+  // or far above 2^-102 (sums and products of binary16 texels, which are multiples of 2^-24, and of their
+  // quotients by bounded factors), or variance sums whose quotient is clamped at 1e-12 or goes into a
+  // binary16 store, where an ulp of a value below 2^-102 does not survive.
   static __device__ __forceinline__ float divs(float a, float b, float y) {
+    const float q0 = a * y;
+    return __builtin_amdgcn_div_fixupf(__builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0), b, a);
+  }
+  // divs for finite operands (b non-zero): the copysign from q0 in place of the fixup gives the same bits
+  // there, for a cheaper instruction.  The call sites whose operands come from clamped binary16 surfaces
+  // (history, guides, filter sums) use it; those fed by caller inputs (radiance, normals, motion) use divs.
+  static __device__ __forceinline__ float divsf(float a, float b, float y) {
     const float q0 = a * y;
     return __builtin_copysignf(__builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0), q0);
   }
@@ -189,6 +198,7 @@ template <> struct Math<true> {
   static __device__ __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float divs(float a, float, float y) { return a * y; }
+  static __device__ __forceinline__ float divsf(float a, float, float y) { return a * y; }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float sqrtn(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float rcpn(float b) { return __builtin_amdgcn_rcpf(b); }
@@ -233,7 +243,7 @@ __device__ __forceinline__ float3 oct_dec(float ex, float ey) {
   // the squared norm is at least 1/3 for every finite (ex, ey)
   float l = mx(Math<false>::sqrtn(vx * vx + vy * vy + vz * vz), 1e-20f);
   const float r = Math<false>::rcpn(l);            // l in [0.57, 2.3e5] for stored binary16 inputs
-  return make_float3(Math<false>::divs(vx, l, r), Math<false>::divs(vy, l, r), Math<false>::divs(vz, l, r));
+  return make_float3(Math<false>::divsf(vx, l, r), Math<false>::divsf(vy, l, r), Math<false>::divsf(vz, l, r));
 }
 
 // octDec with the policy's arithmetic (continuous uses only; validity tests use oct_dec)

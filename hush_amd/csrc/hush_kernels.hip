@@ -133,8 +133,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       }
       if (sumW >= 0.01f) {
         const float rs = F::rcpn(sumW);
-        hcx = F::divs(hcx, sumW, rs); hcy = F::divs(hcy, sumW, rs); hcz = F::divs(hcz, sumW, rs);
-        hm.x = F::divs(hm.x, sumW, rs); hm.y = F::divs(hm.y, sumW, rs); hm.z = F::divs(hm.z, sumW, rs); hm.w = F::divs(hm.w, sumW, rs);
+        hcx = F::divsf(hcx, sumW, rs); hcy = F::divsf(hcy, sumW, rs); hcz = F::divsf(hcz, sumW, rs);
+        hm.x = F::divsf(hm.x, sumW, rs); hm.y = F::divsf(hm.y, sumW, rs); hm.z = F::divsf(hm.z, sumW, rs); hm.w = F::divsf(hm.w, sumW, rs);
         v = sumW; has = true;
       } else {
         float n3 = 0.0f;
@@ -151,8 +151,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         }
         if (n3 > 0.0f) {
           const float rn = F::rcpn(n3);
-          hcx = F::divs(hcx, n3, rn); hcy = F::divs(hcy, n3, rn); hcz = F::divs(hcz, n3, rn);
-          hm.x = F::divs(hm.x, n3, rn); hm.y = F::divs(hm.y, n3, rn); hm.z = F::divs(hm.z, n3, rn); hm.w = F::divs(hm.w, n3, rn);
+          hcx = F::divsf(hcx, n3, rn); hcy = F::divsf(hcy, n3, rn); hcz = F::divsf(hcz, n3, rn);
+          hm.x = F::divsf(hm.x, n3, rn); hm.y = F::divsf(hm.y, n3, rn); hm.z = F::divsf(hm.z, n3, rn); hm.w = F::divsf(hm.w, n3, rn);
           v = 0.1f; has = true;
           mbits |= 0x10;
         }
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   float var4 = F::div(q4.get(), mx(n4, 1.0f));
   const float ln = mx(F::sqrt(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
   const float rln = F::rcpn(ln);
-  const float3 Nn = make_float3(F::divs(Nx, ln, rln), F::divs(Ny, ln, rln), F::divs(Nz, ln, rln));
+  const float3 Nn = make_float3(F::divsf(Nx, ln, rln), F::divsf(Ny, ln, rln), F::divsf(Nz, ln, rln));
   float zmax = -1e30f, zmin = 1e30f, mind = 1e30f;
 #pragma unroll
   for (int j = 0; j < 4; j++)
@@ -461,7 +461,7 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
       }
     }
   const float rsw = F::rcpn(sw);
-  return make_float4(F::divs(scx, sw, rsw), F::divs(scy, sw, rsw), F::divs(scz, sw, rsw), F::div(sv, sw * sw));
+  return make_float4(F::divsf(scx, sw, rsw), F::divsf(scy, sw, rsw), F::divsf(scz, sw, rsw), F::div(sv, sw * sw));
 }
 
 // decoded half-res guide from an F3 texel: octDec(oct normal), block depth (hush.js:337)
@@ -1186,10 +1186,10 @@ __device__ __forceinline__ void down(const TexU &F2, const TexF &GH, int QX, int
   using F = Math<false>;   // stored level values: IEEE under both policies (they set the slope sl)
   float ln = mx(F::sqrt(nx * nx + ny * ny + nz * nz), 1e-6f);
   const float rln = F::rcpn(ln), rN = F::rcpn(N);
-  float2 o = oct_enc_m<FAST>(make_float3(F::divs(nx, ln, rln), F::divs(ny, ln, rln), F::divs(nz, ln, rln)));
-  col = make_float4(qf16(F::divs(clx, N, rN)), qf16(F::divs(cly, N, rN)), qf16(F::divs(clz, N, rN)),
-                    qf16(F::divs(F::divs(vr, N, rN), N, rN)));
-  g = make_float4(qf16(o.x), qf16(o.y), qf16(F::divs(z, N, rN)), 0.0f);
+  float2 o = oct_enc_m<FAST>(make_float3(F::divsf(nx, ln, rln), F::divsf(ny, ln, rln), F::divsf(nz, ln, rln)));
+  col = make_float4(qf16(F::divsf(clx, N, rN)), qf16(F::divsf(cly, N, rN)), qf16(F::divsf(clz, N, rN)),
+                    qf16(F::divsf(F::divsf(vr, N, rN), N, rN)));
+  g = make_float4(qf16(o.x), qf16(o.y), qf16(F::divsf(z, N, rN)), 0.0f);
 }
 
 constexpr int coarse1_lds() { return 2 * (TW + 4) * (TH + 4); }
@@ -1283,7 +1283,7 @@ __device__ __forceinline__ float jbw(float frx, float fry, int j, float3 nq, flo
   float dn = mx(dot3(nq, n), 0.0f);
   float dn2 = dn * dn;
   float dn4 = dn2 * dn2;
-  return wb * Math<FAST>::expn(Math<FAST>::divs(-fabsf(zq - z), jd, rjd)) * dn4 * dn4;
+  return wb * Math<FAST>::expn(Math<FAST>::divsf(-fabsf(zq - z), jd, rjd)) * dn4 * dn4;
 }
 
 // The upsampling taps of a 32x8 tile fall in an 18x6 window of the half-res level (origin x0/2 - 1,
@@ -1384,9 +1384,9 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   float ix = fx, iy = fy, iz = fz;
   if (swh > 1e-4f) {
     const float rh = F::rcpn(swh);
-    ix = F::divs(uBx, swh, rh) + pp.w * (fx - F::divs(uAx, swh, rh));
-    iy = F::divs(uBy, swh, rh) + pp.w * (fy - F::divs(uAy, swh, rh));
-    iz = F::divs(uBz, swh, rh) + pp.w * (fz - F::divs(uAz, swh, rh));
+    ix = F::divsf(uBx, swh, rh) + pp.w * (fx - F::divsf(uAx, swh, rh));
+    iy = F::divsf(uBy, swh, rh) + pp.w * (fy - F::divsf(uAy, swh, rh));
+    iz = F::divsf(uBz, swh, rh) + pp.w * (fz - F::divsf(uAz, swh, rh));
   }
   // quarter-res coarse level (hush.js:604-619)
   const float cpx = ((float)px + 0.5f) / 4.0f - 0.5f, cpy = ((float)py + 0.5f) / 4.0f - 0.5f;
@@ -1407,7 +1407,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   float upx = ix, upy = iy, upz = iz;
   if (swq > 1e-4f) {
     const float rq = F::rcpn(swq);
-    upx = F::divs(uCx, swq, rq); upy = F::divs(uCy, swq, rq); upz = F::divs(uCz, swq, rq);
+    upx = F::divsf(uCx, swq, rq); upy = F::divsf(uCy, swq, rq); upz = F::divsf(uCz, swq, rq);
   }
   ix = ix + (upx - ix) * pp.z;
   iy = iy + (upy - iy) * pp.z;
@@ -1424,23 +1424,17 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
 
 // Karis tone map and its inverse, IEEE under both policies: itm(tm(x)) lands on x exactly for most texels
 // (the reset frame), where any negative error would drop a round-toward-zero store by a full binary16 ulp.
-// c / d for the three channels and the d they share: the shared-reciprocal quotients where their domain holds,
-// else the IEEE quotients (a branch that ordinary texels never take).  tm's colours are finite (binary16 texels, whose stores clamp to +-65504,
-// or their bilinear mix) and its divisor 1 + lum(c) is a multiple of 2^-99 no larger than 65505, so only
-// d = 0 (negative channels with luminance exactly -1, which tm maps to infinities) leaves the domain.  itm's
-// divisor is at least 1e-4 but can be +inf, and its colour can hold those infinities.
+// tm's divisor 1 + lum(c) is 0 for a colour with negative channels and luminance exactly -1; the quotient
+// then follows IEEE (+-inf, or NaN for a zero channel), as the shared-reciprocal quotient does for zero and
+// infinite operands.
 __device__ __forceinline__ float3 tm(float3 c) {
   using M = Math<false>;
   const float d = 1.0f + lum(c.x, c.y, c.z), r = 1.0f / d;
-  if (__builtin_expect(d == 0.0f, 0)) return make_float3(c.x / d, c.y / d, c.z / d);
   return make_float3(M::divs(c.x, d, r), M::divs(c.y, d, r), M::divs(c.z, d, r));
 }
 __device__ __forceinline__ float3 itm(float3 c) {
   using M = Math<false>;
   const float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f), r = 1.0f / d;
-  if (__builtin_expect(!(__builtin_isfinite(d) && __builtin_isfinite(c.x) && __builtin_isfinite(c.y) &&
-                         __builtin_isfinite(c.z)), 0))
-    return make_float3(c.x / d, c.y / d, c.z / d);
   return make_float3(M::divs(c.x, d, r), M::divs(c.y, d, r), M::divs(c.z, d, r));
 }
 __device__ __forceinline__ float3 to_ycocg(float3 c) {
@@ -1698,9 +1692,32 @@ __global__ __launch_bounds__(NT) void k_math_selftest(int which, uint32_t base, 
     const uint32_t e = (u >> 23) & 0xffu;
     if (!((e >= 1u && e <= 252u) || x != x)) return;          // normal, |x| < 2^126, or NaN
     r = Math<false>::rcpn(x); ref = 1.0f / x;
-  } else {
+  } else if (which == 4) {
     if ((u & 0x7fffffffu) >= 1u && (u & 0x7fffffffu) < 0x00800000u) return;   // every input but a subnormal
     r = Math<false>::logn(x); ref = logf(x);
+  } else {
+    // divs(x, d, RN(1/d)) against x / d for every numerator of its domain over divisors that cover its
+    // domain: +-0, +-inf, NaN, the extremes 2^-126 and 2^126 - ulp, and significands with hard patterns at
+    // several exponents
+    const uint32_t ax = u & 0x7fffffffu;
+    if (ax != 0u && ax < 0x0c800000u) return;                 // 0 < |x| < 2^-102
+    const uint32_t dv[] = {0x00000000u, 0x80000000u, 0x7f800000u, 0xff800000u, 0x7fc00000u, 0x00800000u,
+                           0x7e7fffffu, 0x3f800000u, 0xbf800000u, 0x3fffffffu, 0x3f800001u, 0x40400000u,
+                           0x41100000u, 0x3a83126fu, 0x3c23d70au, 0x477fe000u, 0xc0e00001u, 0x2b7fffffu,
+                           0x53000001u, 0x1d7ffffeu, 0x60abcdefu, 0x9f123457u, 0x3ec00000u, 0xbf3504f3u};
+    unsigned nbad = 0;
+    for (int k = 0; k < (int)(sizeof(dv) / sizeof(dv[0])); k++) {
+      const float d = __uint_as_float(dv[k]), y = 1.0f / d;
+      const float q = Math<false>::divs(x, d, y), qr = x / d;
+      // finite non-zero operands need a normal quotient below 2^127 (underflow and overflow are outside)
+      const bool fin = x == x && d == d && fabsf(x) < __builtin_inff() && fabsf(d) < __builtin_inff();
+      if (fin && x != 0.0f && d != 0.0f && !(fabsf(qr) >= 0x1p-126f && fabsf(qr) < 0x1p127f)) continue;
+      if (__float_as_uint(q) != __float_as_uint(qr) && !(q != q && qr != qr)) nbad++;
+    }
+    if (nbad == 0) return;
+    atomicAdd(bad, (unsigned long long)nbad);
+    atomicMin(first, u);
+    return;
   }
   if (__float_as_uint(r) == __float_as_uint(ref) || (r != r && ref != ref)) return;
   atomicAdd(bad, 1ull);

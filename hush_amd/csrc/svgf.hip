@@ -80,18 +80,17 @@ __device__ __forceinline__ float sv_pow(float x, int n, float y) {
 }
 
 // a / b for a b shared by many quotients (rb = sv_rcp(b)): the shared-reciprocal quotient (Math<false>::divs),
-// bit-identical to the IEEE one for a = 0 or 2^-103 <= |a| < inf, 2^-126 <= |b| < 2^126 and a normal or zero
-// quotient; an infinite or NaN a takes a * rb, which gives what a / b gives (+-inf, NaN); an infinite or NaN b
-// has rb = 1 / b itself.  The filters' numerators are differences of binary16 depths (0 or >= 2^-24) and of
-// luminances of binary16 colours (0 or >= 2^-51); the variance pass's divisors lie in [1e-3, 5600] and the
-// a-trous divisors in [1e-6, 2^82] while sigmaL and sigmaZ lie in [0, 2^60] (the host launches the IEEE
-// quotient otherwise), so every finite quotient is normal or zero.
+// bit-identical to the IEEE one for a = 0, +-inf, NaN or 2^-102 <= |a| < inf, b = 0, +-inf, NaN or
+// 2^-126 <= |b| < 2^126, and a normal quotient when both are finite and non-zero; sv_rcp takes the IEEE
+// reciprocal outside the reduced-range reciprocal's domain.  The filters' finite numerators are differences
+// of binary16 depths (0 or >= 2^-24) and of luminances of binary16 colours (0 or >= 2^-51); the variance
+// pass's finite divisors lie in [1e-3, 5600] and the a-trous divisors in [1e-6, 2^82] while sigmaL and
+// sigmaZ lie in [0, 2^60] (the host launches the IEEE quotient otherwise), so every such quotient is normal
+// or zero.
 __device__ __forceinline__ float sv_rcp(float b) {
   return __builtin_fabsf(b) < 0x1p126f ? Math<false>::rcpn(b) : 1.0f / b;
 }
-__device__ __forceinline__ float sv_div(float a, float b, float rb) {
-  return (__builtin_fabsf(a) < __builtin_inff() && __builtin_fabsf(b) < 0x1p126f) ? Math<false>::divs(a, b, rb) : a * rb;
-}
+__device__ __forceinline__ float sv_div(float a, float b, float rb) { return Math<false>::divs(a, b, rb); }
 
 // consistent (svgf.js:32-40)
 __device__ __forceinline__ bool consistent(const SvTemporalArgs &a, int qx, int qy, float3 n, float zexp, float obj) {

@@ -165,7 +165,9 @@ int32_t hush_set_debug_mask(hush_t *h, int32_t enabled);
    where their arguments allow it, against the toolchain's own sqrtf / expf / IEEE division / logf evaluated
    on the current device, over every binary32 input of the domain: which = 0 square root over x >= 2^-96,
    +0, +inf and NaN; 1 exponential over x <= 0 and NaN; 2 exponential over -103.27893 <= x <= 88.72283;
-   3 reciprocal over normal x with |x| < 2^126 and NaN; 4 logarithm over every x but a subnormal.
+   3 reciprocal over normal x with |x| < 2^126 and NaN; 4 logarithm over every x but a subnormal; 5 the
+   shared-reciprocal division x / d for every x of its domain (0, +-inf, NaN, |x| >= 2^-102) over 24 divisors
+   (+-0, +-inf, NaN and normal ones; finite pairs with a normal quotient below 2^127), counted per pair.
    *mismatches = number of inputs whose results differ in any bit (any NaN equals any NaN), *first = the
    smallest such input's bits (0xffffffff when there is none).  Synchronous; allocates 16 bytes of device
    memory for the counters. */

@@ -80,9 +80,11 @@ static real h2r(uint16_t h) {
 }
 
 /* value -> binary16 with round toward zero, clamped to +-65504 (the rgba16float storage write of
-   hush.js:98, also pack2x16float(qf16(a), qf16(b)) of hush.js:101). */
+   hush.js:98, also pack2x16float(qf16(a), qf16(b)) of hush.js:101).  NaN stores as -65504: the clamp
+   min(max(x, -65504), 65504) with IEEE maxNum / minNum, which return the non-NaN operand (WGSL leaves NaN
+   at run time indeterminate; this is what its clamp gives on hardware with IEEE min / max). */
 static uint16_t r2h_rtz(real x) {
-  if (x != x) return 0x7e00;
+  if (x != x) return 0xfbff;
   uint16_t s = 0;
   double a = (double)x;
   if (a < 0 || (a == 0 && signbit(a))) { s = 0x8000; a = -a; }
@@ -107,9 +109,11 @@ static real st(real x) {
 #endif
 }
 
-/* qf16 (hush.js:68,99): clamp to +-65504, then drop the low 13 mantissa bits of the f32 (toward zero). */
+/* qf16 (hush.js:68,99): clamp to +-65504, then drop the low 13 mantissa bits of the f32 (toward zero);
+   NaN clamps to -65504 as in r2h_rtz. */
 static real qf16(real x) {
 #if HO_QUANT
+  if (x != x) return -65504.0f;
 #ifndef HO_DOUBLE
   float c = x < -65504.0f ? -65504.0f : (x > 65504.0f ? 65504.0f : x);
   uint32_t u;

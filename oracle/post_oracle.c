@@ -32,9 +32,9 @@ static float h2f(uint16_t h) {
   return (float)(s ? -v : v);
 }
 
-/* rgba16float store: round toward zero, clamped to +-65504 */
+/* rgba16float store: round toward zero, clamped to +-65504 (NaN to -65504, the clamp's IEEE min / max) */
 static float st(float x) {
-  if (x != x) return x;
+  if (x != x) return -65504.0f;
   double a = fabs((double)x), r;
   if (a >= 65504.0) r = 65504.0;
   else if (a < ldexp(1.0, -14)) r = floor(a * 16777216.0) / 16777216.0;

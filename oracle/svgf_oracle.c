@@ -54,10 +54,11 @@ static real h2r(uint16_t h) {
   return (real)(s ? -v : v);
 }
 
-/* rgba16float store: round toward zero, clamped to +-65504 (the rounding hush.js:98 states) */
+/* rgba16float store: round toward zero, clamped to +-65504 (the rounding hush.js:98 states; NaN to -65504,
+   the clamp with IEEE min / max) */
 static real st(real x) {
 #if HO_QUANT
-  if (x != x) return x;
+  if (x != x) return -65504.0f;
   double a = fabs((double)x), r;
   if (a >= 65504.0) r = 65504.0;
   else if (a < ldexp(1.0, -14)) r = floor(a * 16777216.0) / 16777216.0;

@@ -1,13 +1,16 @@
 /* The shared-reciprocal division of the kernels (Math<false>::divs in hush_amd/csrc/hush_device.h): with
    y = RN(1/b), q0 = RN(a y) and one FMA correction q = RN(q0 + RN(a - b q0) y) (the residual a - b q0 is
-   exact in an FMA), the result must be the correctly rounded a / b (Markstein's one-step correction); the
-   sign of a zero quotient is restored from q0.  Checks, against IEEE division (bit for bit, so -0 != +0):
+   exact in an FMA), the result must be the correctly rounded a / b (Markstein's one-step correction).  The
+   kernels end with v_div_fixup_f32, which for finite non-zero b and finite a returns the correction with the
+   sign of a b; here that is copysign(q, q0), q0 carrying the sign of a b (a zero quotient's sign can be lost
+   by the correction's addition of two zeros).  The fixup's special cases (zero, infinite or NaN operands) are
+   checked on the GPU.  Checks, against IEEE division (bit for bit, so -0 != +0):
      * random operands over wide exponent ranges (argv[1] pairs);
      * every quotient of small integers, and +-0 over divisors of both signs;
      * every numerator significand (2^23) at three exponents for divisors with the hard significands (all
        ones, all ones minus 1 and 2, one ulp above 1, exactly 1, 1.5, 1.5 minus one ulp) at three exponents,
        and for argv[2] random divisors (one numerator exponent each);
-     * numerators of 2^-103 .. 2^-80 over divisors of every normal exponent (argv[1] pairs), normal quotients.
+     * numerators of 2^-102 .. 2^-80 over divisors of every normal exponent (argv[1] pairs), normal quotients.
    Prints the mismatch count. */
 #include <math.h>
 #include <stdint.h>
@@ -57,11 +60,11 @@ int main(int argc, char **argv) {
       for (int ae = -1; ae <= 1; ae++) bad += sweep(fb(((uint32_t)(be + 127) << 23) | hard[s]), ae);
   for (long k = 0; k < nb; k++)
     bad += sweep(fb(((uint32_t)((int)(rnd() % 40) - 20 + 127) << 23) | (uint32_t)(rnd() & 0x7fffff)), (int)(rnd() % 5) - 2);
-  /* the low end of the domain: numerators down to 2^-103 (below it the residual a - b q0 can fall under
+  /* the low end of the domain: numerators down to 2^-102 (below it the residual a - b q0 can fall under
      the subnormal grid and the correction can miss by an ulp) over divisors of any normal exponent, every
      normal quotient counted */
   for (long i = 0; i < n; i++) {
-    const float a = rf(-103, -80), b = rf(-126, 125), r = a / b;
+    const float a = rf(-102, -80), b = rf(-126, 125), r = a / b;
     if (!isnormal(r)) continue;
     const float q = divs(a, b, 1.0f / b);
     bad += memcmp(&q, &r, 4) != 0;

@@ -25,3 +25,16 @@ def inject(g: dict, frame: int) -> dict:
 def nonfinite_mask(g: dict) -> np.ndarray:
     mv = g["motion"].view(np.float16)[..., :2].astype(np.float32)
     return ~np.isfinite(mv).all(-1)
+
+
+def inject_radiance(g: dict, frame: int) -> dict:
+    """copy of the frame with binary16 +inf radiance (a firefly that overflowed the renderer's binary16
+    store): a few isolated texels, one 3x3 block, and one texel with +inf in a single channel"""
+    g = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in g.items()}
+    rad = g["radiance"]
+    h, w = rad.shape[:2]
+    for k in range(6):
+        rad[(7 * k + 3 * frame) % h, (11 * k + 5 * frame) % w, 0:3] = PINF
+    rad[h // 2:h // 2 + 3, w // 3:w // 3 + 3, 0:3] = PINF
+    rad[h // 3, 2 * w // 3, 1] = PINF
+    return g

@@ -2,7 +2,9 @@
 of a * RN(1/b)) is the correctly rounded quotient: the same IEEE operations in C (hardware FMA) against IEEE
 division on 10^7 random operand pairs over wide exponent ranges, every quotient of small integers, and every
 numerator significand for the divisors with the hard significands (all ones and its neighbours, 1, 1.5) and
-16 random divisors (about 700 M quotients)."""
+16 random divisors (about 700 M quotients), and numerators from 2^-102 (the bound below which the residual can
+leave the subnormal grid) over divisors of every normal exponent.  The special operands are checked on the
+GPU (tests/test_gpu_math.py)."""
 import os
 import shutil
 import subprocess

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle.ho import Oracle, SvgfOracle
-from tests.nonfinite import inject
+from tests.nonfinite import inject, inject_radiance
 from tests.parity import compare, force_oracle, gpu_F, gpu_surface, pass_outputs, upload_frame
 
 pytestmark = pytest.mark.gpu
@@ -38,6 +38,33 @@ def test_hush_passes_with_nonfinite_motion():
                     bad.append((f, p, name, st))
             if p == 0:
                 assert np.array_equal(hs.surface("mask").download()[..., 0], o.mask()), f
+    hs.destroy()
+    assert not bad, bad[:4]
+
+
+def test_hush_passes_with_infinite_radiance():
+    """binary16 +inf radiance (overflowed fireflies) through every pass, teacher forced, against the oracle"""
+    import hush_amd
+    from hush_amd.scene import SceneGenerator
+    w, h = 96, 64
+    gen = SceneGenerator("boxes", w, h, 19)
+    hs = hush_amd.Hush(0, {"width": w, "height": h})
+    o = Oracle(w, h, hush_amd.default_weights())
+    surfs, bad = None, []
+    for f in range(3):
+        g = inject_radiance(gen.render(f), f)
+        surfs = upload_frame(g, surfs)
+        o.set_inputs(g)
+        for p in range(8):
+            hs.set_state(f, f == 0)
+            o.set_state(f, f == 0)
+            force_oracle(hs, o)
+            hs.run_pass(p, surfs)
+            o.run_pass(p)
+            for name in pass_outputs(p, f & 1):
+                st = compare(gpu_F(hs) if name == "F" else gpu_surface(hs, name), o.surface(name))
+                if st["max_ulp"] > 1.0 or st["exact"] < 0.999:
+                    bad.append((f, p, name, st))
     hs.destroy()
     assert not bad, bad[:4]
 
