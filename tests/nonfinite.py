@@ -27,14 +27,20 @@ def nonfinite_mask(g: dict) -> np.ndarray:
     return ~np.isfinite(mv).all(-1)
 
 
-def inject_radiance(g: dict, frame: int) -> dict:
+def inject_radiance(g: dict, frame: int, kind: str = "inf") -> dict:
     """copy of the frame with binary16 +inf radiance (a firefly that overflowed the renderer's binary16
-    store): a few isolated texels, one 3x3 block, and one texel with +inf in a single channel"""
+    store): a few isolated texels, one 3x3 block, and one texel with +inf in a single channel.
+    kind "nan": NaN radiance at those texels instead, plus +inf and zero albedo at a few others."""
     g = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in g.items()}
     rad = g["radiance"]
     h, w = rad.shape[:2]
+    bad = PINF if kind == "inf" else QNAN
     for k in range(6):
-        rad[(7 * k + 3 * frame) % h, (11 * k + 5 * frame) % w, 0:3] = PINF
-    rad[h // 2:h // 2 + 3, w // 3:w // 3 + 3, 0:3] = PINF
-    rad[h // 3, 2 * w // 3, 1] = PINF
+        rad[(7 * k + 3 * frame) % h, (11 * k + 5 * frame) % w, 0:3] = bad
+    rad[h // 2:h // 2 + 3, w // 3:w // 3 + 3, 0:3] = bad
+    rad[h // 3, 2 * w // 3, 1] = bad
+    if kind == "nan":
+        mat = g["material"]
+        mat[h // 4, w // 5:w // 5 + 4, 0:3] = PINF
+        mat[3 * h // 4, w // 2:w // 2 + 4, 0:3] = 0
     return g

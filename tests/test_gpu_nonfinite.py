@@ -1,8 +1,10 @@
 """GPU regression for motion vectors that overflowed binary16 (+-inf) or are NaN (tests/nonfinite.py):
 every kernel that reprojects (Hush temporal and TAA, SVGF temporal and resolve, the differentiable frame)
 keeps its index arithmetic defined and in bounds, and matches the oracles as on ordinary frames.
-Tolerances as in test_gpu_parity.py / test_gpu_svgf.py.  Also the TAA tone map's zero divisor: a colour
-whose luminance is exactly -1 tone-maps to infinities, which the stores clamp like the oracle's."""
+Tolerances as in test_gpu_parity.py / test_gpu_svgf.py.  Also non-finite radiance (+inf from an overflowed
+firefly; NaN, with +inf and zero albedo) through every Hush and SVGF pass, teacher forced (stores clamp NaN to
+-65504 on both sides), and the TAA tone map's zero divisor: a colour whose luminance is exactly -1 tone-maps
+to infinities, which the stores clamp like the oracle's."""
 import numpy as np
 import pytest
 
@@ -42,8 +44,10 @@ def test_hush_passes_with_nonfinite_motion():
     assert not bad, bad[:4]
 
 
-def test_hush_passes_with_infinite_radiance():
-    """binary16 +inf radiance (overflowed fireflies) through every pass, teacher forced, against the oracle"""
+@pytest.mark.parametrize("kind", ["inf", "nan"])
+def test_hush_passes_with_infinite_radiance(kind):
+    """binary16 +inf radiance (overflowed fireflies), or NaN radiance with +inf and zero albedo, through every
+    pass, teacher forced, against the oracle"""
     import hush_amd
     from hush_amd.scene import SceneGenerator
     w, h = 96, 64
@@ -52,7 +56,7 @@ def test_hush_passes_with_infinite_radiance():
     o = Oracle(w, h, hush_amd.default_weights())
     surfs, bad = None, []
     for f in range(3):
-        g = inject_radiance(gen.render(f), f)
+        g = inject_radiance(gen.render(f), f, kind)
         surfs = upload_frame(g, surfs)
         o.set_inputs(g)
         for p in range(8):
@@ -90,6 +94,45 @@ def test_svgf_with_nonfinite_motion():
         st = compare(out, o.encode(g2))
         assert st["frac_rel_gt_1e-3"] <= 1e-3 and st["max_rel"] <= 1e-2, (f, st)
     sv.destroy()
+
+
+@pytest.mark.parametrize("kind", ["inf", "nan"])
+def test_svgf_passes_with_nonfinite_radiance(kind):
+    """+inf radiance, or NaN radiance with +inf and zero albedo, through every SVGF pass, teacher forced (the
+    single-ulp flips of ordinary frames compound through the history next to such texels, so each kernel is
+    checked on identical inputs)"""
+    import hush_amd
+    from hush_amd.hip import Surface
+    from hush_amd.scene import SceneGenerator
+    from tests.test_gpu_svgf import NAMES, outputs_of
+    w, h = 96, 64
+    gen = SceneGenerator("boxes", w, h, 10)
+    sv = hush_amd.SVGF(0, w, h)
+    o = SvgfOracle(w, h)
+    iters = sv.iterations
+    keys = ("radiance", "normalDepth", "motion", "material", "emission")
+    prev, bad = None, []
+    for f in range(3):
+        g = inject_radiance(gen.render(f), f, kind)
+        src = prev if prev is not None else g
+        g2 = dict(g, prevNormalDepth=src["normalDepth"], prevMotion=src["motion"])
+        prev = g
+        surfs = {k: Surface(w, h).upload(np.ascontiguousarray(g2[k]).view(np.float16))
+                 for k in keys + ("prevNormalDepth", "prevMotion")}
+        o.set_inputs(g2)
+        for p in range(3 + iters):
+            sv.set_state(f, f == 0)
+            o.set_state(f, f == 0)
+            for name in NAMES:
+                o.surface(name)[...] = sv.surface(name).download().astype(np.float32)
+            sv.run_pass(p, surfs)
+            o.run_pass(p)
+            for name in outputs_of(p, f & 1, iters):
+                st = compare(sv.surface(name).download().astype(np.float32), o.surface(name))
+                if st["max_ulp"] > 1.0 or st["exact"] < 0.999:
+                    bad.append((f, p, name, st))
+    sv.destroy()
+    assert not bad, bad[:4]
 
 
 def test_train_frame_with_nonfinite_motion():
