@@ -294,6 +294,10 @@ class Hush:
     def set_debug_mask(self, on: bool) -> None:
         self._check(self._L.hush_set_debug_mask(self._h, int(bool(on))))
 
+    def set_wide_gathers(self, on: bool) -> None:
+        """debug: the temporal pass's history gathers on 64-bit addresses (their form for surfaces of 4 GiB)"""
+        self._check(self._L.hush_debug_wide_gathers(self._h, int(bool(on))))
+
 
 from .svgf import SVGF, SVGFError, SVGFTimer, SVGF_INPUTS  # noqa: E402
 from .post import Post, PostError  # noqa: E402

@@ -18,7 +18,7 @@ EXPORTS = [
     "hush_set_ieee_math", "hush_get_ieee_math", "hush_set_compensated", "hush_get_compensated", "hush_mlp_forward",
     "hush_denoise", "hush_encode", "hush_output", "hush_last_error", "hush_set_timing", "hush_pass_times",
     "hush_debug_surface", "hush_debug_set_state", "hush_debug_run_pass", "hush_set_debug_mask",
-    "hush_debug_math_selftest",
+    "hush_debug_wide_gathers", "hush_debug_math_selftest",
     # include/svgf.h
     "svgf_default_options", "svgf_create", "svgf_destroy", "svgf_resize", "svgf_reset", "svgf_get_size", "svgf_frame",
     "svgf_get_options", "svgf_encode", "svgf_output", "svgf_last_error", "svgf_set_timing", "svgf_pass_times",
@@ -116,6 +116,7 @@ def load() -> ctypes.CDLL:
     L.hush_debug_set_state.argtypes = [P, ctypes.c_int64, i32]; L.hush_debug_set_state.restype = i32
     L.hush_debug_run_pass.argtypes = [P, i32, I, P]; L.hush_debug_run_pass.restype = i32
     L.hush_set_debug_mask.argtypes = [P, i32]; L.hush_set_debug_mask.restype = i32
+    L.hush_debug_wide_gathers.argtypes = [P, i32]; L.hush_debug_wide_gathers.restype = i32
     L.hush_debug_math_selftest.argtypes = [i32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32)]
     L.hush_debug_math_selftest.restype = i32
     O = ctypes.POINTER(svgf_options)

@@ -297,6 +297,7 @@ static TemporalArgs temporal_args(const hush_t *h, const hush_inputs *in) {
   a.outA = t[S_A0]; a.outMom = t[S_MOM0 + cur]; a.outKey = t[S_KEY0 + cur]; a.outKD = h->kd[cur];
   a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.F3 = h->F[3]; a.outGH = h->gh;
   a.mask = h->mask_on ? h->mask : nullptr; a.mask_pitch = h->mask_pitch; a.aux = h->train_aux; a.compensated = h->compensated ? 1 : 0;
+  a.wide = h->wide_gathers ? 1 : 0;
   a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.reset = h->reset ? 1 : 0; a.E = h->exposure;
   return a;
 }
@@ -495,6 +496,12 @@ extern "C" int32_t hush_debug_run_pass(hush_t *h, int32_t pass, const hush_input
 extern "C" int32_t hush_set_debug_mask(hush_t *h, int32_t enabled) {
   if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_debug_mask: null context");
   h->mask_on = enabled != 0;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_debug_wide_gathers(hush_t *h, int32_t enabled) {
+  if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_debug_wide_gathers: null context");
+  h->wide_gathers = enabled != 0;
   return HUSH_OK;
 }
 

@@ -32,6 +32,7 @@ struct hush_ctx {
   uint8_t *mask = nullptr;
   int mask_pitch = 0;
   bool mask_on = false;
+  bool wide_gathers = false;        // debug: temporal's history gathers on 64-bit addresses at any size
   float *train_aux = nullptr;      // set by the differentiable frame around its temporal pass
   bool ieee = true;                 // IEEE math policy (the oracle's operation-exact mirror); false = FAST
   bool compensated = false;         // compensated (Neumaier) moment sums in the temporal statistics

@@ -32,6 +32,7 @@ struct TemporalArgs {
   int mask_pitch;
   float *aux;                        // optional temporal-blend inputs for the differentiable frame (W x H x TEMPORAL_AUX)
   int compensated;                   // Neumaier-compensated moment sums
+  int wide;                          // history gathers on 64-bit addresses even below 4 GiB (debug)
   int W, H, W2, H2, reset;
   float E;                           // exposure
 };

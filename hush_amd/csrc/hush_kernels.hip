@@ -54,7 +54,14 @@ template <bool KAHAN> struct Sum {
   __device__ __forceinline__ float get() const { return KAHAN ? s + c : s; }
 };
 
-template <bool AUX, bool KAHAN>
+// BUF: the history gathers through buffer descriptors with 32-bit byte offsets (every history surface is
+// under 4 GiB, checked by the host), in place of 64-bit address arithmetic per tap.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t hist_rsrc(const void *p, int pitch, int rows, int texel) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
+                                           (int)((uint32_t)pitch * (uint32_t)rows * (uint32_t)texel), 0x00020000);
+}
+
+template <bool AUX, bool KAHAN, bool BUF>
 __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   constexpr bool FAST = false;
   using F = Math<FAST>;
@@ -94,15 +101,36 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     fx = ppx - bx; fy = ppy - by;
     bix = f2i(bx); biy = f2i(by);
     cx = f2i(floorf(ppx + 0.5f)); cy = f2i(floorf(ppy + 0.5f));
+    if constexpr (BUF) {
+      const __amdgpu_buffer_rsrc_t rk = hist_rsrc(a.pkd.p, a.pkd.pitch, a.H, 16);
+      const __amdgpu_buffer_rsrc_t rc = hist_rsrc(a.pcol.p, a.pcol.pitch, a.H, 8);
+      const __amdgpu_buffer_rsrc_t rm = hist_rsrc(a.pmom.p, a.pmom.pitch, a.H, 8);
+      const __amdgpu_buffer_rsrc_t rp = hist_rsrc(a.ppar0.p, a.ppar0.pitch, a.H2, 8);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const size_t qx = clampi(addw(bix, i & 1), 0, a.W - 1), qy = clampi(addw(biy, i >> 1), 0, a.H - 1);
-      k4[i] = a.pkd.p[qy * a.pkd.pitch + qx];
-      c4r[i] = a.pcol.p[qy * a.pcol.pitch + qx];
-      m4r[i] = a.pmom.p[qy * a.pmom.pitch + qx];
+      for (int i = 0; i < 4; i++) {
+        // qy < 16384 and pitch < 2^23 fit the 24-bit multiply; the product is below 2^28 texels
+        const uint32_t qx = clampi(addw(bix, i & 1), 0, a.W - 1), qy = clampi(addw(biy, i >> 1), 0, a.H - 1);
+        const auto kv = __builtin_amdgcn_raw_buffer_load_b128(rk, (int)((__umul24(qy, a.pkd.pitch) + qx) * 16u), 0, 0);
+        k4[i] = make_float4(__uint_as_float(kv[0]), __uint_as_float(kv[1]), __uint_as_float(kv[2]), __uint_as_float(kv[3]));
+        const auto cv = __builtin_amdgcn_raw_buffer_load_b64(rc, (int)((__umul24(qy, a.pcol.pitch) + qx) * 8u), 0, 0);
+        c4r[i] = make_uint2(cv[0], cv[1]);
+        const auto mv2 = __builtin_amdgcn_raw_buffer_load_b64(rm, (int)((__umul24(qy, a.pmom.pitch) + qx) * 8u), 0, 0);
+        m4r[i] = make_uint2(mv2[0], mv2[1]);
+      }
+      const uint32_t hx = clampi(cx, 0, a.W - 1) >> 1, hy = clampi(cy, 0, a.H - 1) >> 1;
+      const auto pv = __builtin_amdgcn_raw_buffer_load_b64(rp, (int)((__umul24(hy, a.ppar0.pitch) + hx) * 8u), 0, 0);
+      pqr = make_uint2(pv[0], pv[1]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const size_t qx = clampi(addw(bix, i & 1), 0, a.W - 1), qy = clampi(addw(biy, i >> 1), 0, a.H - 1);
+        k4[i] = a.pkd.p[qy * a.pkd.pitch + qx];
+        c4r[i] = a.pcol.p[qy * a.pcol.pitch + qx];
+        m4r[i] = a.pmom.p[qy * a.pmom.pitch + qx];
+      }
+      const size_t hx = clampi(cx, 0, a.W - 1) >> 1, hy = clampi(cy, 0, a.H - 1) >> 1;
+      pqr = a.ppar0.p[hy * a.ppar0.pitch + hx];
     }
-    const size_t hx = clampi(cx, 0, a.W - 1) >> 1, hy = clampi(cy, 0, a.H - 1) >> 1;
-    pqr = a.ppar0.p[hy * a.ppar0.pitch + hx];
   }
 
   // reprojection (hush.js:144-187) ahead of the staging: it needs no LDS, and consuming the gathered taps
@@ -1549,14 +1577,23 @@ static dim3 grid_for(int w, int h) { return dim3(((w + TW - 1) / TW) * ((h + TH 
 #define K_FINE(F) k_fine_final<F>
 #define K_TAA(F) k_taa<F>
 
-hipError_t launch_temporal(const TemporalArgs &a, bool, hipStream_t s) {
+template <bool BUF>
+static void launch_temporal_t(const TemporalArgs &a, hipStream_t s) {
   if (a.compensated) {
-    if (a.aux) hipLaunchKernelGGL((k_temporal<true, true>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL((k_temporal<false, true>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
+    if (a.aux) hipLaunchKernelGGL((k_temporal<true, true, BUF>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_temporal<false, true, BUF>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
   } else {
-    if (a.aux) hipLaunchKernelGGL((k_temporal<true, false>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL((k_temporal<false, false>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
+    if (a.aux) hipLaunchKernelGGL((k_temporal<true, false, BUF>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_temporal<false, false, BUF>), grid_for(a.W, a.H), dim3(NT), 0, s, a);
   }
+}
+hipError_t launch_temporal(const TemporalArgs &a, bool, hipStream_t s) {
+  // buffer descriptors address at most 4 GiB with 32-bit offsets: every history surface must fit
+  const uint64_t lim = (uint64_t)1 << 32;
+  const bool buf = !a.wide && (uint64_t)a.pkd.pitch * a.H * 16 < lim && (uint64_t)a.pcol.pitch * a.H * 8 < lim &&
+                   (uint64_t)a.pmom.pitch * a.H * 8 < lim && (uint64_t)a.ppar0.pitch * a.H2 * 8 < lim;
+  if (buf) launch_temporal_t<true>(a, s);
+  else launch_temporal_t<false>(a, s);
   return hipGetLastError();
 }
 

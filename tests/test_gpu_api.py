@@ -92,6 +92,27 @@ def test_reset_equals_fresh_first_frame():
     assert np.array_equal(out_a.view(np.uint16), out_b.view(np.uint16))
 
 
+@pytest.mark.parametrize("w,h", [(96, 64), (333, 197)])
+def test_wide_history_gathers_give_the_same_bits(w, h):
+    """the temporal pass gathers its history through 32-bit buffer offsets below 4 GiB per surface and through
+    64-bit addresses above; forced onto the 64-bit form, a sequence gives the same bits in every output"""
+    gen = _scene(w, h, 41)
+    frames = [gen.render(f) for f in range(5)]
+    a = hush_amd.Hush(0, {"width": w, "height": h})
+    b = hush_amd.Hush(0, {"width": w, "height": h})
+    b.set_wide_gathers(True)
+    surfs = None
+    for g in frames:
+        surfs = upload_frame(g, surfs)
+        out_a = a.denoise(surfs).download()
+        out_b = b.denoise(surfs).download()
+        assert np.array_equal(out_a.view(np.uint16), out_b.view(np.uint16))
+        for name in ("A0", "X", "F0", "F3", "key0", "key1"):
+            assert np.array_equal(a.surface(name).download(), b.surface(name).download()), name
+    a.destroy()
+    b.destroy()
+
+
 def test_exposure_change_matches_oracle():
     hs, o = _pair(96, 64, exposure=1.0)
     gen = _scene(96, 64, 34, "spheres")
