@@ -341,9 +341,24 @@ def cpu_baseline(frames_g, w, h, n_frames):
     for f in range(n_frames):
         o.denoise(frames_g[f])
     dt = time.perf_counter() - t
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = o.threads()
     return {"value": round(dt / n_frames * 1e3, 3), "unit": "ms/frame", "cores": threads, "kind": "port",
-            "sample": f"{n_frames} frames of the same 1920x1080 boxes sequence, FP32 mirror oracle (C, OpenMP)"}
+            "cpu_model": cpu_model(),
+            "sample": f"{n_frames} frames of the same 1920x1080 boxes sequence, FP32 mirror oracle (C, OpenMP, "
+                      f"{threads} threads)"}
+
+
+def cpu_model():
+    """the host CPU's model name (/proc/cpuinfo), or the platform's processor string"""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
 
 
 def c0_leg(hip, stream, reps=16):
@@ -443,12 +458,21 @@ def main():
     hs = hush_amd.Hush(0, {"width": W, "height": H, "precision": args.precision, "ieee_math": args.math == "ieee"})
     stream = hip.Stream()
 
-    def run(n, start=0):
-        for i in range(n):
-            f = (start + i) % F
-            if f == 0:
-                hs.reset()
-            hs.encode(dev[f], stream=stream.handle)
+    # one playback position across every loop below: the sequence plays on from where the previous loop
+    # stopped and restarts (reset) only at its frame 0, so a run of K frames holds the sequence's own share of
+    # reset frames (one in F) and the short timed run that follows the warm-up holds none
+    pos = [0]
+
+    def next_frame(h):
+        f = pos[0] % F
+        pos[0] += 1
+        if f == 0:
+            h.reset()
+        return f
+
+    def run(n):
+        for _ in range(n):
+            hs.encode(dev[next_frame(hs)], stream=stream.handle)
 
     run(args.warmup)
     stream.synchronize()
@@ -472,10 +496,7 @@ def main():
     acc = np.zeros(8)
     n_t = min(args.steps, 64)
     for i in range(n_t):
-        f = i % F
-        if f == 0:
-            hs.reset()
-        hs.encode(dev[f], timer=timer, stream=stream.handle)
+        hs.encode(dev[next_frame(hs)], timer=timer, stream=stream.handle)
         acc += np.array([ms for _, ms in timer.read()])
     pass_ms = acc / n_t
     names = ["temporal", "infer+mid1", "mid2", "mid3", "coarse1", "coarse2", "fine+final", "taa"]
@@ -498,10 +519,7 @@ def main():
     stream.synchronize()
     evs[0].record(stream.handle)
     for i in range(n_m):
-        f = i % F
-        if f == 0:
-            hs.reset()
-        hs.encode(dev[f], stream=stream.handle)
+        hs.encode(dev[next_frame(hs)], stream=stream.handle)
         evs[i + 1].record(stream.handle)
     evs[-1].synchronize()
     per_frame = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(n_m)])
@@ -515,14 +533,13 @@ def main():
         # the whole frame with the FP16-MFMA MLP, and with the FAST math policy (the other policy than --math)
         def frame_ms(opts):
             hm = hush_amd.Hush(0, dict({"width": W, "height": H}, **opts))
+            pos[0] = 0
             for i in range(args.warmup):
-                hm.encode(dev[i % F], stream=stream.handle)
+                hm.encode(dev[next_frame(hm)], stream=stream.handle)
             stream.synchronize()
             e0.record(stream.handle)
             for i in range(args.steps):
-                if i % F == 0:
-                    hm.reset()
-                hm.encode(dev[i % F], stream=stream.handle)
+                hm.encode(dev[next_frame(hm)], stream=stream.handle)
             e1.record(stream.handle)
             e1.synchronize()
             hm.destroy()

@@ -1588,8 +1588,9 @@ static void launch_temporal_t(const TemporalArgs &a, hipStream_t s) {
   }
 }
 hipError_t launch_temporal(const TemporalArgs &a, bool, hipStream_t s) {
-  // buffer descriptors address at most 4 GiB with 32-bit offsets: every history surface must fit
-  const uint64_t lim = (uint64_t)1 << 32;
+  // buffer descriptors take 32-bit offsets and record counts, passed as int: every history surface must stay
+  // below 2 GiB so that both stay non-negative (larger surfaces take the 64-bit address form)
+  const uint64_t lim = (uint64_t)1 << 31;
   const bool buf = !a.wide && (uint64_t)a.pkd.pitch * a.H * 16 < lim && (uint64_t)a.pcol.pitch * a.H * 8 < lim &&
                    (uint64_t)a.pmom.pitch * a.H * 8 < lim && (uint64_t)a.ppar0.pitch * a.H2 * 8 < lim;
   if (buf) launch_temporal_t<true>(a, s);

@@ -1145,6 +1145,8 @@ extern "C" int32_t hush_train_loss(hush_train_t *t, const void *out, const void 
                                    const void *prev_target, double lambda, void *grad_out, double *loss_dev, void *stream) {
   if (!t || !out || !target || !grad_out || !loss_dev) return tfail(t, HUSH_ERR_ARGUMENT, "hush_train_loss: null argument");
   if ((prev_out == nullptr) != (prev_target == nullptr)) return tfail(t, HUSH_ERR_ARGUMENT, "hush_train_loss: give both prev_out and prev_target, or neither");
+  int r = check_size(t);
+  if (r != HUSH_OK) return r;
   hipError_t e = hipSetDevice(t->h->device);
   if (e == hipSuccess)
     e = t->fp64 ? loss_t<double>(t, (const double *)out, (const double *)target, (const double *)prev_out, (const double *)prev_target,
@@ -1235,6 +1237,10 @@ static hipError_t commit_t(hush_train_t *t, const T *out, hipStream_t s) {
 extern "C" int32_t hush_train_commit(hush_train_t *t, const void *out, void *stream) {
   if (!t || !out) return tfail(t, HUSH_ERR_ARGUMENT, "hush_train_commit: null argument");
   if (!t->forwarded) return tfail(t, HUSH_ERR_ARGUMENT, "hush_train_commit: no forward to commit");
+  // a resize between forward and commit leaves the denoiser's surfaces at another size (and pitch), and its
+  // pending reset must stand: refuse before any launch
+  int r = check_size(t);
+  if (r != HUSH_OK) return r;
   hipError_t e = hipSetDevice(t->h->device);
   if (e == hipSuccess)
     e = t->fp64 ? commit_t<double>(t, (const double *)out, (hipStream_t)stream) : commit_t<float>(t, (const float *)out, (hipStream_t)stream);

@@ -65,7 +65,7 @@ function initDevice(ordinal = 0) {
   const device = {
     ordinal,
     limits: { maxStorageTexturesPerShaderStage: 8 },
-    features: new Set(['shader-f16']),
+    features: new Set(['shader-f16', 'timestamp-query']),
     createTexture(width, height) { return new DeviceTexture(device, width, height); },
     // an 8-bit colour target for Post (the canvas texture of the reference's demo)
     createCanvasTexture(width, height, format) { return new CanvasTexture(device, width, height, format); },
@@ -123,8 +123,10 @@ class Hush {
   }
 
   // One frame (hush.js:813-822). inputs = { radiance, normalDepth, motion, material, emission }: DeviceTextures
-  // of the denoiser's size (extra fields are ignored). timer (optional): an object whose `record(passes)` receives
-  // [{label, ms}] for the 8 passes of this frame, in the reference's label set.
+  // of the denoiser's size (extra fields are ignored). timer (optional, index.d.ts:31-34): { desc(label) } times
+  // every pass, { span: true, spanBegin, spanEnd } the whole denoiser as one span, as with the reference's
+  // GpuTimer (this module's GpuTimer receives the times); an object with `record(passes)` receives
+  // [{label, ms}] for the 8 passes of this frame.
   denoise(inputs, timer, stream) {
     for (const k of INPUTS) {
       const t = inputs[k];
@@ -140,9 +142,97 @@ class Hush {
   encode(inputs, timer, stream) {
     const surf = {};
     for (const k of INPUTS) surf[k] = inputs[k] && (inputs[k].surface || inputs[k]);
-    const out = native.denoise(this._h, surf, stream || 0n, true, !!timer);
-    if (timer && typeof timer.record === 'function') timer.record(native.passTimes(this._h));
+    // the timer's pass descriptors, asked for in the reference's order and with its labels (hush.js:846-849)
+    const descs = timer ? passDescriptors(timer) : null;
+    const writes = descs ? descs.filter((d) => d && d.timestampWrites) : [];
+    const record = timer && typeof timer.record === 'function';
+    const out = native.denoise(this._h, surf, stream || 0n, true, writes.length > 0 || record);
+    if (writes.length || record) {
+      const passes = native.passTimes(this._h);
+      if (writes.length) writeTimestamps(descs, passes);
+      if (record) timer.record(passes);
+    }
     return new DeviceTexture(this.device, out.width, out.height, out);
+  }
+}
+
+const PASS_LABELS = ['temporal', 'infer+mid', 'infer+mid', 'infer+mid', 'coarse', 'coarse', 'fine+final', 'taa'];
+
+// The reference's timer protocol (hush.js:846-849, index.d.ts:31-34): a span timer gets spanBegin('total') on
+// the first pass and spanEnd() on 'taa'; any other timer gets desc(label) for each of the 8 passes.
+function passDescriptors(timer) {
+  if (timer.span) {
+    return PASS_LABELS.map((label, i) => {
+      if (i === 0) return timer.spanBegin('total');
+      return label === 'taa' ? timer.spanEnd() : undefined;
+    });
+  }
+  if (typeof timer.desc === 'function') return PASS_LABELS.map((label) => timer.desc(label));
+  return PASS_LABELS.map(() => undefined);
+}
+
+// Writes each pass's begin / end timestamps (ns from the frame's first event) where its descriptor's
+// timestampWrites asks for them, as a compute pass's timestampWrites do (WebGPU); the query set must be a
+// HIP timestamp set (GpuTimer below), whose `timestamps` array receives them.
+function writeTimestamps(descs, passes) {
+  let t = 0;
+  passes.forEach((p, i) => {
+    const begin = t;
+    t += p.ms;
+    const w = descs[i] && descs[i].timestampWrites;
+    if (!w) return;
+    const qs = w.querySet;
+    if (!qs || !qs.timestamps) throw new Error('Hush.denoise: the timer\'s query set holds no HIP timestamps (use GpuTimer of this module)');
+    if (w.beginningOfPassWriteIndex !== undefined) qs.timestamps[w.beginningOfPassWriteIndex] = begin * 1e6;
+    if (w.endOfPassWriteIndex !== undefined) qs.timestamps[w.endOfPassWriteIndex] = t * 1e6;
+  });
+}
+
+// Pass timer with the interface of the reference's GpuTimer (src/gpu.js:25-77): begin(), desc(label),
+// spanBegin(label) / spanEnd(), resolve(encoder), read() -> Promise<{label: ms}> summed per label.  Its query
+// set holds the timestamps in host memory (ns, Float64), filled by Hush.denoise from the frame's hipEvents.
+class GpuTimer {
+  constructor(device, max = 64) {
+    this.device = device;
+    this.enabled = !!(device && device.features && device.features.has('timestamp-query'));
+    this.max = max;
+    if (!this.enabled) return;
+    this.qs = { type: 'timestamp', count: max * 2, timestamps: new Float64Array(max * 2) };
+    this.labels = [];
+    this.busy = false;
+    this.resolved = false;
+  }
+  begin() { this.labels = []; this.resolved = false; }
+  desc(label) {
+    if (!this.enabled || this.busy || this.labels.length >= this.max) return undefined;
+    const i = this.labels.length;
+    this.labels.push(label);
+    return { timestampWrites: { querySet: this.qs, beginningOfPassWriteIndex: i * 2, endOfPassWriteIndex: i * 2 + 1 } };
+  }
+  spanBegin(label) {
+    if (!this.enabled || this.busy || this.labels.length >= this.max) return undefined;
+    this.spanIndex = this.labels.length;
+    this.labels.push(label);
+    return { timestampWrites: { querySet: this.qs, beginningOfPassWriteIndex: this.spanIndex * 2 } };
+  }
+  spanEnd() {
+    if (this.spanIndex === undefined) return undefined;
+    const i = this.spanIndex;
+    this.spanIndex = undefined;
+    return { timestampWrites: { querySet: this.qs, endOfPassWriteIndex: i * 2 + 1 } };
+  }
+  // the timestamps are already in host memory once denoise() returns; kept for the reference's call sequence
+  resolve() {
+    if (!this.enabled || this.busy || !this.labels.length) return false;
+    this.resolved = true;
+    return true;
+  }
+  async read() {
+    if (!this.enabled || this.busy || !this.labels.length) return null;
+    const t = this.qs.timestamps;
+    const out = {};
+    this.labels.forEach((l, i) => { out[l] = (out[l] === undefined ? 0 : out[l]) + (t[i * 2 + 1] - t[i * 2]) / 1e6; });
+    return out;
   }
 }
 Hush.requiredLimits = REQUIRED_LIMITS;
@@ -180,11 +270,22 @@ class SVGF {
     this._s = null;
   }
 
+  // timer (optional, svgf.js:342-353): desc(label) is asked for each pass (temporal, variance, atrous per
+  // iteration, resolve) and a GpuTimer of this module receives the times; `record(passes)` receives
+  // [{label, ms}]
   encode(g, timer, stream) {
     const surf = {};
     for (const k of SVGF_INPUTS) surf[k] = g[k] && (g[k].surface || g[k]);
-    const out = native.svgfEncode(this._s, surf, stream || 0n, !!timer);
-    if (timer && typeof timer.record === 'function') timer.record(native.svgfPassTimes(this._s));
+    const labels = ['temporal', 'variance', ...Array(this.iterations).fill('atrous'), 'resolve'];
+    const descs = timer && typeof timer.desc === 'function' ? labels.map((l) => timer.desc(l)) : [];
+    const writes = descs.filter((d) => d && d.timestampWrites);
+    const record = timer && typeof timer.record === 'function';
+    const out = native.svgfEncode(this._s, surf, stream || 0n, writes.length > 0 || record);
+    if (writes.length || record) {
+      const passes = native.svgfPassTimes(this._s);
+      if (writes.length) writeTimestamps(descs, passes);
+      if (record) timer.record(passes);
+    }
     return new DeviceTexture(this.device, out.width, out.height, out);
   }
 }
@@ -251,5 +352,5 @@ class Post {
 }
 
 module.exports = {
-  Hush, SVGF, Post, hiddenOf, initDevice, DeviceTexture, CanvasTexture, decodeWeights, defaultWeights, N_FEAT, N_OUT,
+  Hush, SVGF, Post, GpuTimer, hiddenOf, initDevice, DeviceTexture, CanvasTexture, decodeWeights, defaultWeights, N_FEAT, N_OUT,
 };

@@ -184,6 +184,10 @@ class Oracle:
         n = np.ascontiguousarray(n, dtype=self._np)
         return bool(self._f["check"](k.ctypes.data, n.ctypes.data, zprev, obj))
 
+    def threads(self) -> int:
+        """OpenMP threads the oracle's passes run on"""
+        return int(getattr(load(), MODES[self.mode][0] + "threads")())
+
     def hidden_of(self, n) -> int:
         return int(self._f["hidden_of"](int(n)))
 

@@ -33,6 +33,7 @@
  *   here (no WebGPU runtime): parity against the WGSL itself is unpinned at threshold ties.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1024,6 +1025,7 @@ EXPORT int HO_NAME(denoise)(ho_ctx *c) {
 EXPORT int HO_NAME(output_id)(const ho_ctx *c) { return ((c->frame + 1) & 1) ? S_TAA1 : S_TAA0; }
 
 /* binary16 helpers exported for the oracle's own pins */
+EXPORT int HO_NAME(threads)(void) { return omp_get_max_threads(); }   /* OpenMP threads of the passes */
 EXPORT uint16_t HO_NAME(f2h_rtz)(real x) { return r2h_rtz(x); }
 EXPORT real HO_NAME(h2f)(uint16_t h) { return h2r(h); }
 EXPORT real HO_NAME(qf16)(real x) { return qf16(x); }

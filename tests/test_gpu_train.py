@@ -267,3 +267,28 @@ def test_prepare_refuses_bad_inputs():
     fr.prepare(good)                                   # still usable
     fr.destroy()
     hs.destroy()
+
+
+def test_resize_between_forward_and_commit_is_refused():
+    """a resize after the forward leaves the denoiser's surfaces at another size and pitch: the loss and the
+    commit refuse it (HUSH_ERR_SIZE) before any launch, and the resize's pending reset stands"""
+    import torch
+    import hush_amd
+    from hush_amd import train as T
+    from hush_amd.scene import SceneGenerator
+    hs = hush_amd.Hush(0, {"width": 64, "height": 48})
+    fr = T.DiffFrame(hs)
+    w = torch.tensor(T.weights_to_float(hush_amd.default_weights()), dtype=torch.float32, device="cuda")
+    g = SceneGenerator("spheres", 64, 48, 3).render(0, clean=True)
+    fr.prepare(T.upload(g))
+    out = fr.forward(w)
+    torch.cuda.synchronize()
+    hs.resize(32, 24)
+    with pytest.raises(hush_amd.HushError, match="resized") as e:
+        fr.commit(out)
+    assert e.value.code == -3                          # HUSH_ERR_SIZE
+    with pytest.raises(hush_amd.HushError, match="resized"):
+        fr.loss(out, T.target_of(g, torch.float32))
+    assert hs.frame == 0
+    fr.destroy()
+    hs.destroy()

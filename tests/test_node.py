@@ -146,3 +146,34 @@ def test_node_post_matches_oracle(tmp_path):
     out = np.fromfile(tmp_path / "target.bin", np.uint8).reshape(h, w, 4)
     ref, _, _ = ho.post_encode(left, right, split=0.4, exposure=1.25, bgra=True)
     assert np.abs(out.astype(int) - ref.astype(int)).max() <= 1
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_reference_timer_protocol(tmp_path):
+    """the reference's timer shapes (index.d.ts:31-34): {desc(label)} through the demo's wrapper gets one
+    entry per reference label (hush.js:858-870), SVGF's passes land in the same timer (svgf.js:353),
+    {span: true, spanBegin, spanEnd} gets one 'total' span; timed frames are bit-identical to untimed ones"""
+    from hush_amd.scene import SceneGenerator
+    w, h, frames = 128, 72, 4
+    gen = SceneGenerator("boxes", w, h, seed=19)
+    for f in range(frames):
+        g = gen.render(f)
+        for k in ("radiance", "normalDepth", "motion", "material", "emission"):
+            g[k].tofile(tmp_path / f"f{f}_{k}.bin")
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "timer_sequence.js"), str(tmp_path), str(w), str(h),
+                        str(frames)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "timer_sequence: ok" in r.stdout, r.stdout + r.stderr
+    res = json.loads((tmp_path / "timer.json").read_text())
+    hush_keys = ["hush.temporal", "hush.infer+mid", "hush.coarse", "hush.fine+final", "hush.taa"]
+    for f, d in enumerate(res):
+        if f % 2 == 0:
+            assert sorted(d) == sorted(hush_keys + ["temporal", "variance", "atrous", "resolve"]), d
+        else:
+            assert list(d) == ["total"], d
+        assert all(np.isfinite(v) and v > 0 for v in d.values()), d
+    # a span covers the eight passes: about their sum in the frames timed per pass
+    per_pass = np.mean([sum(d[k] for k in hush_keys) for d in res[0::2]])
+    total = np.mean([d["total"] for d in res[1::2]])
+    assert 0.3 * per_pass <= total <= 3.0 * per_pass, (per_pass, total)
