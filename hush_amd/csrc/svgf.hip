@@ -57,45 +57,25 @@ __device__ __forceinline__ int2 pixel_of(int W, int H) {
   return make_int2(o.x + (int)(threadIdx.x & (TW - 1)), o.y + (int)(threadIdx.x / TW));
 }
 
-// pow(x, sigmaN) (svgf.js:137, 205) for x = max(dot, 0) >= 0; n > 0: x^n by binary exponentiation in FP32
-// double-float arithmetic (each value an unevaluated sum hi + lo; the FMA gives every product's rounding error
-// exactly, Dekker / Knuth).  After at most 2 log2(n) products the relative error is ~2^-44, so hi + lo rounds
-// to the correctly rounded x^n except within ~2^-44 of a rounding boundary.  For x^128, checked on the host
-// against FP64 squarings over every binary32 x in [0.4375, 2.5) with x^128 >= 2^-100: equal but for 10 of the
-// 20.9 M inputs, where it is one ulp low (quad precision sides with FP64 there; the C library's powf differs
-// from FP64 in 44,120 of them).  Smaller powers lose the error term to underflow and keep a few ulps.
-// An overflowing product turns hi into NaN; for x >= 1 every partial power is at most x^n, so that happens
-// only when x^n itself overflows, and the result is +inf.
-struct ff { float hi, lo; };
-__device__ __forceinline__ ff ff_mul(ff a, ff b) {
-  const float p = a.hi * b.hi;
-  float e = __builtin_fmaf(a.hi, b.hi, -p);           // a.hi b.hi = p + e exactly
-  e = __builtin_fmaf(a.hi, b.lo, e);
-  e = __builtin_fmaf(a.lo, b.hi, e);
-  const float hi = p + e;                              // |e| <= ulp(p): fast two-sum
-  return ff{hi, (p - hi) + e};
-}
-__device__ __forceinline__ float ff_fin(ff r, float x) {
-  const float v = r.hi + r.lo;
-  return (v != v && x == x) ? __builtin_inff() : v;
-}
-// x^128 (the default sigmaN): seven double-float squarings
+// pow(x, sigmaN) (svgf.js:137, 205); an integer exponent 1 <= n < 65536 runs as binary exponentiation in
+// FP32 (x^128, the default, is seven squarings, the form of hush.js's own pow128), other exponents as powf.
+// WGSL bounds pow's error only (that of exp2(y log2 x)); the squarings' error, at most ~(2n - 1) half-ulps,
+// lies within it, and the oracle's FP32 mirror evaluates the same products in the same order.
 __device__ __forceinline__ float sv_pow128(float x) {
-  ff b{x, 0.0f};
 #pragma unroll
-  for (int k = 0; k < 7; k++) b = ff_mul(b, b);
-  return ff_fin(b, x);
+  for (int k = 0; k < 7; k++) x = x * x;
+  return x;
 }
 __device__ __forceinline__ float sv_pow(float x, int n, float y) {
   if (n > 0) {
-    ff b{x, 0.0f}, r{1.0f, 0.0f};
+    float b = x, r = 1.0f;
     for (int k = n;;) {
-      if (k & 1) r = ff_mul(r, b);
+      if (k & 1) r = r * b;
       k >>= 1;
       if (!k) break;
-      b = ff_mul(b, b);
+      b = b * b;
     }
-    return ff_fin(r, x);
+    return r;
   }
   return powf(x, y);
 }

@@ -38,9 +38,10 @@ def load():
 class Oracle:
     """One FP32-mirror (or FP64) restatement of the Hush class state machine."""
 
-    def __init__(self, width, height, weights, exposure=1.0, f16=False, mode="fp32"):
-        L = load()
+    def __init__(self, width, height, weights, exposure=1.0, f16=False, mode="fp32", lib=None):
+        L = load() if lib is None else lib
         self.mode = mode
+        self._lib = L
         pre, self._ct, self._np = MODES[mode]
         self._f = {}
         for name in ("create", "destroy", "resize", "reset", "set_exposure", "set_state", "frame", "hidden",
@@ -186,7 +187,7 @@ class Oracle:
 
     def threads(self) -> int:
         """OpenMP threads the oracle's passes run on"""
-        return int(getattr(load(), MODES[self.mode][0] + "threads")())
+        return int(getattr(self._lib, MODES[self.mode][0] + "threads")())
 
     def hidden_of(self, n) -> int:
         return int(self._f["hidden_of"](int(n)))

@@ -18,17 +18,29 @@
  * round-toward-zero to binary16 (clamped to +-65504, subnormals kept), and pack2x16float(qf16(a), qf16(b))
  * is the same RTZ conversion.  The inputs arrive as binary16 bit patterns and are widened exactly.
  *
- * Parity status of each function (pins are in tests/test_oracle_*.py):
+ * Parity status of each function (pins are in tests/test_oracle_*.py; the closed-form pins of
+ * tests/test_oracle_closedform.py run the FP64-ideal build of this same source, and each is shown to fail
+ * on a one-token mistake planted in a rebuilt copy):
  *   f16 RTZ / qf16 / halfToFloat      pinned: exhaustive and closed-form checks against numpy float16.
  *   lum, octEnc, octDec               pinned: round trip, octant folds, closed forms.
  *   hiddenOf                          pinned: quadratic closed form.
- *   temporal (check, reprojection,
- *     blend, features, block)         pinned: independent numpy restatement (oracle/hush_np.py) plus
- *                                     hand-built reprojection cases and static-scene invariants.
- *   MLP + activations                 pinned: numpy FP64 brute force, f = 0 closed form.
- *   filt (a-trous), down, coarse      pinned: numpy restatement, constant-field / all-sky invariants.
- *   fine+final (jbw upsample)         pinned: numpy restatement, constant-field invariant.
- *   TAA                               pinned: numpy restatement, reset passthrough, fixed point.
+ *   check                             pinned: hand-built cases either side of every threshold.
+ *   pixel(): 3x3 stats, lag, blend    pinned: closed forms on stripes with a static history (the lag's
+ *                                     second division by the count, the clamp, len, nfast, tvar).
+ *   reprojection fallback             pinned: the 3x3 search around floor(p + mv + 0.5), v = 0.1.
+ *   block(): the 20 features          pinned: constant plane, 1..4 valid pixels, depth step, normal
+ *                                     crease, luminance statistics of a known 2x2 / 4x4 window.
+ *   MLP                               pinned: numpy FP64 brute force, f = 0 closed form.
+ *   activations + mid1 wiring         pinned: closed forms with the output bias alone.
+ *   filt (a-trous)                    pinned: binomial-kernel moments (R = 1, 2; step 1, 2), the centre
+ *                                     weight, sum w^2 var / (sum w)^2, the luminance edge-stopping slope,
+ *                                     zg on a depth ramp and next to sky, prefilter clamping vs tap
+ *                                     skipping at a corner, the pow128 normal weight.
+ *   down, coarse                      pinned: count-weighted means and vr / N / N on a one-texel level.
+ *   fine+final (jbw upsamples)        pinned: depth step, normal term, kappa, fallback, coarse level.
+ *   TAA                               pinned: the 1.25 sigma clip of a known window, the 1/10 blend, the
+ *                                     inclusive uv bounds.
+ *   oracle/hush_np.py, a second restatement in numpy, remains a cross-check of the whole sequence.
  *   WebGPU store rounding, FMA contraction and transcendental ULPs of the browser stack cannot be run
  *   here (no WebGPU runtime): parity against the WGSL itself is unpinned at threshold ties.
  */

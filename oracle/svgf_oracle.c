@@ -8,7 +8,9 @@
  *     sub-texel weight precision is implementation-defined in WebGPU, so this oracle uses exact
  *     bilinear weights at uv * size - 0.5 with clamp-to-edge (parity unpinned at weight rounding);
  *   - round() is WGSL's round-half-to-even (svgf.js:74);
- *   - pow(x, sigmaN) follows the C library (WGSL bounds its error only).
+ *   - pow(x, sigmaN) (WGSL bounds its error only, as that of exp2(y log2 x)): an integer exponent
+ *     1 <= n < 65536 is binary exponentiation in the build's arithmetic (x^128 = seven squarings, the form
+ *     of hush.js:70-72 pow128; within WGSL's bound), other exponents the C library's pow.
  */
 #include <math.h>
 #include <stdint.h>
@@ -40,6 +42,18 @@ typedef double real;
 #ifndef HO_PREFIX
 #define HO_PREFIX ho32_
 #endif
+/* pow(x, y) as the passes evaluate it: binary exponentiation for an integer 1 <= y < 65536, else POW */
+static real spow(real x, real y) {
+  if (!(y >= 1 && y < 65536) || FLOOR(y) != y) return POW(x, y);
+  real b = x, r = 1;
+  for (int k = (int)y;;) {
+    if (k & 1) r = r * b;
+    k >>= 1;
+    if (!k) break;
+    b = b * b;
+  }
+  return r;
+}
 #define HO_CAT2(a, b) a##b
 #define HO_CAT(a, b) HO_CAT2(a, b)
 #define HO_NAME(x) HO_CAT(HO_PREFIX, x)
@@ -235,7 +249,7 @@ static void variance(sv_ctx *c, int cur) {
           if (ndq[3] < 0) continue;
           real len2 = SQRT((real)dx * (real)dx + (real)dy * (real)dy);
           real wz = EXP(-FABS(ndq[3] - nd[3]) / (RX(0.02) * nd[3] * len2 + RX(1e-3)));
-          real wn = POW(maxr(0, ndq[0] * nd[0] + ndq[1] * nd[1] + ndq[2] * nd[2]), c->o.sigma_n);
+          real wn = spow(maxr(0, ndq[0] * nd[0] + ndq[1] * nd[1] + ndq[2] * nd[2]), c->o.sigma_n);
           real w = wz * wn;
           sw += w;
           const real *cq = at(c->tmp_color, W, qx, qy), *mq = at(c->hist_mom[cur], W, qx, qy);
@@ -293,7 +307,7 @@ static void atrous(sv_ctx *c, int iter, int src, int cur) {
           const real *cq = at(in, W, qx, qy);
           real dist = SQRT((real)dx * (real)dx + (real)dy * (real)dy) * (real)step;
           real wz = FABS(ndq[3] - z) / (c->o.sigma_z * zg * dist + RX(1e-3));
-          real wn = POW(maxr(0, ndq[0] * nd[0] + ndq[1] * nd[1] + ndq[2] * nd[2]), c->o.sigma_n);
+          real wn = spow(maxr(0, ndq[0] * nd[0] + ndq[1] * nd[1] + ndq[2] * nd[2]), c->o.sigma_n);
           real wl = FABS(lum3(cq) - lp) / denomL;
           real w = (real)h[dx + 2] * (real)h[dy + 2] * wn * EXP(-wz - wl);
           sw += w;
