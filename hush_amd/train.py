@@ -254,7 +254,6 @@ def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256)
     `ckpt` (+ `ckpt_every`) saves weights, Adam state and step; `resume` continues from such a file;
     `time_limit` (seconds) stops early after a whole step.  Returns (weights tensor, [(step, loss)])."""
     import torch
-    from .scene import GpuSceneGenerator, SceneGenerator
     w0 = random_weights(seed) if init == "random" else weights_to_float(default_weights())
     weights = torch.tensor(w0, dtype=torch.float32, device="cuda", requires_grad=True)
     opt = torch.optim.Adam([weights], lr=lr)
@@ -271,54 +270,13 @@ def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256)
         torch.save({"weights": weights.detach(), "opt": opt.state_dict(), "step": step,
                     "hist": torch.tensor(hist, dtype=torch.float64)}, ckpt)
 
-    cw, ch = crop
-    slots = []
-    for s in range(sequences):
-        hs = Hush(0, {"width": cw, "height": ch})      # temporal / coarse passes only: no MLP weights used
-        fr = DiffFrame(hs, forward=forward)
-        gen = GpuSceneGenerator("train", scene_size[0], scene_size[1], 0) if gpu_scenes else None
-        ins = {k: torch.empty((ch, cw, 4), dtype=torch.float16, device="cuda") for k in GpuSceneGenerator.KEYS}
-        tgt = [torch.empty((ch, cw, 4), dtype=torch.float32, device="cuda") for _ in range(2)]
-        slots.append((hs, fr, gen, ins, tgt))
+    slots = make_slots(sequences, crop, scene_size, forward, gpu_scenes)
     t0 = time.perf_counter()
     for step in range(start, steps):
         opt.zero_grad(set_to_none=False)
-        total = 0.0
-        for s in range(sequences):
-            hs, fr, gen, ins, tgts = slots[s]
-            rs = np.random.default_rng([seed, step, s])
-            gseed = int(rs.integers(1 << 30))
-            if gpu_scenes:
-                gen.set_seed(gseed)
-            else:
-                gen = SceneGenerator("train", scene_size[0], scene_size[1], gseed)
-            x0 = int(rs.integers(0, scene_size[0] - cw + 1))
-            y0 = int(rs.integers(0, scene_size[1] - ch + 1))
-            f0 = int(rs.integers(0, 96))
-            hs.reset()
-            prev_out = prev_tgt = None
-            for f in range(frames):
-                if gpu_scenes:
-                    tgt = tgts[f & 1]
-                    gen.render(f0 + f, ins, crop=(x0, y0, cw, ch), clean=tgt)
-                    fins = ins
-                else:
-                    g = gen.render(f0 + f, crop=(x0, y0, cw, ch), clean=True)
-                    fins, tgt = upload(g), target_of(g, torch.float32)
-                if gen.is_cut(f0 + f):
-                    hs.reset()
-                    prev_out = prev_tgt = None
-                fr.prepare(fins)
-                out = frame_apply(weights, fr)
-                loss = loss_apply(out, tgt, prev_out, prev_tgt, lam if prev_out is not None else 0.0, fr)
-                (loss / (sequences * frames)).backward()
-                total += float(loss.detach())
-                fr.commit(out.detach())
-                prev_out, prev_tgt = out.detach(), tgt
-            if not gpu_scenes:
-                gen.close()
+        total, _ = run_batch(weights, slots, seed, step, frames, crop, scene_size, lam, gpu_scenes)
         opt.step()
-        hist.append((step, total / (sequences * frames)))
+        hist.append((step, total))
         last = step == steps - 1
         if log and (step % max(1, steps // 20) == 0 or last):
             log(f"[train] step {step} loss {hist[-1][1]:.5f} ({time.perf_counter() - t0:.1f}s)")
@@ -330,12 +288,81 @@ def train(steps: int = 100, sequences: int = 2, frames: int = 8, crop=(256, 256)
             if ckpt:
                 save(step)
             break
+    close_slots(slots)
+    return weights.detach(), hist
+
+
+def make_slots(sequences: int, crop, scene_size, forward: str = "fp32", gpu_scenes: bool = True):
+    """one denoiser + training context (and GPU scene generator and input tensors) per sequence slot"""
+    import torch
+    from .scene import GpuSceneGenerator
+    cw, ch = crop
+    slots = []
+    for _ in range(sequences):
+        hs = Hush(0, {"width": cw, "height": ch})      # temporal / coarse passes only: no MLP weights used
+        fr = DiffFrame(hs, forward=forward)
+        gen = GpuSceneGenerator("train", scene_size[0], scene_size[1], 0) if gpu_scenes else None
+        ins = {k: torch.empty((ch, cw, 4), dtype=torch.float16, device="cuda") for k in GpuSceneGenerator.KEYS}
+        tgt = [torch.empty((ch, cw, 4), dtype=torch.float32, device="cuda") for _ in range(2)]
+        slots.append((hs, fr, gen, ins, tgt))
+    return slots
+
+
+def close_slots(slots) -> None:
     for hs, fr, gen, _, _ in slots:
         fr.destroy()
         hs.destroy()
         if gen is not None:
             gen.close()
-    return weights.detach(), hist
+
+
+def run_batch(weights, slots, seed: int, step: int, frames: int, crop, scene_size, lam: float = 0.5,
+              gpu_scenes: bool = True, backward: bool = True):
+    """One step's batch: for each slot a seeded crop of a 'train' sequence (scene seed, crop origin and first
+    frame drawn from (seed, step, slot)), `frames` frames from a reset history, each frame's loss (L1 + lam x
+    temporal term) accumulated into weights.grad / (slots x frames) when `backward`.  Returns (mean loss,
+    number of camera cuts inside the batch, each of which reset the history)."""
+    from .scene import SceneGenerator
+    cw, ch = crop
+    total, cuts = 0.0, 0
+    for s, (hs, fr, gen, ins, tgts) in enumerate(slots):
+        rs = np.random.default_rng([seed, step, s])
+        gseed = int(rs.integers(1 << 30))
+        if gpu_scenes:
+            gen.set_seed(gseed)
+        else:
+            gen = SceneGenerator("train", scene_size[0], scene_size[1], gseed)
+        x0 = int(rs.integers(0, scene_size[0] - cw + 1))
+        y0 = int(rs.integers(0, scene_size[1] - ch + 1))
+        f0 = int(rs.integers(0, 96))
+        hs.reset()
+        prev_out = prev_tgt = None
+        for f in range(frames):
+            if gpu_scenes:
+                tgt = tgts[f & 1]
+                gen.render(f0 + f, ins, crop=(x0, y0, cw, ch), clean=tgt)
+                fins = ins
+            else:
+                import torch
+                g = gen.render(f0 + f, crop=(x0, y0, cw, ch), clean=True)
+                fins, tgt = upload(g), target_of(g, torch.float32)
+            if gen.is_cut(f0 + f):
+                hs.reset()
+                prev_out = prev_tgt = None
+                cuts += f > 0
+            fr.prepare(fins)
+            out = frame_apply(weights, fr)
+            loss = loss_apply(out, tgt, prev_out, prev_tgt, lam if prev_out is not None else 0.0, fr)
+            if backward:
+                (loss / (len(slots) * frames)).backward()
+            total += float(loss.detach())
+            fr.commit(out.detach())
+            prev_out, prev_tgt = out.detach(), tgt
+        if not gpu_scenes:
+            gen.close()
+    return total / (len(slots) * frames), cuts
+
+
 
 
 # ------------------------------------------------------------------------------------------------ validation

@@ -43,6 +43,16 @@ def pass_outputs(p: int, cur: int) -> list:
             3: ["Bh0"], 4: ["C1", "CG"], 5: ["C0"], 6: [f"hist{cur}", "X"], 7: [f"taa{cur}"]}[p]
 
 
+def log_parity(case: str, stats: list) -> None:
+    """appends per-frame comparison statistics to the JSON-lines file HUSH_PARITY_LOG names (if set)"""
+    import json
+    import os
+    path = os.environ.get("HUSH_PARITY_LOG")
+    if path:
+        with open(path, "a") as fh:
+            fh.write(json.dumps({"case": case, "stats": stats}) + "\n")
+
+
 def ulp16(x: np.ndarray) -> np.ndarray:
     """spacing of binary16 at |x| (normal range), 2^-24 below it"""
     ax = np.abs(x.astype(np.float64))
@@ -60,7 +70,8 @@ def compare(gpu: np.ndarray, ref: np.ndarray) -> dict:
     rel = d / np.maximum(np.abs(r), 1e-3)
     return {"n": int(d.size), "exact": float(np.mean(d == 0)), "max_ulp": float(ulps.max(initial=0)),
             "p9999_ulp": float(np.quantile(ulps, 0.9999)) if d.size else 0.0, "max_rel": float(rel.max(initial=0)),
-            "frac_rel_gt_1e-3": float(np.mean(rel > 1e-3))}
+            "frac_rel_gt_1e-3": float(np.mean(rel > 1e-3)), "frac_rel_gt_1e-2": float(np.mean(rel > 1e-2)),
+            "frac_ulp_gt_1": float(np.mean(ulps > 1.0))}
 
 
 def tonemap(c: np.ndarray) -> np.ndarray:

@@ -13,7 +13,7 @@ import pytest
 
 import hush_amd
 from oracle.ho import Oracle
-from tests.parity import compare, upload_frame
+from tests.parity import compare, log_parity, upload_frame
 
 pytestmark = pytest.mark.gpu
 
@@ -24,6 +24,7 @@ def _scene(w, h, seed, kind="boxes"):
 
 
 def _check(stats, what):
+    log_parity(f"api-{what}", stats)
     for st in stats:
         assert st["mask_exact"] == 1.0, (what, st)
         assert st["frac_rel_gt_1e-3"] <= 1e-3, (what, st)

@@ -132,7 +132,8 @@ def test_mlp_tail_rows(hs, feats):
 
 
 def test_pipeline_psnr_per_precision():
-    """whole pipeline, 8 frames at 320x180: PSNR of each MLP precision against the FP64 oracle"""
+    """whole pipeline, 8 frames at 320x180: PSNR of each MLP precision against the FP64 oracle; FP16 within
+    0.05 dB of the FP32 path, BF16 within 0.25 dB, FP8 within 2.5 dB (measured: +0.04, -0.12, -1.25)"""
     from hush_amd.scene import SceneGenerator
     w, h, frames = 320, 180, 8
     gen = SceneGenerator("boxes", w, h, 21)
@@ -152,6 +153,8 @@ def test_pipeline_psnr_per_precision():
     print("PSNR vs FP64 oracle (dB):", res)
     assert res["fp32"] > 30
     assert abs(res["fp16-mfma"] - res["fp32"]) <= 0.05, res
+    assert res["bf16-mfma"] >= res["fp32"] - 0.25, res
+    assert res["fp8-mfma"] >= res["fp32"] - 2.5, res
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp16-mfma", "bf16-mfma"])

@@ -7,20 +7,22 @@ Both math policies are checked.  Tolerances (FP32 path):
   * reprojection validity masks (tap bits, fallback, has, sky): bit-exact under either policy;
   * IEEE policy, teacher forced: at most 1 binary16 ULP per stored value (RTZ boundary flips from the
     1-ULP differences of expf / logf between the GPU's ocml and glibc) and >= 99.9% bit-exact;
-  * end-to-end HDR output, IEEE: |gpu - oracle| <= 1e-3 * max(|oracle|, 1e-3) for >= 99.9% of values, the
-    rest within 1e-2 relative (history feedback compounds single-ULP flips over frames);
+  * end-to-end HDR output, IEEE, 8 frames at 640x360: |gpu - oracle| <= 1e-3 * max(|oracle|, 1e-3) for
+    >= 99.99% of values, all within 8e-3 relative, 99.99% within one binary16 ulp (measured: 4.1e-5 beyond
+    1e-3, max 3.8e-3; the residual is one binary16 round-toward-zero ulp, 4.9e-4 .. 9.8e-4 relative, flipped
+    by single-ulp expf / logf differences between the GPU's math library and the host's, and carried by the
+    history feedback);
   * FAST policy (hardware rcp / sqrt / exp2 / log2, 1-2 fp32 ulp, in the spatial filters only) is an
     opt-in outside the drop-in tolerance: teacher forced <= 8 binary16 ULP with <= 1% of values beyond
     1e-3 relative; end to end <= 2% of values beyond 1e-3 and none beyond 0.25.
 """
-import json
-import os
 
 import numpy as np
 import pytest
 
 from oracle.ho import Oracle
-from tests.parity import (INPUT_KEYS, compare, force_oracle, gpu_F, gpu_surface, pass_outputs, upload_frame)
+from tests.parity import (INPUT_KEYS, compare, force_oracle, gpu_F, gpu_surface, log_parity, pass_outputs,
+                          upload_frame)
 
 pytestmark = pytest.mark.gpu
 
@@ -74,10 +76,7 @@ def teacher_forced(kind, w, h, frames, seed, exposure=1.0, ieee=True):
 
 
 def _report(tag, stats):
-    path = os.environ.get("HUSH_PARITY_LOG")
-    if path:
-        with open(path, "a") as fh:
-            fh.write(json.dumps({"case": tag, "stats": stats}) + "\n")
+    log_parity(tag, stats)
     worst = sorted(stats, key=lambda s: -s["max_ulp"])[:6]
     for s in worst:
         print(tag, s)
@@ -123,8 +122,10 @@ def test_end_to_end_sequence(ieee):
     for st in stats:
         assert st["mask_exact"] == 1.0, st
         if ieee:
-            assert st["frac_rel_gt_1e-3"] <= 1e-3, st
-            assert st["max_rel"] <= 1e-2, st
+            # measured: <= 4.1e-5 of the values beyond 1e-3 relative, max 3.8e-3, 99.99% within one ulp
+            assert st["frac_rel_gt_1e-3"] <= 1e-4, st
+            assert st["max_rel"] <= 8e-3, st
+            assert st["p9999_ulp"] <= 1.0, st
         else:
             assert st["frac_rel_gt_1e-3"] <= 2e-2, st
             assert st["max_rel"] <= 0.25, st

@@ -292,3 +292,37 @@ def test_resize_between_forward_and_commit_is_refused():
     assert hs.frame == 0
     fr.destroy()
     hs.destroy()
+
+
+def test_training_step_at_the_end_to_end_configuration():
+    """The end-to-end training configuration's step: 16 sequences x 24 frames of 256x256 crops of 1920x1080
+    'train' scenes (300 primitives, 30% moving, a camera cut every 48 frames resets the history), FP16-MFMA
+    forward and FP32 backward.  The batch holds camera cuts; its loss is finite; the gradient points the way of
+    the FP32-forward gradient of the same batch (cosine >= 0.99); one Adam step lowers the batch's loss."""
+    import torch
+    from hush_amd import train as T
+    seq, frames, crop, scene = 16, 24, (256, 256), (1920, 1080)
+    w = torch.tensor(T.random_weights(0), dtype=torch.float32, device="cuda", requires_grad=True)
+    res = {}
+    for fwd in ("fp32", "fp16-mfma"):
+        slots = T.make_slots(seq, crop, scene, forward=fwd)
+        w.grad = None
+        loss, cuts = T.run_batch(w, slots, seed=0, step=0, frames=frames, crop=crop, scene_size=scene)
+        res[fwd] = (loss, cuts, w.grad.detach().clone())
+        if fwd == "fp32":
+            T.close_slots(slots)
+    l16, cuts, g16 = res["fp16-mfma"]
+    l32, _, g32 = res["fp32"]
+    cos = float(torch.dot(g16, g32) / (g16.norm() * g32.norm()))
+    print(f"loss fp16-mfma {l16:.6f} fp32 {l32:.6f}, cuts {cuts}, gradient cosine {cos:.5f}")
+    assert cuts >= 1
+    assert np.isfinite(l16) and np.isfinite(l32) and torch.isfinite(g16).all()
+    assert abs(l16 - l32) <= 1e-2 * l32, (l16, l32)
+    assert cos >= 0.99, cos
+    opt = torch.optim.Adam([w], lr=1e-3)
+    w.grad = g16
+    opt.step()
+    after, _ = T.run_batch(w, slots, seed=0, step=0, frames=frames, crop=crop, scene_size=scene, backward=False)
+    T.close_slots(slots)
+    print(f"batch loss before {l16:.6f} after one Adam step {after:.6f}")
+    assert np.isfinite(after) and after < l16, (l16, after)
