@@ -26,7 +26,7 @@ EXPORTS = [
     # include/hush_train.h
     "hush_train_create", "hush_train_destroy", "hush_train_last_error", "hush_train_prepare", "hush_train_forward",
     "hush_train_loss", "hush_train_backward", "hush_train_commit", "hush_train_a0", "hush_train_set_a0_fixed",
-    "hush_train_set_forward_precision", "hush_train_mlp_out",
+    "hush_train_set_forward_precision", "hush_train_mlp_out", "hush_loss_noalloc",
     # include/hush_scene.h
     "hush_scene_create", "hush_scene_destroy", "hush_scene_last_error", "hush_scene_render",
     # include/post.h

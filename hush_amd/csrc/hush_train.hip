@@ -8,6 +8,7 @@
 // weight gradient is reduced in FP64: a partial sum per workgroup, then one fixed-order sum per weight.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "hush.h"
@@ -1154,6 +1155,60 @@ extern "C" int32_t hush_train_loss(hush_train_t *t, const void *out, const void 
                 : loss_t<float>(t, (const float *)out, (const float *)target, (const float *)prev_out, (const float *)prev_target,
                                 lambda, (float *)grad_out, loss_dev, (hipStream_t)stream);
   if (e != hipSuccess) return tfail(t, HUSH_ERR_DEVICE, "hush_train_loss", e);
+  return HUSH_OK;
+}
+
+// ---- the loss without a context and without allocation (hush_loss_noalloc)
+// The per-workgroup partial sums of tk_loss live in this file-scope array; each GPU holds its own copy, whose
+// device address hipGetSymbolAddress gives once per device.  tk_loss writes every partial it launches and
+// tk_sum reads exactly those, so nothing needs clearing between calls; calls that overlap on one GPU share
+// the array and race.
+constexpr int NOALLOC_PARTS = 65536;                     // frames up to 65536 * 256 = 16,777,216 pixels
+static __device__ __align__(256) double scratch[NOALLOC_PARTS];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[64];                         // per device ordinal
+
+static int scratch_ptr(double **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return tfail(nullptr, HUSH_ERR_DEVICE, "hush_loss_noalloc: hipGetDevice", e);
+  if (dev < 0 || dev >= 64) return tfail(nullptr, HUSH_ERR_UNSUPPORTED, "hush_loss_noalloc: device ordinals above 63 are not supported");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));   // with `dev` current: this GPU's copy
+    if (e != hipSuccess || !p) return tfail(nullptr, HUSH_ERR_DEVICE, "hush_loss_noalloc: hipGetSymbolAddress of the scratch array", e);
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_loss_noalloc(int32_t fp64, int32_t width, int32_t height, const void *out, const void *target,
+                                     const void *prev_out, const void *prev_target, double lambda, void *grad_out,
+                                     double *loss_dev, void *stream) {
+  if (!out || !target || !grad_out || !loss_dev) return tfail(nullptr, HUSH_ERR_ARGUMENT, "hush_loss_noalloc: null argument");
+  if ((prev_out == nullptr) != (prev_target == nullptr)) return tfail(nullptr, HUSH_ERR_ARGUMENT, "hush_loss_noalloc: give both prev_out and prev_target, or neither");
+  if (!(width > 0 && height > 0)) return tfail(nullptr, HUSH_ERR_SIZE, "hush_loss_noalloc: non-positive size");
+  const long n = (long)width * height, nb = (n + NTH - 1) / NTH;
+  if (nb > NOALLOC_PARTS) {
+    std::snprintf(g_terr, sizeof g_terr, "hush_loss_noalloc: %dx%d exceeds the static scratch (at most %ld pixels)",
+                  width, height, (long)NOALLOC_PARTS * NTH);
+    return HUSH_ERR_SIZE;
+  }
+  double *part = nullptr;
+  int r = scratch_ptr(&part);
+  if (r != HUSH_OK) return r;
+  hipStream_t s = (hipStream_t)stream;
+  if (fp64)
+    hipLaunchKernelGGL(tk_loss<double>, dim3((unsigned)nb), dim3(NTH), 0, s, (const double *)out, (const double *)target,
+                       (const double *)prev_out, (const double *)prev_target, lambda, (int)n, (double *)grad_out, part);
+  else
+    hipLaunchKernelGGL(tk_loss<float>, dim3((unsigned)nb), dim3(NTH), 0, s, (const float *)out, (const float *)target,
+                       (const float *)prev_out, (const float *)prev_target, (float)lambda, (int)n, (float *)grad_out, part);
+  hipLaunchKernelGGL(tk_sum, dim3(1), dim3(NTH), 0, s, (const double *)part, (int)nb, 1.0 / (double)n, loss_dev);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return tfail(nullptr, HUSH_ERR_DEVICE, "hush_loss_noalloc", e);
   return HUSH_OK;
 }
 

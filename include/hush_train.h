@@ -55,6 +55,18 @@ int32_t hush_train_forward(hush_train_t *t, const void *weights, void *out, void
 int32_t hush_train_loss(hush_train_t *t, const void *out, const void *target, const void *prev_out,
                         const void *prev_target, double lambda, void *grad_out, double *loss_dev, void *stream);
 
+/* hush_train_loss without a training context and without any allocation.  No counterpart in the
+   reference: for callers that cannot allocate device memory (code under HIP graph capture, tight serving or
+   training loops, harnesses that forbid hipMalloc).  Same loss, gradient and FP64 reduction as
+   hush_train_loss (the same kernels over the same workgroups, so the same bits) for width x height pixels
+   of float (fp64 = 0) or double (fp64 = 1) rgba values, on the current HIP device (`stream` must belong to
+   it).  The per-workgroup partial sums live in a static device array of 512 KiB per GPU, held until the
+   process ends: at most 16,777,216 pixels (4096 x 4096), larger sizes return HUSH_ERR_SIZE before any
+   launch.  Calls that overlap on one GPU share that array and race.  Errors: hush_train_last_error(NULL). */
+int32_t hush_loss_noalloc(int32_t fp64, int32_t width, int32_t height, const void *out, const void *target,
+                          const void *prev_out, const void *prev_target, double lambda, void *grad_out,
+                          double *loss_dev, void *stream);
+
 /* backward of the last forward: grad_out (W x H x 4) -> grad_w (812 values, overwritten); grad_a0
    (optional, W x H x 4) receives the gradient with respect to the temporal output A0 */
 int32_t hush_train_backward(hush_train_t *t, const void *weights, const void *grad_out, void *grad_w, void *grad_a0,
