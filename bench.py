@@ -419,6 +419,10 @@ def main():
                     help="after the warm-up frames, time only the MLP stage (hush_mlp_forward over the frame's half-res "
                          "feature rows, every precision, --steps launches each) and print it as one JSON line; for "
                          "profiling the MLP kernels on their own")
+    ap.add_argument("--train-only", action="store_true",
+                    help="after generating the inputs, time only the differentiable frame (the training leg: 1080p "
+                         "forward + loss + backward, and the 8 x 16 x 512^2 step) and print it as one JSON line; for "
+                         "profiling the training kernels on their own")
     ap.add_argument("--post-only", action="store_true",
                     help="time only the display post (Post.encode of two resident seeded HDR images into a "
                          "width x height bgra8 target, --steps encodes) and print it as one JSON line")
@@ -455,6 +459,9 @@ def main():
     hip.synchronize()
     print(f"[bench] inputs resident in HBM ({F * 5 * W * H * 8 / 1e9:.2f} GB)", file=sys.stderr, flush=True)
 
+    if args.train_only:
+        print(json.dumps({"train_fp32": train_leg(dev, W, H), "width": W, "height": H}), flush=True)
+        return
     hs = hush_amd.Hush(0, {"width": W, "height": H, "precision": args.precision, "ieee_math": args.math == "ieee"})
     stream = hip.Stream()
 

@@ -713,12 +713,13 @@ __global__ __launch_bounds__(NTH) void tk_sum(const double *part, int n, double 
 constexpr int MB = 128;                 // texels per workgroup of the weight-gradient reduction
 constexpr int NV = 20 + 4 * HID + 12;   // f, h1, h2, gh1, gh2, go
 
+// One texel of the MLP backward: features f, hidden activations h1, h2 (recomputed forward), the output
+// gradient go through the activations (hush.js:357-366) and the hidden gradients gh2, gh1 through the ReLUs;
+// all zero for a sky or out-of-range texel.
 template <class T>
-__global__ __launch_bounds__(MB) void tk_mlp_bwd(TexU F0, TexU F1, TexU F2, TexF GH, int W2, int H2, const T *w, const T *o,
-                                                   const T *par, const T *gpar, double *part) {
-  __shared__ T sv[NV][MB];
-  const int t = threadIdx.x, i = blockIdx.x * MB + t;
-  T f[20], h1[HID], h2[HID], go[12], gh1[HID], gh2[HID];
+__device__ __forceinline__ void mlp_bwd_texel(TexU F0, TexU F1, TexU F2, TexF GH, int W2, int H2, const T *w, const T *o,
+                                              const T *par, const T *gpar, int i, T *f, T *h1, T *h2, T *go, T *gh1,
+                                              T *gh2) {
   bool live = false;
   if (i < W2 * H2) {
     const int x = i % W2, y = i / W2;
@@ -754,13 +755,33 @@ __global__ __launch_bounds__(MB) void tk_mlp_bwd(TexU F0, TexU F1, TexU F2, TexF
     for (int k = 0; k < HID; k++) { h1[k] = h2[k] = gh1[k] = gh2[k] = T(0); }
     for (int k = 0; k < 12; k++) go[k] = T(0);
   }
-  for (int k = 0; k < 20; k++) sv[k][t] = f[k];
+}
+
+// rows of the per-texel value table: f, h1, h2, gh1, gh2, go; one column per texel (padded by one against
+// LDS bank conflicts between rows)
+constexpr int RF = 0, RH1 = 20, RH2 = 20 + HID, RG1 = 20 + 2 * HID, RG2 = 20 + 3 * HID, RGO = 20 + 4 * HID;
+constexpr int SVP = MB + 1;
+
+template <class T>
+__device__ __forceinline__ void store_texel(T (*sv)[SVP], int t, const T *f, const T *h1, const T *h2, const T *go,
+                                            const T *gh1, const T *gh2) {
+  for (int k = 0; k < 20; k++) sv[RF + k][t] = f[k];
   for (int k = 0; k < HID; k++) {
-    sv[20 + k][t] = h1[k]; sv[20 + HID + k][t] = h2[k]; sv[20 + 2 * HID + k][t] = gh1[k]; sv[20 + 3 * HID + k][t] = gh2[k];
+    sv[RH1 + k][t] = h1[k]; sv[RH2 + k][t] = h2[k]; sv[RG1 + k][t] = gh1[k]; sv[RG2 + k][t] = gh2[k];
   }
-  for (int k = 0; k < 12; k++) sv[20 + 4 * HID + k][t] = go[k];
+  for (int k = 0; k < 12; k++) sv[RGO + k][t] = go[k];
+}
+
+// The MLP weight gradient, FP64 (the gradient-check contexts): a partial per 128-texel workgroup, each weight
+// the FP64 sum of its products over the workgroup's texels in texel order
+__global__ __launch_bounds__(MB) void tk_mlp_bwd64(TexU F0, TexU F1, TexU F2, TexF GH, int W2, int H2, const double *w,
+                                                  const double *o, const double *par, const double *gpar, double *part) {
+  __shared__ double sv[NV][SVP];
+  const int t = threadIdx.x;
+  double f[20], h1[HID], h2[HID], go[12], gh1[HID], gh2[HID];
+  mlp_bwd_texel<double>(F0, F1, F2, GH, W2, H2, w, o, par, gpar, blockIdx.x * MB + t, f, h1, h2, go, gh1, gh2);
+  store_texel<double>(sv, t, f, h1, h2, go, gh1, gh2);
   __syncthreads();
-  constexpr int RF = 0, RH1 = 20, RH2 = 20 + HID, RG1 = 20 + 2 * HID, RG2 = 20 + 3 * HID, RGO = 20 + 4 * HID;
   for (int k = t; k < NW; k += MB) {
     int ra, rb;          // rows of sv whose product sums to weight k (rb < 0: a bias, the sum of row ra)
     if (k < OB1) { ra = RG1 + k / 20; rb = RF + k % 20; }
@@ -771,21 +792,108 @@ __global__ __launch_bounds__(MB) void tk_mlp_bwd(TexU F0, TexU F1, TexU F2, TexF
     else { ra = RGO + (k - OB3); rb = -1; }
     double acc = 0.0;
     if (rb >= 0)
-      for (int j = 0; j < MB; j++) acc += (double)sv[ra][j] * (double)sv[rb][j];
+      for (int j = 0; j < MB; j++) acc += sv[ra][j] * sv[rb][j];
     else
-      for (int j = 0; j < MB; j++) acc += (double)sv[ra][j];
+      for (int j = 0; j < MB; j++) acc += sv[ra][j];
     part[(size_t)blockIdx.x * NW + k] = acc;
   }
 }
 
-// per weight: fixed-order FP64 sum over the workgroup partials
+// The MLP weight gradient, FP32 (training contexts), on the matrix cores.  Per texel tile the gradients are
+// outer-product sums over the texels: dW1 = sum gh1 f^T, dW2 = sum gh2 h1^T, dW3 = sum go h2^T and the biases
+// sum gh1, sum gh2, sum go.  Each is a 16 x 16 tile D = A B with the texels as the inner dimension, so wave v
+// of the workgroup feeds its 64 texels of each 128-texel tile, four at a time, to v_mfma_f32_16x16x4f32 (lane
+// l: A[l % 16][l / 16], B[l / 16][l % 16]; D[4 (l / 16) + r][l % 16] in accumulator r).  Six tiles:
+//   T0  A = gh1, B = f[0..15]                -> dW1[:, 0..15]
+//   T1  A = gh1, B = (f[16..19], 1, 0 ...)   -> dW1[:, 16..19], db1
+//   T2  A = gh2, B = h1                      -> dW2
+//   T3  A = (go, 0), B = h2                  -> dW3
+//   T4  A = gh2, B = (1, 0 ...)              -> db2
+//   T5  A = (go, 0), B = (1, 0 ...)          -> db3
+// The accumulators run over MT tiles per workgroup (FP32 fused multiply-add chains in texel order); the two
+// waves' sums add in wave order and the workgroup writes one partial per weight, which the FP64 stage
+// (tk_wsum1 / tk_wsum2) reduces.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int MT = 4;                   // 128-texel tiles per workgroup of the FP32 form
+__global__ __launch_bounds__(MB) void tk_mlp_bwd_mfma(TexU F0, TexU F1, TexU F2, TexF GH, int W2, int H2, const float *w,
+                                                      const float *o, const float *par, const float *gpar, double *part) {
+  __shared__ float sv[NV][SVP];
+  __shared__ f32x4 red[6][64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
+  f32x4 acc[6];
+  for (int q = 0; q < 6; q++) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int it = 0; it < MT; it++) {
+    float f[20], h1[HID], h2[HID], go[12], gh1[HID], gh2[HID];
+    mlp_bwd_texel<float>(F0, F1, F2, GH, W2, H2, w, o, par, gpar, (blockIdx.x * MT + it) * MB + t, f, h1, h2, go, gh1, gh2);
+    if (it) __syncthreads();                     // the previous tile's operands have been read
+    store_texel<float>(sv, t, f, h1, h2, go, gh1, gh2);
+    __syncthreads();
+#pragma unroll 4
+    for (int st = 0; st < 16; st++) {
+      const int j = wv * 64 + st * 4 + lk;        // texel of this lane's k
+      const float a1 = sv[RG1 + li][j], a2 = sv[RG2 + li][j], ao = li < 12 ? sv[RGO + li][j] : 0.0f;
+      const float bf0 = sv[RF + li][j], bf1 = li < 4 ? sv[RF + 16 + li][j] : (li == 4 ? 1.0f : 0.0f);
+      const float bh1 = sv[RH1 + li][j], bh2 = sv[RH2 + li][j], b1 = li == 0 ? 1.0f : 0.0f;
+      acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bf0, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bf1, acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, bh1, acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ao, bh2, acc[3], 0, 0, 0);
+      acc[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b1, acc[4], 0, 0, 0);
+      acc[5] = __builtin_amdgcn_mfma_f32_16x16x4f32(ao, b1, acc[5], 0, 0, 0);
+    }
+  }
+  if (wv == 1)
+    for (int q = 0; q < 6; q++) red[q][lane] = acc[q];
+  __syncthreads();
+  if (wv == 0) {
+    double *pb = part + (size_t)blockIdx.x * NW;
+    for (int r = 0; r < 4; r++) {
+      const int i = 4 * lk + r, j = li;           // D[i][j] in accumulator r of this lane
+      float d[6];
+      for (int q = 0; q < 6; q++) d[q] = acc[q][r] + red[q][lane][r];
+      pb[OW1 + i * 20 + j] = d[0];
+      if (j < 4) pb[OW1 + i * 20 + 16 + j] = d[1];
+      else if (j == 4) pb[OB1 + i] = d[1];
+      pb[OW2 + i * HID + j] = d[2];
+      if (i < 12) pb[OW3 + i * HID + j] = d[3];
+      if (j == 0) pb[OB2 + i] = d[4];
+      if (j == 0 && i < 12) pb[OB3 + i] = d[5];
+    }
+  }
+}
+
+// Per weight: the FP64 sum of the nb workgroup partials part[b * NW + k], in a fixed order (the same bits on
+// every run), in two stages so that it spreads over the GPU.  Stage 1: workgroup (x, y) takes 64 consecutive
+// weights (one per lane, so each wave reads 512 contiguous bytes per partial row) and the y-th of WSL slices
+// of the rows; wave v sums rows v, v + 4, ... of the slice, then the four wave sums add in wave order.
+// Stage 2: the WSL slice sums of each weight, the same way.  A single thread per weight walking all rows
+// was a dependent chain of nb loads (2.6 ms at 1920x1080).
+constexpr int WSL = 64;
+__global__ __launch_bounds__(NTH) void tk_wsum1(const double *part, int nb, double *mid) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, k = blockIdx.x * 64 + lane;
+  const int per = (nb + WSL - 1) / WSL, b0 = blockIdx.y * per, b1 = min(nb, b0 + per);
+  double acc = 0.0;
+  if (k < NW) {
+#pragma unroll 8
+    for (int b = b0 + wv; b < b1; b += 4) acc += part[(size_t)b * NW + k];
+  }
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && k < NW) mid[(size_t)blockIdx.y * NW + k] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
 template <class T>
-__global__ __launch_bounds__(NTH) void tk_wsum(const double *part, int nb, T *gw) {
-  const int k = tid1();
-  if (k >= NW) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; b++) s += part[(size_t)b * NW + k];
-  gw[k] = (T)s;
+__global__ __launch_bounds__(NTH) void tk_wsum2(const double *mid, T *gw) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, k = blockIdx.x * 64 + lane;
+  double acc = 0.0;
+  if (k < NW) {
+#pragma unroll
+    for (int q = 0; q < WSL / 4; q++) acc += mid[(size_t)(wv + 4 * q) * NW + k];
+  }
+  red[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && k < NW) gw[k] = (T)(((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
 }
 
 
@@ -947,12 +1055,12 @@ extern "C" int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out
   t->h = h; t->fp64 = fp64 != 0; t->tsz = fp64 ? 8 : 4;
   t->W = h->W; t->H = h->H; t->W2 = h->W2; t->H2 = h->H2; t->W4 = h->W4; t->H4 = h->H4;
   const size_t N = (size_t)t->W * t->H, N2 = (size_t)t->W2 * t->H2, ts = t->tsz;
-  t->nwb = (int)((N2 + MB - 1) / MB);
+  t->nwb = (int)((N2 + (size_t)MB * (fp64 ? 1 : MT) - 1) / ((size_t)MB * (fp64 ? 1 : MT)));
   t->nlb = (int)((N + NTH - 1) / NTH);
   // half: o, par, gpar, coefH (12 each), mid x3, gmid x3 (4 each); full: a0, fine, X, prev, gX, gfine, gill, gpf
   // (4 each), taac, coefF (12 each); FP64 partials
   const size_t half = N2 * ts * (7 * 12 + 6 * 4) + N2 * 16 * 4, full = N * ts * (9 * 4 + 2 * 12) + N * 4 * TEMPORAL_AUX;
-  const size_t parts = ((size_t)2 * t->nwb * NW + t->nlb + 64) * sizeof(double);
+  const size_t parts = ((size_t)2 * t->nwb * NW + (size_t)WSL * NW + t->nlb + 64) * sizeof(double);
   const size_t total = half + full + parts + 64 * 256;
   e = hipMalloc(&t->arena, total);
   if (e == hipSuccess) e = hipMemset(t->arena, 0, total);
@@ -973,7 +1081,7 @@ extern "C" int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out
   t->aux = (float *)take(N * 4 * TEMPORAL_AUX);
   for (int k = 0; k < 3; k++) t->Fprev[k] = (uint4 *)take(N2 * 16);
   t->GHprev = (float4 *)take(N2 * 16);
-  t->wpart = (double *)take((size_t)2 * t->nwb * NW * sizeof(double));
+  t->wpart = (double *)take(((size_t)2 * t->nwb + WSL) * NW * sizeof(double));   // partials, then the slice sums
   t->lpart = (double *)take((size_t)t->nlb * sizeof(double));
   *out = t;
   return HUSH_OK;
@@ -1212,6 +1320,17 @@ extern "C" int32_t hush_loss_noalloc(int32_t fp64, int32_t width, int32_t height
   return HUSH_OK;
 }
 
+// the MLP weight gradient's workgroup partials: FP64 contexts one per 128 texels, FP32 contexts one per
+// MT x 128 texels on the matrix cores (t->nwb of them either way)
+template <class T>
+static void launch_mlp_bwd(hush_train_t *t, TexU F0, TexU F1, TexU F2, TexF GH, const T *w, const T *o, const T *par,
+                           const T *gpar, double *part, hipStream_t s) {
+  if constexpr (sizeof(T) == 8)
+    hipLaunchKernelGGL(tk_mlp_bwd64, dim3(t->nwb), dim3(MB), 0, s, F0, F1, F2, GH, t->W2, t->H2, w, o, par, gpar, part);
+  else
+    hipLaunchKernelGGL(tk_mlp_bwd_mfma, dim3(t->nwb), dim3(MB), 0, s, F0, F1, F2, GH, t->W2, t->H2, w, o, par, gpar, part);
+}
+
 template <class T>
 static hipError_t backward_t(hush_train_t *t, const T *w, const T *gout, T *gw, T *ga0, hipStream_t s) {
   hush_t *h = t->h;
@@ -1245,19 +1364,20 @@ static hipError_t backward_t(hush_train_t *t, const T *w, const T *gout, T *gw, 
   hipLaunchKernelGGL((tk_filt<T, 1, 2, false, 1, false, true>), grid1(N2), dim3(NTH), 0, s, filt_args<T>(t, 2, (const T *)t->gmid[1], (T *)t->gmid[0]));
   hipLaunchKernelGGL((tk_filt_gather<T, 1, 2, false, 1>), grid1(N2), dim3(NTH), 0, s, filt_args<T>(t, 2, (const T *)t->gmid[1], (T *)t->gmid[0]));
   hipLaunchKernelGGL((tk_filt<T, 1, 1, true, 0, false, true>), grid1(N2), dim3(NTH), 0, s, filt_args<T>(t, 1, (const T *)t->gmid[0], nullptr));
-  hipLaunchKernelGGL(tk_mlp_bwd<T>, dim3(t->nwb), dim3(MB), 0, s, h->F[0], h->F[1], h->F[2], h->gh, t->W2, t->H2, w, (const T *)t->o,
-                     (const T *)t->par, (const T *)t->gpar, t->wpart);
+  launch_mlp_bwd<T>(t, h->F[0], h->F[1], h->F[2], h->gh, w, (const T *)t->o, (const T *)t->par, (const T *)t->gpar, t->wpart, s);
   int nb = t->nwb;
   if (chain) {
     // temporal blend -> the previous frame's alpha / gamma -> its MLP (features of that frame)
     hipLaunchKernelGGL(tk_zero<T>, grid1(N2 * 12), dim3(NTH), 0, s, (T *)t->gpprev, N2 * 12);
     hipLaunchKernelGGL(tk_blend_bwd<T>, grid1(N), dim3(NTH), 0, s, (const float *)t->aux, (const T *)t->pprev, (const T *)ga, (int)N,
                        (T *)t->gpprev);
-    hipLaunchKernelGGL(tk_mlp_bwd<T>, dim3(t->nwb), dim3(MB), 0, s, fprev_tex(t, 0), fprev_tex(t, 1), fprev_tex(t, 2), TexF{t->GHprev, t->W2},
-                       t->W2, t->H2, w, (const T *)t->oprev, (const T *)t->pprev, (const T *)t->gpprev, t->wpart + (size_t)t->nwb * NW);
+    launch_mlp_bwd<T>(t, fprev_tex(t, 0), fprev_tex(t, 1), fprev_tex(t, 2), TexF{t->GHprev, t->W2}, w, (const T *)t->oprev,
+                      (const T *)t->pprev, (const T *)t->gpprev, t->wpart + (size_t)t->nwb * NW, s);
     nb *= 2;
   }
-  hipLaunchKernelGGL(tk_wsum<T>, grid1(NW), dim3(NTH), 0, s, (const double *)t->wpart, nb, gw);
+  double *mid = t->wpart + (size_t)2 * t->nwb * NW;
+  hipLaunchKernelGGL(tk_wsum1, dim3((NW + 63) / 64, WSL), dim3(NTH), 0, s, (const double *)t->wpart, nb, mid);
+  hipLaunchKernelGGL(tk_wsum2<T>, dim3((NW + 63) / 64), dim3(NTH), 0, s, (const double *)mid, gw);
   return hipGetLastError();
 }
 
