@@ -55,7 +55,7 @@ template <bool KAHAN> struct Sum {
 };
 
 // BUF: the history gathers through buffer descriptors with 32-bit byte offsets (every history surface is
-// under 4 GiB, checked by the host), in place of 64-bit address arithmetic per tap.
+// under 2 GiB, checked by the host), in place of 64-bit address arithmetic per tap.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t hist_rsrc(const void *p, int pitch, int rows, int texel) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
                                            (int)((uint32_t)pitch * (uint32_t)rows * (uint32_t)texel), 0x00020000);
@@ -87,6 +87,24 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     mr = unpack4(mraw);
   }
   const bool use_hist = in_img && !a.reset && nd.w >= 0.0f;
+  // the 84 apron texels of the staged tile (threads 0..83): their normalDepth, radiance and material are
+  // loaded here, with the pixel's own, so that no load waits between the reprojection and the barrier
+  int atx = 0, aty = 0;
+  bool a_in = false;
+  uint2 a_nd = make_uint2(0u, 0u), a_rd = a_nd, a_mt = a_nd;
+  if (tid < 2 * SW + 2 * TH) {
+    if (tid < SW) { atx = tid; aty = 0; }
+    else if (tid < 2 * SW) { atx = tid - SW; aty = SH - 1; }
+    else if (tid < 2 * SW + TH) { atx = 0; aty = tid - 2 * SW + 1; }
+    else { atx = SW - 1; aty = tid - 2 * SW - TH + 1; }
+    const int qx = x0 - 1 + atx, qy = y0 - 1 + aty;
+    a_in = inside(qx, qy, a.W, a.H);
+    if (a_in) {
+      a_nd = a.nd.p[(size_t)qy * a.nd.pitch + qx];
+      a_rd = a.rad.p[(size_t)qy * a.rad.pitch + qx];
+      a_mt = a.mat.p[(size_t)qy * a.mat.pitch + qx];
+    }
+  }
 
   // reprojection fetches (hush.js:144-170): the four bilinear taps of the decoded key, colour and moments
   // and the nearest half-res par0 texel, issued before the tile staging so that their latency overlaps the
@@ -189,31 +207,27 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     }
     has = has && hm.w > 0.0f;
     if (has) mbits |= 0x20;
-  // tile staging (hush.js:287-294): each thread its own pixel, then 84 threads the 1-texel apron
+  // tile staging (hush.js:287-294): each thread its own pixel, threads 0..83 also an apron texel.
+  // radiance / max(albedo, 1e-3): the quotient through the exact reciprocal (binary16 numerators, divisors in
+  // [1e-3, 65504]: the domain where divs equals IEEE division, special operands included)
   {
     float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (in_img && nd.w >= 0.0f)
-      e = make_float4(F::div(rd.x, mx(mr.x, 1e-3f)), F::div(rd.y, mx(mr.y, 1e-3f)), F::div(rd.z, mx(mr.z, 1e-3f)), 1.0f);
+    if (in_img && nd.w >= 0.0f) {
+      const float bx = mx(mr.x, 1e-3f), by = mx(mr.y, 1e-3f), bz = mx(mr.z, 1e-3f);
+      e = make_float4(F::divs(rd.x, bx, F::rcpn(bx)), F::divs(rd.y, by, F::rcpn(by)), F::divs(rd.z, bz, F::rcpn(bz)), 1.0f);
+    }
     const int c = (ly + 1) * SW + lx + 1;
     const float l = lum(e.x, e.y, e.z);
     sI[c] = e; sL[c] = l; sQ[c] = E * qf16(l);
   }
   if (tid < 2 * SW + 2 * TH) {
-    int tx, ty;
-    if (tid < SW) { tx = tid; ty = 0; }
-    else if (tid < 2 * SW) { tx = tid - SW; ty = SH - 1; }
-    else if (tid < 2 * SW + TH) { tx = 0; ty = tid - 2 * SW + 1; }
-    else { tx = SW - 1; ty = tid - 2 * SW - TH + 1; }
-    const int qx = x0 - 1 + tx, qy = y0 - 1 + ty;
     float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (inside(qx, qy, a.W, a.H)) {
-      float4 qnd = ld(a.nd, qx, qy);
-      if (qnd.w >= 0.0f) {
-        float4 r = ld(a.rad, qx, qy), m = ld(a.mat, qx, qy);
-        e = make_float4(F::div(r.x, mx(m.x, 1e-3f)), F::div(r.y, mx(m.y, 1e-3f)), F::div(r.z, mx(m.z, 1e-3f)), 1.0f);
-      }
+    if (a_in && unpack2(a_nd.y).y >= 0.0f) {
+      const float4 r = unpack4(a_rd), m = unpack4(a_mt);
+      const float bx = mx(m.x, 1e-3f), by = mx(m.y, 1e-3f), bz = mx(m.z, 1e-3f);
+      e = make_float4(F::divs(r.x, bx, F::rcpn(bx)), F::divs(r.y, by, F::rcpn(by)), F::divs(r.z, bz, F::rcpn(bz)), 1.0f);
     }
-    const int c = ty * SW + tx;
+    const int c = aty * SW + atx;
     const float l = lum(e.x, e.y, e.z);
     sI[c] = e; sL[c] = l; sQ[c] = E * qf16(l);
   }
