@@ -79,16 +79,18 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   const bool in_img = inside(px, py, a.W, a.H);
 
   // this pixel's G-buffer (also its own staged texel)
-  float4 nd = make_float4(0.0f, 0.0f, 0.0f, -1.0f), mv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mr = mv, rd = mv;
-  uint2 mraw = make_uint2(0u, 0u);
+  // all four loads leave before any of them is waited for (the scheduler had put the first one's wait, and
+  // its unpacking, between it and the other three)
+  uint2 ndr = make_uint2(0u, 0xbc000000u), mvr = make_uint2(0u, 0u), rdr = mvr, mraw = mvr;   // nd = (0, 0, 0, -1)
   if (in_img) {
-    nd = ld(a.nd, px, py); mv = ld(a.mv, px, py); rd = ld(a.rad, px, py);
-    mraw = a.mat.p[(size_t)py * a.mat.pitch + px];
-    mr = unpack4(mraw);
+    const size_t o = (size_t)py;
+    ndr = a.nd.p[o * a.nd.pitch + px]; mvr = a.mv.p[o * a.mv.pitch + px]; rdr = a.rad.p[o * a.rad.pitch + px];
+    mraw = a.mat.p[o * a.mat.pitch + px];
+    __builtin_amdgcn_sched_barrier(0);
   }
-  const bool use_hist = in_img && !a.reset && nd.w >= 0.0f;
   // the 84 apron texels of the staged tile (threads 0..83): their normalDepth, radiance and material are
-  // loaded here, with the pixel's own, so that no load waits between the reprojection and the barrier
+  // loaded here, with the pixel's own and before any wait, so that no load waits between the reprojection
+  // and the barrier
   int atx = 0, aty = 0;
   bool a_in = false;
   uint2 a_nd = make_uint2(0u, 0u), a_rd = a_nd, a_mt = a_nd;
@@ -103,8 +105,11 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       a_nd = a.nd.p[(size_t)qy * a.nd.pitch + qx];
       a_rd = a.rad.p[(size_t)qy * a.rad.pitch + qx];
       a_mt = a.mat.p[(size_t)qy * a.mat.pitch + qx];
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
+  const float4 nd = unpack4(ndr), mv = unpack4(mvr), rd = unpack4(rdr), mr = unpack4(mraw);
+  const bool use_hist = in_img && !a.reset && nd.w >= 0.0f;
 
   // reprojection fetches (hush.js:144-170): the four bilinear taps of the decoded key, colour and moments
   // and the nearest half-res par0 texel, issued before the tile staging so that their latency overlaps the
@@ -1494,26 +1499,56 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
   const int x0 = org.x, y0 = org.y;
   const int lx = tid & 31, ly = tid >> 5;
   const int px = x0 + lx, py = y0 + ly;
+  // Loads first, waits later: the motion vector, the texel of X and an apron texel of the staged tile
+  // (threads 0..83), then, as soon as the motion vector is in, the four bilinear history taps, so that their
+  // latency overlaps the staging, its barrier and the 3x3 statistics.  Every address is clamped into the
+  // image and every lane loads (lanes outside the image or without history discard the values): loads
+  // under no lane-dependent branch let each wait count only the loads it needs.
+  const bool own = px < a.W && py < a.H, use_hist = own && !a.reset;
+  const int cpx = clampi(px, 0, a.W - 1), cpy = clampi(py, 0, a.H - 1);
+  const uint2 mvr = a.mv.p[(size_t)cpy * a.mv.pitch + cpx];
+  const uint2 xr = a.X.p[(size_t)cpy * a.X.pitch + cpx];
+  int atx = 0, aty = 0;
+  uint2 axr = make_uint2(0u, 0u);
+  if (tid < 2 * SW + 2 * TH) {
+    if (tid < SW) { atx = tid; aty = 0; }
+    else if (tid < 2 * SW) { atx = tid - SW; aty = SH - 1; }
+    else if (tid < 2 * SW + TH) { atx = 0; aty = tid - 2 * SW + 1; }
+    else { atx = SW - 1; aty = tid - 2 * SW - TH + 1; }
+    axr = a.X.p[(size_t)clampi(y0 - AP + aty, 0, a.H - 1) * a.X.pitch + clampi(x0 - AP + atx, 0, a.W - 1)];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  const float szx = (float)a.W, szy = (float)a.H;
+  const float2 mv = unpack2(mvr.x);
+  // (px + 0.5 + mv) / size with the host's correctly rounded 1 / size: equal to the quotient for every finite
+  // numerator; a non-finite one fails the range test below either way
+  const float ux = Math<false>::divs((float)px + 0.5f + mv.x, szx, a.rW), uy = Math<false>::divs((float)py + 0.5f + mv.y, szy, a.rH);
+  const bool in_uv = use_hist && ux >= 0.0f && uy >= 0.0f && ux <= 1.0f && uy <= 1.0f;
+  const float ppx = ux * szx - 0.5f, ppy = uy * szy - 0.5f;
+  const float bx = floorf(ppx), by = floorf(ppy);
+  const float fx = ppx - bx, fy = ppy - by;
+  uint2 hr4[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {       // f2i saturates (NaN -> 0) and the clamp keeps every tap in the image
+    const int qx = clampi(addw(f2i(bx), j & 1), 0, a.W - 1), qy = clampi(addw(f2i(by), j >> 1), 0, a.H - 1);
+    hr4[j] = a.prev.p[(size_t)qy * a.prev.pitch + qx];
+  }
+  __builtin_amdgcn_sched_barrier(0);
   // tile staging of to_ycocg(tm(X)), clamp to edge: each thread its own texel (whose tone-mapped colour it
-  // keeps), then 84 threads the 1-texel apron
-  const float4 xc = ld(a.X, clampi(px, 0, a.W - 1), clampi(py, 0, a.H - 1));
+  // keeps), threads 0..83 also an apron texel
+  const float4 xc = unpack4(xr);
   const float3 cur = tm(make_float3(xc.x, xc.y, xc.z));
   {
     const float3 y = to_ycocg(cur);
     sX[(ly + AP) * SW + lx + AP] = make_float4(y.x, y.y, y.z, 0.0f);
   }
   if (tid < 2 * SW + 2 * TH) {
-    int tx, ty;
-    if (tid < SW) { tx = tid; ty = 0; }
-    else if (tid < 2 * SW) { tx = tid - SW; ty = SH - 1; }
-    else if (tid < 2 * SW + TH) { tx = 0; ty = tid - 2 * SW + 1; }
-    else { tx = SW - 1; ty = tid - 2 * SW - TH + 1; }
-    const float4 x = ld(a.X, clampi(x0 - AP + tx, 0, a.W - 1), clampi(y0 - AP + ty, 0, a.H - 1));
+    const float4 x = unpack4(axr);
     const float3 y = to_ycocg(tm(make_float3(x.x, x.y, x.z)));
-    sX[ty * SW + tx] = make_float4(y.x, y.y, y.z, 0.0f);
+    sX[aty * SW + atx] = make_float4(y.x, y.y, y.z, 0.0f);
   }
   __syncthreads();
-  if (px >= a.W || py >= a.H) return;
+  if (!own) return;
   if (a.reset) {
     float3 r = itm(cur);
     st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
@@ -1537,22 +1572,14 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
   }
   const float sdx = F::sqrtn(mx(F::divs(vx, 9.0f, r9), 1e-12f)), sdy = F::sqrtn(mx(F::divs(vy, 9.0f, r9), 1e-12f)),
               sdz = F::sqrtn(mx(F::divs(vz, 9.0f, r9), 1e-12f));
-  const float4 mv = ld(a.mv, px, py);
-  const float szx = (float)a.W, szy = (float)a.H;
-  // (px + 0.5 + mv) / size with the host's correctly rounded 1 / size: equal to the quotient for every finite
-  // numerator; a non-finite one fails the range test below either way
-  const float ux = Math<false>::divs((float)px + 0.5f + mv.x, szx, a.rW), uy = Math<false>::divs((float)py + 0.5f + mv.y, szy, a.rH);
   float3 res = cur;
-  if (ux >= 0.0f && uy >= 0.0f && ux <= 1.0f && uy <= 1.0f) {
-    float ppx = ux * szx - 0.5f, ppy = uy * szy - 0.5f;
-    float bx = floorf(ppx), by = floorf(ppy);
-    float fx = ppx - bx, fy = ppy - by;
+  if (in_uv) {
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       int ox = j & 1, oy = j >> 1;
       float wb = (ox == 1 ? fx : 1.0f - fx) * (oy == 1 ? fy : 1.0f - fy);
-      float4 t = ld(a.prev, clampi(addw(f2i(bx), ox), 0, a.W - 1), clampi(addw(f2i(by), oy), 0, a.H - 1));
+      const float4 t = unpack4(hr4[j]);
       hx += wb * t.x; hy += wb * t.y; hz += wb * t.z;
     }
     float3 hh = to_ycocg(tm(make_float3(hx, hy, hz)));
