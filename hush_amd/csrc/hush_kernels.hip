@@ -113,12 +113,14 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
 
   // reprojection fetches (hush.js:144-170): the four bilinear taps of the decoded key, colour and moments
   // and the nearest half-res par0 texel, issued before the tile staging so that their latency overlaps the
-  // staging, its barrier and the 3x3 statistics; colour, moments and par0 stay packed until used
+  // staging's arithmetic; colour, moments and par0 stay packed until used
   float fx = 0.0f, fy = 0.0f;
   int bix = 0, biy = 0, cx = 0, cy = 0;
   float4 k4[4];
   uint2 c4r[4], m4r[4], pqr = make_uint2(0u, 0u);
-  if (use_hist) {
+  // every lane gathers (the taps are clamped into the surfaces, so a lane without history reads valid
+  // texels it then ignores): with no branch around them, each later wait counts only the loads it needs
+  {
     const float ppx = (float)px + mv.x, ppy = (float)py + mv.y;
     const float bx = floorf(ppx), by = floorf(ppy);
     fx = ppx - bx; fy = ppy - by;
@@ -156,7 +158,34 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     }
   }
 
-  // reprojection (hush.js:144-187) ahead of the staging: it needs no LDS, and consuming the gathered taps
+  // the staging's arithmetic runs while the gathers are in flight (its inputs arrived with the first loads)
+  __builtin_amdgcn_sched_barrier(0);
+  // tile staging (hush.js:287-294): each thread its own pixel, threads 0..83 also an apron texel.
+  // radiance / max(albedo, 1e-3): the quotient through the exact reciprocal (binary16 numerators, divisors in
+  // [1e-3, 65504]: the domain where divs equals IEEE division, special operands included)
+  {
+    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (in_img && nd.w >= 0.0f) {
+      const float bx = mx(mr.x, 1e-3f), by = mx(mr.y, 1e-3f), bz = mx(mr.z, 1e-3f);
+      e = make_float4(F::divs(rd.x, bx, F::rcpn(bx)), F::divs(rd.y, by, F::rcpn(by)), F::divs(rd.z, bz, F::rcpn(bz)), 1.0f);
+    }
+    const int c = (ly + 1) * SW + lx + 1;
+    const float l = lum(e.x, e.y, e.z);
+    sI[c] = e; sL[c] = l; sQ[c] = E * qf16(l);
+  }
+  if (tid < 2 * SW + 2 * TH) {
+    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (a_in && unpack2(a_nd.y).y >= 0.0f) {
+      const float4 r = unpack4(a_rd), m = unpack4(a_mt);
+      const float bx = mx(m.x, 1e-3f), by = mx(m.y, 1e-3f), bz = mx(m.z, 1e-3f);
+      e = make_float4(F::divs(r.x, bx, F::rcpn(bx)), F::divs(r.y, by, F::rcpn(by)), F::divs(r.z, bz, F::rcpn(bz)), 1.0f);
+    }
+    const int c = aty * SW + atx;
+    const float l = lum(e.x, e.y, e.z);
+    sI[c] = e; sL[c] = l; sQ[c] = E * qf16(l);
+  }
+
+  // reprojection (hush.js:144-187) ahead of the staging barrier: it needs no LDS, and consuming the gathered taps
   // here keeps only its results live across the staging barrier and the 3x3 statistics
   const float3 n = make_float3(nd.x, nd.y, nd.z);
   uint32_t mbits = 0;
@@ -212,30 +241,6 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     }
     has = has && hm.w > 0.0f;
     if (has) mbits |= 0x20;
-  // tile staging (hush.js:287-294): each thread its own pixel, threads 0..83 also an apron texel.
-  // radiance / max(albedo, 1e-3): the quotient through the exact reciprocal (binary16 numerators, divisors in
-  // [1e-3, 65504]: the domain where divs equals IEEE division, special operands included)
-  {
-    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (in_img && nd.w >= 0.0f) {
-      const float bx = mx(mr.x, 1e-3f), by = mx(mr.y, 1e-3f), bz = mx(mr.z, 1e-3f);
-      e = make_float4(F::divs(rd.x, bx, F::rcpn(bx)), F::divs(rd.y, by, F::rcpn(by)), F::divs(rd.z, bz, F::rcpn(bz)), 1.0f);
-    }
-    const int c = (ly + 1) * SW + lx + 1;
-    const float l = lum(e.x, e.y, e.z);
-    sI[c] = e; sL[c] = l; sQ[c] = E * qf16(l);
-  }
-  if (tid < 2 * SW + 2 * TH) {
-    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (a_in && unpack2(a_nd.y).y >= 0.0f) {
-      const float4 r = unpack4(a_rd), m = unpack4(a_mt);
-      const float bx = mx(m.x, 1e-3f), by = mx(m.y, 1e-3f), bz = mx(m.z, 1e-3f);
-      e = make_float4(F::divs(r.x, bx, F::rcpn(bx)), F::divs(r.y, by, F::rcpn(by)), F::divs(r.z, bz, F::rcpn(bz)), 1.0f);
-    }
-    const int c = aty * SW + atx;
-    const float l = lum(e.x, e.y, e.z);
-    sI[c] = e; sL[c] = l; sQ[c] = E * qf16(l);
-  }
   __syncthreads();
 
   // feature inputs go straight to LDS (sky / outside: zeros with depth -1, hush.js:112, 297)
