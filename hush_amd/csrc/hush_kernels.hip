@@ -220,11 +220,30 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         float n3 = 0.0f;
         hcx = hcy = hcz = 0.0f;
         hm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // the nine keys in one batch of loads (clamped; a texel outside the image fails `inside`), then the
+        // colour and moments of the consistent ones in a second batch
+        float4 kq[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++)
+          kq[j] = ldf(a.pkd, clampi(addw(cx, j % 3 - 1), 0, W - 1), clampi(addw(cy, j / 3 - 1), 0, H - 1));
+        uint32_t okm = 0;
+#pragma unroll
+        for (int j = 0; j < 9; j++)
+          if (inside(addw(cx, j % 3 - 1), addw(cy, j / 3 - 1), W, H) && check(kq[j], n, mv.w, mv.z)) okm |= 1u << j;
+        uint2 cq[9], mq[9];
+#pragma unroll
         for (int j = 0; j < 9; j++) {
-          int qx = addw(cx, j % 3 - 1), qy = addw(cy, j / 3 - 1);
-          if (inside(qx, qy, W, H) && check(ldf(a.pkd, qx, qy), n, mv.w, mv.z)) {
+          cq[j] = mq[j] = make_uint2(0u, 0u);
+          if (okm >> j & 1u) {
+            const int qx = addw(cx, j % 3 - 1), qy = addw(cy, j / 3 - 1);
+            cq[j] = a.pcol.p[(size_t)qy * a.pcol.pitch + qx]; mq[j] = a.pmom.p[(size_t)qy * a.pmom.pitch + qx];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 9; j++) {
+          if (okm >> j & 1u) {
             n3 += 1.0f;
-            float4 cc = ld(a.pcol, qx, qy), mm = ld(a.pmom, qx, qy);
+            const float4 cc = unpack4(cq[j]), mm = unpack4(mq[j]);
             hcx += cc.x; hcy += cc.y; hcz += cc.z;
             hm.x += mm.x; hm.y += mm.y; hm.z += mm.z; hm.w += mm.w;
           }
