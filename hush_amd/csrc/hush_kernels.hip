@@ -1379,41 +1379,76 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   const int2 org = tile_origin<TW, TH>(a.W, a.H);
   const int x0 = org.x, y0 = org.y;
   const int hx0 = (x0 >> 1) - 1, hy0 = (y0 >> 1) - 1, qx0 = (x0 >> 2) - 1, qy0 = (y0 >> 2) - 1;
-  for (int i = tid; i < SN; i += NT) {
-    int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
-    const float4 a0 = ld(a.A0, qx, qy);
-    sA[i] = a0;
-    sL[i] = lum(a0.x, a0.y, a0.z);
-    const float4 kd = ldf(a.kd, qx, qy);
-    sG[i] = make_float4(kd.x, kd.y, kd.z, pair_of(kd.w).x);
+  const int lx = tid & 31, ly = tid >> 5;
+  const int px = x0 + lx, py = y0 + ly;
+  // Every load of the tile is issued before the first wait: the staged A0 / key texels (two per thread for
+  // threads 0..83), one window texel (threads 0..211), and the pixel's own material and emission, which the
+  // pixel phase after the barrier reads.
+  uint2 a0r[2];
+  float4 kdr[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int i = tid + k * NT;
+    a0r[k] = make_uint2(0u, 0u); kdr[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < SN) {
+      const int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
+      a0r[k] = a.A0.p[(size_t)qy * a.A0.pitch + qx];
+      kdr[k] = a.kd.p[(size_t)qy * a.kd.pitch + qx];
+    }
   }
   // windows: texels outside the level are staged clamped and never read (the taps skip them)
+  float4 w4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint4 wu = make_uint4(0u, 0u, 0u, 0u);
+  uint2 wm = make_uint2(0u, 0u), wp1 = wm, wp2 = wm;
   if (tid < HWN) {
     const int qx = clampi(hx0 + tid % HWW, 0, a.W2 - 1), qy = clampi(hy0 + tid / HWW, 0, a.H2 - 1);
-    const float4 gh = ldf(a.GH, qx, qy);
-    const uint4 g2 = ldu(a.F2, qx, qy);
-    const float4 m = ld(a.mid, qx, qy);
-    const float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
-    sHG[tid] = make_float4(gh.x, gh.y, gh.z, pair_of(gh.w).x);
+    w4 = a.GH.p[(size_t)qy * a.GH.pitch + qx];
+    wu = a.F2.p[(size_t)qy * a.F2.pitch + qx];
+    wm = a.mid.p[(size_t)qy * a.mid.pitch + qx];
+  } else if (tid < HWN + QWN) {
+    const int i = tid - HWN;
+    const int qx = clampi(qx0 + i % QWW, 0, a.W4 - 1), qy = clampi(qy0 + i / QWW, 0, a.H4 - 1);
+    w4 = a.cgd.p[(size_t)qy * a.cgd.pitch + qx];
+    wm = a.coarse.p[(size_t)qy * a.coarse.pitch + qx];
+  } else if (tid < HWN + QWN + PWN) {
+    const int i = tid - HWN - QWN;
+    const int qx = clampi((x0 >> 1) + i % (TW / 2), 0, a.W2 - 1), qy = clampi((y0 >> 1) + i / (TW / 2), 0, a.H2 - 1);
+    wm = a.par0.p[(size_t)qy * a.par0.pitch + qx];
+    wp1 = a.par1.p[(size_t)qy * a.par1.pitch + qx];
+    wp2 = a.par2.p[(size_t)qy * a.par2.pitch + qx];
+  }
+  const int cpx = clampi(px, 0, a.W - 1), cpy = clampi(py, 0, a.H - 1);
+  const uint2 albr = a.mat.p[(size_t)cpy * a.mat.pitch + cpx], emir = a.emi.p[(size_t)cpy * a.emi.pitch + cpx];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int i = tid + k * NT;
+    if (i < SN) {
+      const float4 a0 = unpack4(a0r[k]);
+      sA[i] = a0;
+      sL[i] = lum(a0.x, a0.y, a0.z);
+      sG[i] = make_float4(kdr[k].x, kdr[k].y, kdr[k].z, pair_of(kdr[k].w).x);
+    }
+  }
+  if (tid < HWN) {
+    const float4 m = unpack4(wm);
+    const float2 a01 = unpack2(wu.z), a23 = unpack2(wu.w);
+    sHG[tid] = make_float4(w4.x, w4.y, w4.z, pair_of(w4.w).x);
     sHM[tid] = make_float4(m.x, m.y, m.z, a01.x);
     sHA[tid] = make_float2(a01.y, a23.x);
   } else if (tid < HWN + QWN) {
     const int i = tid - HWN;
-    const int qx = clampi(qx0 + i % QWW, 0, a.W4 - 1), qy = clampi(qy0 + i / QWW, 0, a.H4 - 1);
-    sQG[i] = ldf(a.cgd, qx, qy);
-    sQC[i] = ld(a.coarse, qx, qy);
+    sQG[i] = w4;
+    sQC[i] = unpack4(wm);
   } else if (tid < HWN + QWN + PWN) {
     const int i = tid - HWN - QWN;
-    const int qx = clampi((x0 >> 1) + i % (TW / 2), 0, a.W2 - 1), qy = clampi((y0 >> 1) + i / (TW / 2), 0, a.H2 - 1);
-    const float4 p0 = ld(a.par0, qx, qy);
-    sPar[i] = make_float4(ld(a.par1, qx, qy).x, ld(a.par2, qx, qy).x, p0.z, p0.w);
+    const float4 p0 = unpack4(wm);
+    sPar[i] = make_float4(unpack4(wp1).x, unpack4(wp2).x, p0.z, p0.w);
   }
   __syncthreads();
-  const int lx = tid & 31, ly = tid >> 5;
-  const int px = x0 + lx, py = y0 + ly;
   if (!inside(px, py, a.W, a.H)) return;
   const int tx = lx + AP, ty = ly + AP;
-  const float4 alb = ld(a.mat, px, py), emi = ld(a.emi, px, py);
+  const float4 alb = unpack4(albr), emi = unpack4(emir);
   const float4 gp = sG[ty * SW + tx];
   if (gp.w < 0.0f) {
     st(a.outHist, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
