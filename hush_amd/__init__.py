@@ -103,7 +103,8 @@ class Hush:
 
     `device` is a HIP device ordinal, or an object with `ordinal` and an optional `limits` mapping that is
     checked against `Hush.requiredLimits` like the reference.  `precision` selects the MLP arithmetic
-    ("fp32" drop-in, "f16", "fp16-mfma", "bf16-mfma", "fp8-mfma"); `f16=True` means "f16".
+    ("fp32" drop-in, "f16", "fp16-mfma", "bf16-mfma", "fp8-mfma"); `f16=True` means "f16".  `fp8_scales`
+    sets the fp8 form's static per-layer input scales (see `fp8_scales`).
     """
 
     requiredLimits = MappingProxyType({"maxStorageTexturesPerShaderStage": 8})
@@ -140,6 +141,12 @@ class Hush:
             self.ieee_math = False
         if compensated:
             self.compensated = True
+        fp8 = opts.get("fp8_scales")
+        if fp8 is None and self.hidden == 16:
+            from .quant import load_scales
+            fp8 = load_scales(w)
+        if fp8 is not None:
+            self.fp8_scales = fp8
         self._timer_on = False
 
     # ---- helpers
@@ -196,6 +203,22 @@ class Hush:
     @precision.setter
     def precision(self, p) -> None:
         self._check(self._L.hush_set_precision(self._h, PRECISIONS[p] if isinstance(p, str) else int(p)))
+
+    @property
+    def fp8_scales(self) -> tuple:
+        """the fp8 form's static input scales of the three layers (powers of two): each layer's input x is
+        stored as the E4M3 value of x / s.  Defaults: the calibration stored for the built-in weights, else
+        bounds derived from the weights (hush_create)."""
+        s = (ctypes.c_float * 3)()
+        self._check(self._L.hush_get_fp8_scales(self._h, s))
+        return tuple(float(v) for v in s)
+
+    @fp8_scales.setter
+    def fp8_scales(self, scales) -> None:
+        v = [float(x) for x in scales]
+        if len(v) != 3:
+            raise HushError("fp8_scales: three scales, one per layer")
+        self._check(self._L.hush_set_fp8_scales(self._h, (ctypes.c_float * 3)(*v)))
 
     @property
     def ieee_math(self) -> bool:

@@ -15,6 +15,7 @@ SCENE_LIB_PATH = os.path.join(HERE, "libhush_scene.so")
 EXPORTS = [
     "hush_hidden_of", "hush_create", "hush_destroy", "hush_resize", "hush_reset", "hush_set_exposure",
     "hush_get_exposure", "hush_get_size", "hush_hidden", "hush_frame", "hush_set_precision", "hush_get_precision",
+    "hush_set_fp8_scales", "hush_get_fp8_scales",
     "hush_set_ieee_math", "hush_get_ieee_math", "hush_set_compensated", "hush_get_compensated", "hush_mlp_forward",
     "hush_denoise", "hush_encode", "hush_output", "hush_last_error", "hush_set_timing", "hush_pass_times",
     "hush_debug_surface", "hush_debug_set_state", "hush_debug_run_pass", "hush_set_debug_mask",
@@ -101,6 +102,8 @@ def load() -> ctypes.CDLL:
     L.hush_frame.argtypes = [P]; L.hush_frame.restype = ctypes.c_int64
     L.hush_set_precision.argtypes = [P, i32]; L.hush_set_precision.restype = i32
     L.hush_get_precision.argtypes = [P]; L.hush_get_precision.restype = i32
+    L.hush_set_fp8_scales.argtypes = [P, ctypes.POINTER(f32)]; L.hush_set_fp8_scales.restype = i32
+    L.hush_get_fp8_scales.argtypes = [P, ctypes.POINTER(f32)]; L.hush_get_fp8_scales.restype = i32
     L.hush_set_ieee_math.argtypes = [P, i32]; L.hush_set_ieee_math.restype = i32
     L.hush_get_ieee_math.argtypes = [P]; L.hush_get_ieee_math.restype = i32
     L.hush_set_compensated.argtypes = [P, i32]; L.hush_set_compensated.restype = i32

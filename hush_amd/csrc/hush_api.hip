@@ -1,5 +1,6 @@
 // Host side of the Hush C ABI (include/hush.h): context, surface arena, pass sequencing.
 // Mirrors the Hush class of src/denoise/hush.js:725-875 with HIP device buffers in place of GPUTextures.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -57,14 +58,44 @@ static bool precision_built(int p, int hidden) {
   return p == HUSH_MLP_FP32 || p == HUSH_MLP_F16 || ((p == HUSH_MLP_FP16_MFMA || p == HUSH_MLP_BF16_MFMA || p == HUSH_MLP_FP8_MFMA) && hidden == 16);
 }
 
-// smallest power of two s with max|w| / s <= 448 (the largest fp8 E4M3 value)
-static float fp8_scale(const float *w, int n) {
-  float m = 0.0f;
-  for (int i = 0; i < n; i++) m = std::fmax(m, std::fabs(w[i]));
-  if (!(m > 0.0f)) return 1.0f;
+// fp8 input scales are powers of two in [2^-20, 2^20] (so 448 s is an exact binary16 value below 2^8 s)
+static const int FP8_SCALE_EXP = 20;
+// smallest power of two s with m / s <= 448 (the largest fp8 E4M3 value), within the scale range; 1 for m == 0
+static float pow2_over_448(double m) {
+  if (!(m > 0.0)) return 1.0f;
   int e;
-  std::frexp(m / 448.0f, &e);
-  return std::ldexp(1.0f, e);
+  std::frexp(m / 448.0, &e);
+  return std::ldexp(1.0f, std::min(std::max(e, -FP8_SCALE_EXP), FP8_SCALE_EXP));
+}
+static float fp8_scale(const float *w, int n) {
+  double m = 0.0;
+  for (int i = 0; i < n; i++) m = std::fmax(m, std::fabs((double)w[i]));
+  return pow2_over_448(m);
+}
+// Input scales of the fp8 form from the weights alone: features taken as bounded by 16 in magnitude (the
+// logarithmic features of binary16 G-buffer values at unit exposure stay below log(1 + 65504) = 11.1, the
+// normal, depth and count features below 2; material inputs and extreme exposures can exceed it, and such
+// inputs saturate at 448 sx), each hidden unit by |b| + sum |w| x (the bound of its inputs)
+static void fp8_default_input_scales(const float *w, float *sx) {
+  const float *W1 = w, *B1 = w + 320, *W2 = w + 336, *B2 = w + 592;
+  const double fb = 16.0;
+  double m1 = 0.0, m2 = 0.0;
+  for (int i = 0; i < 16; i++) {
+    double a = std::fabs((double)B1[i]);
+    for (int j = 0; j < 20; j++) a += std::fabs((double)W1[i * 20 + j]) * fb;
+    m1 = std::fmax(m1, a);
+  }
+  for (int i = 0; i < 16; i++) {
+    double a = std::fabs((double)B2[i]);
+    for (int j = 0; j < 16; j++) a += std::fabs((double)W2[i * 16 + j]) * m1;
+    m2 = std::fmax(m2, a);
+  }
+  sx[0] = pow2_over_448(fb); sx[1] = pow2_over_448(m1); sx[2] = pow2_over_448(m2);
+}
+static bool fp8_scale_ok(float x) {
+  if (!(x > 0.0f) || !std::isfinite(x)) return false;
+  int e;
+  return std::frexp(x, &e) == 0.5f && e - 1 >= -FP8_SCALE_EXP && e - 1 <= FP8_SCALE_EXP;
 }
 
 extern "C" int32_t hush_hidden_of(size_t n) {
@@ -175,10 +206,11 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
     return HUSH_ERR_DEVICE;
   }
   if (hid == 16) {   // fragment-order weights of the MFMA forms
+    h->fp8_sw[0] = fp8_scale(wf.data(), 320); h->fp8_sw[1] = fp8_scale(wf.data() + 336, 256);
+    h->fp8_sw[2] = fp8_scale(wf.data() + 608, 192);
+    fp8_default_input_scales(wf.data(), h->fp8_sx);
     e = hipMalloc(&h->d_wq, sizeof(MfmaPack));
-    if (e == hipSuccess)
-      e = launch_pack_mlp(h->d_w, (MfmaPack *)h->d_wq, fp8_scale(wf.data(), 320), fp8_scale(wf.data() + 336, 256),
-                          fp8_scale(wf.data() + 608, 192), nullptr);
+    if (e == hipSuccess) e = launch_pack_mlp(h->d_w, (MfmaPack *)h->d_wq, h->fp8_sw, h->fp8_sx, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
       hip_fail(nullptr, e, "MFMA weight packing");
@@ -250,6 +282,29 @@ extern "C" int32_t hush_get_precision(const hush_t *h) { return h ? h->precision
 extern "C" int32_t hush_set_ieee_math(hush_t *h, int32_t on) {
   if (!h) return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_set_ieee_math: null context");
   h->ieee = on != 0;
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_set_fp8_scales(hush_t *h, const float *scales) {
+  if (!h || !scales) return fail(h, HUSH_ERR_ARGUMENT, "hush_set_fp8_scales: null argument");
+  if (!h->d_wq) return fail(h, HUSH_ERR_UNSUPPORTED, "hush_set_fp8_scales: the fp8 form needs hidden width 16");
+  for (int i = 0; i < 3; i++)
+    if (!fp8_scale_ok(scales[i]))
+      return fail(h, HUSH_ERR_ARGUMENT, "hush_set_fp8_scales: scale %d (%g) is not a power of two in [2^-20, 2^20]", i,
+                  (double)scales[i]);
+  hipError_t e = hipSetDevice(h->device);
+  if (e == hipSuccess) e = hipDeviceSynchronize();               // no frame in flight reads the old pack
+  if (e != hipSuccess) return hip_fail(h, e, "hush_set_fp8_scales");
+  for (int i = 0; i < 3; i++) h->fp8_sx[i] = scales[i];
+  e = launch_pack_mlp(h->d_w, (MfmaPack *)h->d_wq, h->fp8_sw, h->fp8_sx, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(h, e, "hush_set_fp8_scales: weight packing");
+  return HUSH_OK;
+}
+
+extern "C" int32_t hush_get_fp8_scales(const hush_t *h, float *scales) {
+  if (!h || !scales) return HUSH_ERR_ARGUMENT;
+  for (int i = 0; i < 3; i++) scales[i] = h->fp8_sx[i];
   return HUSH_OK;
 }
 extern "C" int32_t hush_get_ieee_math(const hush_t *h) { return h ? (h->ieee ? 1 : 0) : -1; }
@@ -438,7 +493,8 @@ extern "C" int32_t hush_mlp_forward(hush_t *h, int32_t precision, const void *fe
   if (h->hidden != 16 || !precision_built(precision, h->hidden))
     return fail(h, HUSH_ERR_UNSUPPORTED, "hush_mlp_forward: built for hidden width 16 (this blob: %d)", h->hidden);
   if (n < 0 || n > (int64_t)1 << 30) return fail(h, HUSH_ERR_ARGUMENT, "hush_mlp_forward: bad row count %lld", (long long)n);
-  if (((uintptr_t)features & 3) != 0 || ((uintptr_t)out & 3) != 0) return fail(h, HUSH_ERR_ARGUMENT, "hush_mlp_forward: pointers must be 4-byte aligned");
+  if (((uintptr_t)features & 7) != 0 || ((uintptr_t)out & 15) != 0)
+    return fail(h, HUSH_ERR_ARGUMENT, "hush_mlp_forward: features must be 8-byte and outputs 16-byte aligned");
   if (n == 0) return HUSH_OK;
   hipError_t e = hipSetDevice(h->device);
   if (e == hipSuccess) e = launch_mlp(precision, (const uint32_t *)features, out, (int)n, h->d_w, h->d_wq, (hipStream_t)stream);

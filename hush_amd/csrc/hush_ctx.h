@@ -25,6 +25,8 @@ struct hush_ctx {
   float *d_w = nullptr;
   void *d_wq = nullptr;
   float *d_wscale = nullptr;
+  float fp8_sw[3] = {1.0f, 1.0f, 1.0f};   // fp8 form: per-layer weight scales (from the weights)
+  float fp8_sx[3] = {1.0f, 1.0f, 1.0f};   // fp8 form: per-layer static input scales
   void *arena = nullptr;
   hush::Tex tex[S_COUNT] = {};
   hush::TexU F[4] = {};

@@ -11,11 +11,17 @@ enum MlpMode { MLP_FP32 = 0, MLP_F16 = 1, MLP_FP16_MFMA = 2, MLP_BF16_MFMA = 3, 
 
 // weight fragments of the MFMA MLP forms in lane order (hidden width 16), packed on the device
 struct MfmaPack {
-  uint2 h[4][64];          // fp16: L1 k 0-15, L1 k 16-31, L2, L3 (4 x 16-bit per lane)
-  uint2 b[4][64];          // bf16, same order
-  uint2 q[3][64];          // fp8 E4M3: L1, L2, L3, k = 8(l/16) .. +7 (8 x 8-bit per lane)
+  uint4 h1[64];            // fp16 layer 1 for 16x16x32: unit l%16, k = 8(l/16) .. +7 (8 x 16-bit per lane)
+  uint4 b1w[64];           // bf16 layer 1, same order
+  uint2 h[2][64];          // fp16 layers 2, 3 for 16x16x16: k = 4(l/16) .. +3 (4 x 16-bit per lane)
+  uint2 b[2][64];          // bf16 layers 2, 3
+  uint2 q[3][64];          // fp8 E4M3: L1 k = 8(l/16) .. +7; L2, L3 units 4(l/16) .. +3 then 4 zeros
   float b1[16], b2[16], b3[16];
   float sw[3];             // fp8 weight scale per layer (power of two)
+  float sx[3];             // fp8 static scale of each layer's input (power of two)
+  uint32_t xclamp;         // binary16 pair (448 sx[0], 448 sx[0]), capped at 65504: the layer-1 input clamp
+  float qc[3];             // sx[L] sw[L] / sx[L+1] (L < 2), sx[2] sw[2]: E4M3 sums -> next E4M3 input / output
+  float qb1[16], qb2[16];  // b1 / sx[1], b2 / sx[2]
 };
 
 constexpr int TEMPORAL_AUX = 20;
@@ -95,7 +101,7 @@ hipError_t launch_coarse1_infer(const Coarse1Args &c, const InferArgs &a, int mo
 hipError_t launch_coarse2_mid3(const AtrousArgs &c, const AtrousArgs &m, bool fast, hipStream_t s);
 hipError_t launch_fine_final(const FineArgs &a, bool fast, hipStream_t s);
 hipError_t launch_taa(const TaaArgs &a, bool fast, hipStream_t s);
-hipError_t launch_pack_mlp(const float *w, MfmaPack *p, float s1, float s2, float s3, hipStream_t s);
+hipError_t launch_pack_mlp(const float *w, MfmaPack *p, const float sw[3], const float sx[3], hipStream_t s);
 hipError_t launch_mlp(int mode, const uint32_t *feat, float *out, int n, const float *w, const void *wq, hipStream_t s);
 
 }  // namespace hush

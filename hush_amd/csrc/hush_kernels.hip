@@ -675,24 +675,22 @@ __global__ __launch_bounds__(NT) void k_infer(InferArgs a) {
 // ============================================================================ MFMA forms of the MLP
 // The MLP 20-16-16-12 of 64 texels per wave, in four groups of 16 texels.  Each layer is D = W . X^T on a
 // 16x16 MFMA tile: A = weight fragment (16 units x K), B = activations (K x 16 texels), D = 16 units x 16
-// texels with fp32 accumulation.  With the 16x16x16 f16 / bf16 shapes, lane l holds D[4(l/16)+i][l%16],
-// which is exactly the B fragment (rows k = 4(l/16)+i, column l%16) of the next layer: hidden activations
-// stay in registers.  The fp8 form uses the 16x16x32 shape (8 values per lane, k = 8(l/16)+i) and
-// regroups the hidden values with lane shuffles.  Weights are the hidden-width-16 blob only.
+// texels with fp32 accumulation; lane l holds D[4(l/16)+i][l%16].  Weights are the hidden-width-16 blob only.
 
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint32_t pack_h2(float a, float b) {       // binary16 pair, round to nearest even
   half2v h = {(_Float16)a, (_Float16)b};
   return __builtin_bit_cast(uint32_t, h);
 }
-__device__ __forceinline__ uint32_t bf16_rne(float x) {               // bfloat16 bits, round to nearest even
-  uint32_t u = __float_as_uint(x);
-  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf2(float a, float b) {      // bfloat16 pair, round to nearest even
+  const bf16x2v h = {(__bf16)a, (__bf16)b};
+  return __builtin_bit_cast(uint32_t, h);
 }
-__device__ __forceinline__ uint32_t pack_bf2(float a, float b) { return bf16_rne(a) | (bf16_rne(b) << 16); }
 // four floats -> four fp8 (E4M3) bytes, saturated to +-448, round to nearest even
 __device__ __forceinline__ uint32_t pack_fp8x4(float a, float b, float c, float d) {
   const float M = 448.0f;
@@ -700,49 +698,52 @@ __device__ __forceinline__ uint32_t pack_fp8x4(float a, float b, float c, float 
   r = __builtin_amdgcn_cvt_pk_fp8_f32(fminf(fmaxf(c, -M), M), fminf(fmaxf(d, -M), M), r, true);
   return (uint32_t)r;
 }
-// smallest power of two s with max / s <= 448 (1 for max == 0)
-__device__ __forceinline__ float pow2_scale(float m) {
-  if (!(m > 0.0f)) return 1.0f;
-  int e;
-  frexpf(m / 448.0f, &e);
-  return ldexpf(1.0f, e);
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-  return v;
-}
-
-// one lane per weight-fragment slot: converts the fp32-widened blob into the three fragment sets with the
-// conversion instructions the MFMAs read, so the formats agree by construction
-__global__ __launch_bounds__(64) void k_pack_mlp(const float *__restrict__ w, MfmaPack *p, float s1, float s2, float s3) {
+// one lane per weight-fragment slot: converts the fp32-widened blob into the fragment sets with the
+// conversion instructions the MFMAs read, so the formats agree by construction.  sw: the fp8 weight scale of
+// each layer; sx: the fp8 static activation scale of each layer's input (powers of two).
+__global__ __launch_bounds__(64) void k_pack_mlp(const float *__restrict__ w, MfmaPack *p, float sw1, float sw2,
+                                                 float sw3, float sx1, float sx2, float sx3) {
   const int l = threadIdx.x, m = l & 15, kb = l >> 4;
   const float *W1 = w, *B1 = W1 + 320, *W2 = B1 + 16, *B2 = W2 + 256, *W3 = B2 + 16, *B3 = W3 + 192;
   auto w1 = [&](int mm, int k) { return k < 20 ? W1[mm * 20 + k] : 0.0f; };
   auto w2 = [&](int mm, int k) { return k < 16 ? W2[mm * 16 + k] : 0.0f; };
   auto w3 = [&](int mm, int k) { return (mm < 12 && k < 16) ? W3[mm * 16 + k] : 0.0f; };
-  float v[4][4];
+  float v1[8], v[2][4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    v[0][i] = w1(m, 4 * kb + i); v[1][i] = w1(m, 16 + 4 * kb + i);
-    v[2][i] = w2(m, 4 * kb + i); v[3][i] = w3(m, 4 * kb + i);
-  }
+  for (int i = 0; i < 8; i++) v1[i] = w1(m, 8 * kb + i);
 #pragma unroll
-  for (int b = 0; b < 4; b++) {
+  for (int i = 0; i < 4; i++) { v[0][i] = w2(m, 4 * kb + i); v[1][i] = w3(m, 4 * kb + i); }
+  p->h1[l] = make_uint4(pack_h2(v1[0], v1[1]), pack_h2(v1[2], v1[3]), pack_h2(v1[4], v1[5]), pack_h2(v1[6], v1[7]));
+  p->b1w[l] = make_uint4(pack_bf2(v1[0], v1[1]), pack_bf2(v1[2], v1[3]), pack_bf2(v1[4], v1[5]), pack_bf2(v1[6], v1[7]));
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
     p->h[b][l] = make_uint2(pack_h2(v[b][0], v[b][1]), pack_h2(v[b][2], v[b][3]));
     p->b[b][l] = make_uint2(pack_bf2(v[b][0], v[b][1]), pack_bf2(v[b][2], v[b][3]));
   }
-  const float r1 = 1.0f / s1, r2 = 1.0f / s2, r3 = 1.0f / s3;     // powers of two: exact
+  // fp8: layer 1 takes k = 8kb .. 8kb+7 as the f16 form does; layers 2 and 3 put hidden unit 4kb+j at
+  // k = 8kb+j (j < 4) and zero weights at 8kb+4 .. 8kb+7, so each lane's B fragment is its own D fragment of
+  // the layer before, padded with zeros
+  const float r1 = 1.0f / sw1, r2 = 1.0f / sw2, r3 = 1.0f / sw3;     // powers of two: exact
   float q[3][8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    q[0][i] = w1(m, 8 * kb + i) * r1; q[1][i] = w2(m, 8 * kb + i) * r2; q[2][i] = w3(m, 8 * kb + i) * r3;
+    q[0][i] = w1(m, 8 * kb + i) * r1;
+    q[1][i] = i < 4 ? w2(m, 4 * kb + i) * r2 : 0.0f;
+    q[2][i] = i < 4 ? w3(m, 4 * kb + i) * r3 : 0.0f;
   }
 #pragma unroll
   for (int L = 0; L < 3; L++)
     p->q[L][l] = make_uint2(pack_fp8x4(q[L][0], q[L][1], q[L][2], q[L][3]), pack_fp8x4(q[L][4], q[L][5], q[L][6], q[L][7]));
-  if (l < 16) { p->b1[l] = B1[l]; p->b2[l] = B2[l]; p->b3[l] = l < 12 ? B3[l] : 0.0f; }
-  if (l == 0) { p->sw[0] = s1; p->sw[1] = s2; p->sw[2] = s3; }
+  if (l < 16) {
+    p->b1[l] = B1[l]; p->b2[l] = B2[l]; p->b3[l] = l < 12 ? B3[l] : 0.0f;
+    p->qb1[l] = B1[l] / sx2; p->qb2[l] = B2[l] / sx3;                  // powers of two: exact
+  }
+  if (l == 0) {
+    p->sw[0] = sw1; p->sw[1] = sw2; p->sw[2] = sw3;
+    p->sx[0] = sx1; p->sx[1] = sx2; p->sx[2] = sx3;
+    p->xclamp = pack_h2(fminf(448.0f * sx1, 65504.0f), fminf(448.0f * sx1, 65504.0f));   // exact for sx1 >= 2^-20
+    p->qc[0] = sx1 * sw1 / sx2; p->qc[1] = sx2 * sw2 / sx3; p->qc[2] = sx3 * sw3;
+  }
 }
 
 // bias + ReLU on a D fragment (units 4kb+i)
@@ -753,185 +754,129 @@ __device__ __forceinline__ f32x4 bias_relu(f32x4 d, const float *b, int kb) {
   return r;
 }
 
-// The MLP of this wave's 64 texels.  u: this lane's features as 10 binary16 pairs (F0.xyzw, F1.xyzw,
-// F2.xy); o: the 12 outputs of this lane's texel.  sX (64 x 16 words) and sO (64 x 16 floats) are this
-// wave's LDS scratch; every lane of the workgroup must call this (it holds workgroup barriers).
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// eight binary16 features (four pairs) -> eight bfloat16, round to nearest even (each binary16 value is exact
+// in fp32)
+__device__ __forceinline__ uint4 bf16x8_of(uint4 x) {
+  const float2 a = unpack2(x.x), b = unpack2(x.y), c = unpack2(x.z), d = unpack2(x.w);
+  return make_uint4(pack_bf2(a.x, a.y), pack_bf2(b.x, b.y), pack_bf2(c.x, c.y), pack_bf2(d.x, d.y));
+}
+
+// The MFMA chains of one wave's 64 texels as four independent 16-texel tiles, interleaved layer by layer so
+// the matrix cores overlap one tile's latency with the others.  xf[g]: lane (kb, tl)'s layer-1 B fragment of
+// tile g, features 8kb .. 8kb+7 of its texel as binary16 pairs (zero beyond feature 19); d[g] receives units
+// 4kb .. 4kb+3 of the output layer, bias included.
+//   binary16 / bfloat16: layer 1 is one v_mfma_f32_16x16x32_f16 / _bf16 (K = 20 padded to 32); layers 2 and
+//     3 are 16x16x16, whose B fragment (k = 4kb .. 4kb+3) is exactly the D fragment of the layer before, so
+//     the hidden activations never leave the registers.
+//   fp8 (E4M3): three v_mfma_f32_16x16x32_fp8_fp8 with a power-of-two scale per layer for the weights (sw)
+//     and a static one per layer for its input (sx, set on the host: no reductions in the kernel); layers 2
+//     and 3 use half of K (each lane's 4 hidden units and 4 zeros), so the hidden values stay in their lanes.
 template <int MODE>
-__device__ void mlp_mfma(const MfmaPack *__restrict__ P, const uint32_t *u, float *o, uint32_t *sX, float *sO, int lane) {
+__device__ __forceinline__ void mlp_mfma_x4(const MfmaPack *__restrict__ P, const uint4 *xf, f32x4 *d, int lane) {
   const int kb = lane >> 4, tl = lane & 15;
-  if (MODE == MLP_FP16_MFMA) {
+  const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+  if constexpr (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
+    uint2 hb[4];
+    if constexpr (MODE == MLP_FP16_MFMA) {
+      const f16x8 w1 = __builtin_bit_cast(f16x8, P->h1[lane]);
 #pragma unroll
-    for (int j = 0; j < 16; j++) sX[lane * 16 + j] = j < 10 ? u[j] : 0u;
-  } else if (MODE == MLP_BF16_MFMA) {
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-      float2 f = j < 10 ? unpack2(u[j]) : make_float2(0.0f, 0.0f);
-      sX[lane * 16 + j] = pack_bf2(f.x, f.y);
-    }
-  } else {                                             // fp8: keep the binary16 features, scale per group below
-#pragma unroll
-    for (int j = 0; j < 16; j++) sX[lane * 16 + j] = j < 10 ? u[j] : 0u;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int g = 0; g < 4; g++) {
-    const int t = 16 * g + tl;                        // texel (row of this wave) of this lane's B column
-    f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
-      const uint2 x0 = *(const uint2 *)&sX[t * 16 + 2 * kb];        // k = 4kb .. 4kb+3
-      const uint2 x1 = *(const uint2 *)&sX[t * 16 + 8 + 2 * kb];    // k = 16 + 4kb ..
-      const uint2 (*Wf)[64] = MODE == MLP_FP16_MFMA ? P->h : P->b;
-      if (MODE == MLP_FP16_MFMA) {
-        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, Wf[0][lane]), __builtin_bit_cast(f16x4, x0), d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, Wf[1][lane]), __builtin_bit_cast(f16x4, x1), d, 0, 0, 0);
-        f32x4 h = bias_relu(d, P->b1, kb);
-        uint2 hb = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, Wf[2][lane]), __builtin_bit_cast(f16x4, hb), d, 0, 0, 0);
-        h = bias_relu(d, P->b2, kb);
-        hb = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, Wf[3][lane]), __builtin_bit_cast(f16x4, hb), d, 0, 0, 0);
-      } else {
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, Wf[0][lane]), __builtin_bit_cast(s16x4, x0), d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, Wf[1][lane]), __builtin_bit_cast(s16x4, x1), d, 0, 0, 0);
-        f32x4 h = bias_relu(d, P->b1, kb);
-        uint2 hb = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, Wf[2][lane]), __builtin_bit_cast(s16x4, hb), d, 0, 0, 0);
-        h = bias_relu(d, P->b2, kb);
-        hb = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, Wf[3][lane]), __builtin_bit_cast(s16x4, hb), d, 0, 0, 0);
-      }
+      for (int g = 0; g < 4; g++) d[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w1, __builtin_bit_cast(f16x8, xf[g]), z, 0, 0, 0);
     } else {
-      // fp8, 16x16x32: lane l holds k = 8(l/16) .. +7 of texel l%16; power-of-two scale per layer and group
-      float xv[8];
+      const bf16x8 w1 = __builtin_bit_cast(bf16x8, P->b1w[lane]);
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const float2 f = unpack2(sX[t * 16 + 4 * kb + i]);
-        xv[2 * i] = f.x; xv[2 * i + 1] = f.y;
+      for (int g = 0; g < 4; g++)
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, __builtin_bit_cast(bf16x8, bf16x8_of(xf[g])), z, 0, 0, 0);
+    }
+#pragma unroll
+    for (int L = 0; L < 2; L++) {
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const f32x4 h = bias_relu(d[g], L == 0 ? P->b1 : P->b2, kb);
+        hb[g] = MODE == MLP_FP16_MFMA ? make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]))
+                                      : make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
       }
-      float am = 0.0f;
+      if constexpr (MODE == MLP_FP16_MFMA) {
+        const f16x4 wl = __builtin_bit_cast(f16x4, P->h[L][lane]);
 #pragma unroll
-      for (int i = 0; i < 8; i++) am = fmaxf(am, fabsf(xv[i]));
-      float sx = pow2_scale(wave_max(am)), rx = 1.0f / sx;
-      uint2 xb = make_uint2(pack_fp8x4(xv[0] * rx, xv[1] * rx, xv[2] * rx, xv[3] * rx),
-                            pack_fp8x4(xv[4] * rx, xv[5] * rx, xv[6] * rx, xv[7] * rx));
-      d = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[0][lane]),
-                                                     (long)__builtin_bit_cast(uint64_t, xb), d, 0, 0, 0);
+        for (int g = 0; g < 4; g++) d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(wl, __builtin_bit_cast(f16x4, hb[g]), z, 0, 0, 0);
+      } else {
+        const s16x4 wl = __builtin_bit_cast(s16x4, P->b[L][lane]);
 #pragma unroll
-      for (int L = 1; L < 3; L++) {
-        const float scale = sx * P->sw[L - 1];
-        f32x4 h;
-#pragma unroll
-        for (int i = 0; i < 4; i++) h[i] = mx(d[i] * scale + (L == 1 ? P->b1 : P->b2)[4 * kb + i], 0.0f);
-        // regroup: this lane (kb' = kb < 2) needs units 8kb' .. 8kb'+7 of texel tl, held by lanes
-        // tl + 32kb' (units 8kb' .. +3) and tl + 32kb' + 16 (units 8kb'+4 .. +7)
-        float hv[8];
-        const int src = tl + 32 * (kb & 1);
-#pragma unroll
-        for (int i = 0; i < 4; i++) { hv[i] = __shfl(h[i], src); hv[4 + i] = __shfl(h[i], src + 16); }
-        am = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 4; i++) am = fmaxf(am, h[i]);
-        sx = pow2_scale(wave_max(am));
-        rx = 1.0f / sx;
-        const bool live = kb < 2;                     // k >= 16 are zero for the 16-wide layers
-#pragma unroll
-        for (int i = 0; i < 8; i++) hv[i] = live ? hv[i] * rx : 0.0f;
-        xb = make_uint2(pack_fp8x4(hv[0], hv[1], hv[2], hv[3]), pack_fp8x4(hv[4], hv[5], hv[6], hv[7]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[L][lane]),
-                                                       (long)__builtin_bit_cast(uint64_t, xb), d, 0, 0, 0);
+        for (int g = 0; g < 4; g++) d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl, __builtin_bit_cast(s16x4, hb[g]), z, 0, 0, 0);
       }
-      const float scale = sx * P->sw[2];
-#pragma unroll
-      for (int i = 0; i < 4; i++) d[i] = d[i] * scale;
-    }
-    if (kb < 3) {
-#pragma unroll
-      for (int i = 0; i < 4; i++) sO[t * 16 + 4 * kb + i] = d[i] + P->b3[4 * kb + i];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < 12; j++) o[j] = sO[lane * 16 + j];
-}
-
-// The MFMA chains of one wave's 64 texels as four independent 16-texel tiles (binary16 or bfloat16 MFMA):
-// x0[g], x1[g] are lane (kb, tl)'s B fragments of tile g (feature words 2kb, 2kb+1 and, for kb = 0, words
-// 8, 9 as binary16 pairs); d[g] receives units 4kb .. 4kb+3 of the output layer before its bias.  The four
-// chains are interleaved layer by layer so the matrix cores overlap one tile's latency with the others.
-template <int MODE>
-__device__ __forceinline__ void mlp_mfma_x4(const MfmaPack *__restrict__ P, const uint2 *x0, const uint2 *x1,
-                                            f32x4 *d, int lane) {
-  const int kb = lane >> 4;
-  uint2 hb[4];
-  if (MODE == MLP_FP16_MFMA) {
-    const f16x4 w0 = __builtin_bit_cast(f16x4, P->h[0][lane]), w1 = __builtin_bit_cast(f16x4, P->h[1][lane]);
-    const f16x4 w2 = __builtin_bit_cast(f16x4, P->h[2][lane]), w3 = __builtin_bit_cast(f16x4, P->h[3][lane]);
-#pragma unroll
-    for (int g = 0; g < 4; g++) {
-      d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w0, __builtin_bit_cast(f16x4, x0[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-      d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w1, __builtin_bit_cast(f16x4, x1[g]), d[g], 0, 0, 0);
-    }
-#pragma unroll
-    for (int g = 0; g < 4; g++) {
-      const f32x4 h = bias_relu(d[g], P->b1, kb);
-      hb[g] = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
     }
 #pragma unroll
     for (int g = 0; g < 4; g++)
-      d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w2, __builtin_bit_cast(f16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-      const f32x4 h = bias_relu(d[g], P->b2, kb);
-      hb[g] = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
-    }
-#pragma unroll
-    for (int g = 0; g < 4; g++)
-      d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(w3, __builtin_bit_cast(f16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+      for (int i = 0; i < 4; i++) d[g][i] = d[g][i] + P->b3[4 * kb + i];
   } else {
-    const s16x4 w0 = __builtin_bit_cast(s16x4, P->b[0][lane]), w1 = __builtin_bit_cast(s16x4, P->b[1][lane]);
-    const s16x4 w2 = __builtin_bit_cast(s16x4, P->b[2][lane]), w3 = __builtin_bit_cast(s16x4, P->b[3][lane]);
+    // layer-1 input: the binary16 features clamped to +-448 sx[0] (packed binary16 min / max), then one scaled
+    // conversion per pair (E4M3 of x / sx[0], round to nearest even; it does not saturate by itself)
+    const half2v cl = __builtin_bit_cast(half2v, P->xclamp), ncl = -cl;
+    const float s1 = P->sx[0];
+    auto cvt4 = [&](uint32_t lo, uint32_t hi) {
+      const half2v a = __builtin_elementwise_min(__builtin_elementwise_max(__builtin_bit_cast(half2v, lo), ncl), cl);
+      const half2v b = __builtin_elementwise_min(__builtin_elementwise_max(__builtin_bit_cast(half2v, hi), ncl), cl);
+      s16x2 r = {0, 0};
+      r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(r, a, s1, false);
+      r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(r, b, s1, true);
+      return __builtin_bit_cast(uint32_t, r);
+    };
 #pragma unroll
     for (int g = 0; g < 4; g++) {
-      const float2 a0 = unpack2(x0[g].x), a1 = unpack2(x0[g].y), c0 = unpack2(x1[g].x), c1 = unpack2(x1[g].y);
-      const uint2 y0 = make_uint2(pack_bf2(a0.x, a0.y), pack_bf2(a1.x, a1.y));
-      const uint2 y1 = make_uint2(pack_bf2(c0.x, c0.y), pack_bf2(c1.x, c1.y));
-      d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w0, __builtin_bit_cast(s16x4, y0), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-      d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1, __builtin_bit_cast(s16x4, y1), d[g], 0, 0, 0);
+      const uint2 xb = make_uint2(cvt4(xf[g].x, xf[g].y), cvt4(xf[g].z, xf[g].w));
+      d[g] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[0][lane]),
+                                                        (long)__builtin_bit_cast(uint64_t, xb), z, 0, 0, 0);
     }
+    // layers 2 and 3: lane (kb, tl) feeds its own units 4kb .. 4kb+3 at k = 8kb .. 8kb+3 and zeros above
+    // (the weights there are zero too), so no value crosses lanes
+    // the next layer's E4M3 input directly: (d sx sw + b) / sx' = d qc + qb (sx' the next layer's input
+    // scale, folded into qc and the prescaled bias qb), ReLU and saturation at 448 in one median
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-      const f32x4 h = bias_relu(d[g], P->b1, kb);
-      hb[g] = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
+    for (int L = 1; L < 3; L++) {
+      const float sc = P->qc[L - 1];
+      const float *bias = L == 1 ? P->qb1 : P->qb2;
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        float h[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) h[i] = __builtin_amdgcn_fmed3f(__builtin_fmaf(d[g][i], sc, bias[4 * kb + i]), 0.0f, 448.0f);
+        int r = __builtin_amdgcn_cvt_pk_fp8_f32(h[0], h[1], 0, false);
+        r = __builtin_amdgcn_cvt_pk_fp8_f32(h[2], h[3], r, true);
+        const uint2 xb = make_uint2((uint32_t)r, 0u);
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[L][lane]),
+                                                          (long)__builtin_bit_cast(uint64_t, xb), z, 0, 0, 0);
+      }
     }
+    const float sc = P->qc[2];
 #pragma unroll
     for (int g = 0; g < 4; g++)
-      d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2, __builtin_bit_cast(s16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-      const f32x4 h = bias_relu(d[g], P->b2, kb);
-      hb[g] = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
-    }
-#pragma unroll
-    for (int g = 0; g < 4; g++)
-      d[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3, __builtin_bit_cast(s16x4, hb[g]), f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+      for (int i = 0; i < 4; i++) d[g][i] = __builtin_fmaf(d[g][i], sc, P->b3[4 * kb + i]);
   }
 }
 
-// Binary16 and bfloat16 MFMA forms.  Lane (kb, tl) of wave v loads its B fragments of the wave's four
+// layer-1 B fragment of lane (kb, tl) from the feature surfaces: F0 (kb 0), F1 (kb 1), F2.xy (kb 2), zero
+__device__ __forceinline__ uint4 feature_frag(const TexU &F0, const TexU &F1, const TexU &F2, int x, int y, int kb, bool live) {
+  if (!live || kb == 3) return make_uint4(0u, 0u, 0u, 0u);
+  if (kb == 2) {
+    const uint2 v = *(const uint2 *)(F2.p + (size_t)y * F2.pitch + x);
+    return make_uint4(v.x, v.y, 0u, 0u);
+  }
+  const TexU &F = kb == 0 ? F0 : F1;
+  return F.p[(size_t)y * F.pitch + x];
+}
+
+// MFMA forms in the frame's inference pass.  Lane (kb, tl) of wave v loads its B fragments of the wave's four
 // 16-texel tiles straight from F0 / F1 / F2 (tile g = texels 16(g&1) .. +15 of tile row 2v + (g>>1)), the
 // four MFMA chains run interleaved, and the D fragments (+ bias) go through LDS at a 48-byte texel stride
 // (16-byte stores and loads, conflict free) so each lane ends with the 12 outputs of its own texel.
-// LDS: the staged tile, then per wave the 64 x 3 output float4s (fp16 / bf16) or the fp8 form's 64 x 16
-// activation words and 64 x 16 outputs
+// LDS: the staged tile, then per wave the 64 x 3 output float4s
 template <int MODE>
-constexpr int infer_mfma_lds() {
-  return infer_lds() + ((MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) ? 4 * 64 * 3 : 2 * 4 * 64 * 16 / 4);
-}
+constexpr int infer_mfma_lds() { return infer_lds() + 4 * 64 * 3; }
 template <int MODE, bool FAST>
 __device__ __forceinline__ void infer_mfma_tile(const InferArgs &a, int n, float4 *lds) {
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
@@ -950,64 +895,35 @@ __device__ __forceinline__ void infer_mfma_tile(const InferArgs &a, int n, float
   const int X = x0 + lx, Y = y0 + ly;
   const int wv = tid >> 6, lane = tid & 63, kb = lane >> 4, tl = lane & 15;
   const MfmaPack *__restrict__ P = (const MfmaPack *)a.wq;
-  if constexpr (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
-    float4 (*sO)[64 * 3] = (float4 (*)[64 * 3])(lds + 2 * SN);
-    uint2 fx0[4], fx1[4];
-    const TexU &Fk = kb < 2 ? a.F0 : a.F1;
+  float4 (*sO)[64 * 3] = (float4 (*)[64 * 3])(lds + 2 * SN);
+  uint4 xf[4];
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-      const int gx = x0 + 16 * (g & 1) + tl, gy = y0 + 2 * wv + (g >> 1);
-      const bool live = inside(gx, gy, a.W2, a.H2);
-      const size_t o = live ? (size_t)gy : 0;
-      const uint2 *rowk = (const uint2 *)(Fk.p + o * Fk.pitch + (live ? gx : 0));
-      const uint2 *row2 = (const uint2 *)(a.F2.p + o * a.F2.pitch + (live ? gx : 0));
-      fx0[g] = live ? rowk[kb & 1] : make_uint2(0, 0);                   // feature words 2kb, 2kb+1
-      fx1[g] = (live && kb == 0) ? row2[0] : make_uint2(0, 0);          // words 8, 9
-    }
-    f32x4 d[4];
-    mlp_mfma_x4<MODE>(P, fx0, fx1, d, lane);
-    if (kb < 3) {
-#pragma unroll
-      for (int g = 0; g < 4; g++)
-        sO[wv][(16 * g + tl) * 3 + kb] = make_float4(d[g][0] + P->b3[4 * kb], d[g][1] + P->b3[4 * kb + 1],
-                                                     d[g][2] + P->b3[4 * kb + 2], d[g][3] + P->b3[4 * kb + 3]);
-    }
-    __syncthreads();
-    if (!inside(X, Y, a.W2, a.H2)) return;
-    const int tx = lx + AP, ty = ly + AP;
-    if (sG[ty * SW + tx].w < 0.0f) {
-      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      st(a.out0, X, Y, z); st(a.out1, X, Y, z); st(a.out2, X, Y, z); st(a.outMid, X, Y, z);
-      return;
-    }
-    float o[12];
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const float4 v = sO[wv][lane * 3 + j];
-      o[4 * j] = v.x; o[4 * j + 1] = v.y; o[4 * j + 2] = v.z; o[4 * j + 3] = v.w;
-    }
-    infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
-  } else {
-    uint32_t (*sX)[64 * 16] = (uint32_t (*)[64 * 16])(lds + 2 * SN);
-    float (*sOq)[64 * 16] = (float (*)[64 * 16])(lds + 2 * SN + 4 * 64 * 16 / 4);
-    const bool in_img = inside(X, Y, a.W2, a.H2);
-    uint32_t u[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (in_img) {
-      const uint4 f0 = ldu(a.F0, X, Y), f1 = ldu(a.F1, X, Y), f2 = ldu(a.F2, X, Y);
-      u[0] = f0.x; u[1] = f0.y; u[2] = f0.z; u[3] = f0.w; u[4] = f1.x; u[5] = f1.y; u[6] = f1.z; u[7] = f1.w;
-      u[8] = f2.x; u[9] = f2.y;
-    }
-    float o[12];
-    mlp_mfma<MODE>(P, u, o, sX[wv], sOq[wv], lane);   // holds the tile barrier too
-    if (!in_img) return;
-    const int tx = lx + AP, ty = ly + AP;
-    if (sG[ty * SW + tx].w < 0.0f) {
-      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      st(a.out0, X, Y, z); st(a.out1, X, Y, z); st(a.out2, X, Y, z); st(a.outMid, X, Y, z);
-      return;
-    }
-    infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
+  for (int g = 0; g < 4; g++) {
+    const int gx = x0 + 16 * (g & 1) + tl, gy = y0 + 2 * wv + (g >> 1);
+    xf[g] = feature_frag(a.F0, a.F1, a.F2, gx, gy, kb, inside(gx, gy, a.W2, a.H2));
   }
+  f32x4 d[4];
+  mlp_mfma_x4<MODE>(P, xf, d, lane);
+  if (kb < 3) {
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+      sO[wv][(16 * g + tl) * 3 + kb] = make_float4(d[g][0], d[g][1], d[g][2], d[g][3]);
+  }
+  __syncthreads();
+  if (!inside(X, Y, a.W2, a.H2)) return;
+  const int tx = lx + AP, ty = ly + AP;
+  if (sG[ty * SW + tx].w < 0.0f) {
+    const float4 zr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    st(a.out0, X, Y, zr); st(a.out1, X, Y, zr); st(a.out2, X, Y, zr); st(a.outMid, X, Y, zr);
+    return;
+  }
+  float o[12];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const float4 v = sO[wv][lane * 3 + j];
+    o[4 * j] = v.x; o[4 * j + 1] = v.y; o[4 * j + 2] = v.z; o[4 * j + 3] = v.w;
+  }
+  infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
 }
 template <int MODE, bool FAST>
 __global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
@@ -1017,7 +933,7 @@ __global__ __launch_bounds__(NT) void k_infer_mfma(InferArgs a) {
 
 // The MLP's 12 outputs (before the activations, bias included) of every half-res texel, read from F0..F2
 // in place with the binary16 / bfloat16 MFMA forms: the training forward's MFMA option.  Texels in raster
-// order, 64 per wave in four 16-texel tiles; lane (kb, tl) of tile g loads the B fragments of texel
+// order, 64 per wave in four 16-texel tiles; lane (kb, tl) of tile g loads the B fragment of texel
 // 64w + 16g + tl as k_infer_mfma does.  Sky texels are computed too (the caller masks them).
 template <int MODE>
 __global__ __launch_bounds__(NT) void k_mlp_tex(TexU F0, TexU F1, TexU F2, int W2, int H2, float *__restrict__ out,
@@ -1025,26 +941,20 @@ __global__ __launch_bounds__(NT) void k_mlp_tex(TexU F0, TexU F1, TexU F2, int W
   const MfmaPack *__restrict__ P = (const MfmaPack *)wq;
   const int tid = threadIdx.x, lane = tid & 63, kb = lane >> 4, tl = lane & 15;
   const long base = (long)blockIdx.x * NT + (tid & ~63), n = (long)W2 * H2;
-  const TexU &Fk = kb < 2 ? F0 : F1;
-  uint2 x0[4], x1[4];
+  uint4 xf[4];
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     const long r = base + 16 * g + tl;
     const bool live = r < n;
-    const int x = live ? (int)(r % W2) : 0, y = live ? (int)(r / W2) : 0;
-    const uint2 *rowk = (const uint2 *)(Fk.p + (size_t)y * Fk.pitch + x);
-    const uint2 *row2 = (const uint2 *)(F2.p + (size_t)y * F2.pitch + x);
-    x0[g] = live ? rowk[kb & 1] : make_uint2(0, 0);
-    x1[g] = (live && kb == 0) ? row2[0] : make_uint2(0, 0);
+    xf[g] = feature_frag(F0, F1, F2, live ? (int)(r % W2) : 0, live ? (int)(r / W2) : 0, kb, live);
   }
   f32x4 d[4];
-  mlp_mfma_x4<MODE>(P, x0, x1, d, lane);
+  mlp_mfma_x4<MODE>(P, xf, d, lane);
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     const long r = base + 16 * g + tl;
     if (r < n && kb < 3)
-      *(float4 *)(out + r * 12 + 4 * kb) = make_float4(d[g][0] + P->b3[4 * kb], d[g][1] + P->b3[4 * kb + 1],
-                                                       d[g][2] + P->b3[4 * kb + 2], d[g][3] + P->b3[4 * kb + 3]);
+      *(float4 *)(out + r * 12 + 4 * kb) = make_float4(d[g][0], d[g][1], d[g][2], d[g][3]);
   }
 }
 
@@ -1056,7 +966,7 @@ template <int MODE>
 __global__ __launch_bounds__(NT) void k_mlp(const uint32_t *__restrict__ feat, float *__restrict__ out, int n,
                                              const float *__restrict__ w, const void *__restrict__ wq) {
   const int tid = threadIdx.x;
-  if (MODE == MLP_FP32 || MODE == MLP_F16) {
+  if constexpr (MODE == MLP_FP32 || MODE == MLP_F16) {
     const long r = (long)blockIdx.x * NT + tid;
     if (r >= n) return;
     float f[20], o[12];
@@ -1073,115 +983,30 @@ __global__ __launch_bounds__(NT) void k_mlp(const uint32_t *__restrict__ feat, f
     dst[0] = make_float4(o[0], o[1], o[2], o[3]);
     dst[1] = make_float4(o[4], o[5], o[6], o[7]);
     dst[2] = make_float4(o[8], o[9], o[10], o[11]);
-    return;
-  }
-  const MfmaPack *__restrict__ P = (const MfmaPack *)wq;
-  const int lane = tid & 63, kb = lane >> 4, tl = lane & 15;
-  const long base = ((long)blockIdx.x * NT + (tid & ~63));          // first row of this wave
-  if (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
-    // the wave's 64 rows as four independent 16-row tiles: every load first, then the four MFMA chains
-    // interleaved layer by layer, so the matrix cores overlap one tile's latency with the others
-    uint2 x0[4], x1[4];
+  } else {
+    const MfmaPack *__restrict__ P = (const MfmaPack *)wq;
+    const int lane = tid & 63, kb = lane >> 4, tl = lane & 15;
+    const long base = ((long)blockIdx.x * NT + (tid & ~63));          // first row of this wave
+    // the wave's 64 rows as four 16-row tiles: every load first (rows are 40 B, 8-byte aligned: words
+    // 4kb .. 4kb+3 as two 8-byte loads), then the four MFMA chains interleaved
+    uint4 xf[4];
 #pragma unroll
     for (int g = 0; g < 4; g++) {
       const long r = base + 16 * g + tl;
-      const bool live = r < n;
-      const uint32_t *row = feat + (live ? r : 0) * 10;
-      x0[g] = live ? *(const uint2 *)(row + 2 * kb) : make_uint2(0, 0);                       // k = 4kb ..
-      x1[g] = (live && kb == 0) ? *(const uint2 *)(row + 8) : make_uint2(0, 0);              // k = 16 .. 19
+      const bool live = r < n && kb < 3;
+      const uint2 *row = (const uint2 *)(feat + (live ? r : 0) * 10);
+      const uint2 lo = live ? row[2 * kb] : make_uint2(0u, 0u);
+      const uint2 hi = (live && kb < 2) ? row[2 * kb + 1] : make_uint2(0u, 0u);
+      xf[g] = make_uint4(lo.x, lo.y, hi.x, hi.y);
     }
     f32x4 d[4];
-    mlp_mfma_x4<MODE>(P, x0, x1, d, lane);
+    mlp_mfma_x4<MODE>(P, xf, d, lane);
 #pragma unroll
     for (int g = 0; g < 4; g++) {
       const long r = base + 16 * g + tl;
       if (r < n && kb < 3)
-        *(float4 *)(out + r * 12 + 4 * kb) = make_float4(d[g][0] + P->b3[4 * kb], d[g][1] + P->b3[4 * kb + 1],
-                                                         d[g][2] + P->b3[4 * kb + 2], d[g][3] + P->b3[4 * kb + 3]);
+        *(float4 *)(out + r * 12 + 4 * kb) = make_float4(d[g][0], d[g][1], d[g][2], d[g][3]);
     }
-    return;
-  }
-#pragma unroll
-  for (int g = 0; g < 4; g++) {
-    const long r = base + 16 * g + tl;
-    const bool live = r < n;
-    const uint32_t *row = feat + (live ? r : 0) * 10;
-    f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (MODE == MLP_FP16_MFMA || MODE == MLP_BF16_MFMA) {
-      uint2 x0 = live ? *(const uint2 *)(row + 2 * kb) : make_uint2(0, 0);                       // k = 4kb ..
-      uint2 x1 = (live && kb == 0) ? *(const uint2 *)(row + 8) : make_uint2(0, 0);              // k = 16 .. 19
-      if (MODE == MLP_FP16_MFMA) {
-        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, P->h[0][lane]), __builtin_bit_cast(f16x4, x0), d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, P->h[1][lane]), __builtin_bit_cast(f16x4, x1), d, 0, 0, 0);
-        f32x4 h = bias_relu(d, P->b1, kb);
-        uint2 hb = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, P->h[2][lane]), __builtin_bit_cast(f16x4, hb), d, 0, 0, 0);
-        h = bias_relu(d, P->b2, kb);
-        hb = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, P->h[3][lane]), __builtin_bit_cast(f16x4, hb), d, 0, 0, 0);
-      } else {
-        const float2 a0 = unpack2(x0.x), a1 = unpack2(x0.y), c0 = unpack2(x1.x), c1 = unpack2(x1.y);
-        x0 = make_uint2(pack_bf2(a0.x, a0.y), pack_bf2(a1.x, a1.y));
-        x1 = make_uint2(pack_bf2(c0.x, c0.y), pack_bf2(c1.x, c1.y));
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, P->b[0][lane]), __builtin_bit_cast(s16x4, x0), d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, P->b[1][lane]), __builtin_bit_cast(s16x4, x1), d, 0, 0, 0);
-        f32x4 h = bias_relu(d, P->b1, kb);
-        uint2 hb = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, P->b[2][lane]), __builtin_bit_cast(s16x4, hb), d, 0, 0, 0);
-        h = bias_relu(d, P->b2, kb);
-        hb = make_uint2(pack_bf2(h[0], h[1]), pack_bf2(h[2], h[3]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, P->b[3][lane]), __builtin_bit_cast(s16x4, hb), d, 0, 0, 0);
-      }
-    } else {
-      // fp8, 16x16x32: lane holds k = 8kb .. 8kb+7 (words 4kb .. 4kb+3; k >= 20 zero)
-      float xv[8];
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int wd = 4 * kb + i;
-        const float2 f = (live && wd < 10) ? unpack2(row[wd]) : make_float2(0.0f, 0.0f);
-        xv[2 * i] = f.x; xv[2 * i + 1] = f.y;
-      }
-      float am = 0.0f;
-#pragma unroll
-      for (int i = 0; i < 8; i++) am = fmaxf(am, fabsf(xv[i]));
-      float sx = pow2_scale(wave_max(am)), rx = 1.0f / sx;
-      uint2 xb = make_uint2(pack_fp8x4(xv[0] * rx, xv[1] * rx, xv[2] * rx, xv[3] * rx),
-                            pack_fp8x4(xv[4] * rx, xv[5] * rx, xv[6] * rx, xv[7] * rx));
-      d = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[0][lane]),
-                                                     (long)__builtin_bit_cast(uint64_t, xb), d, 0, 0, 0);
-#pragma unroll
-      for (int L = 1; L < 3; L++) {
-        const float scale = sx * P->sw[L - 1];
-        f32x4 h;
-#pragma unroll
-        for (int i = 0; i < 4; i++) h[i] = mx(d[i] * scale + (L == 1 ? P->b1 : P->b2)[4 * kb + i], 0.0f);
-        float hv[8];
-        const int src = tl + 32 * (kb & 1);
-#pragma unroll
-        for (int i = 0; i < 4; i++) { hv[i] = __shfl(h[i], src); hv[4 + i] = __shfl(h[i], src + 16); }
-        am = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 4; i++) am = fmaxf(am, h[i]);
-        sx = pow2_scale(wave_max(am));
-        rx = 1.0f / sx;
-#pragma unroll
-        for (int i = 0; i < 8; i++) hv[i] = kb < 2 ? hv[i] * rx : 0.0f;
-        xb = make_uint2(pack_fp8x4(hv[0], hv[1], hv[2], hv[3]), pack_fp8x4(hv[4], hv[5], hv[6], hv[7]));
-        d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        d = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)__builtin_bit_cast(uint64_t, P->q[L][lane]),
-                                                       (long)__builtin_bit_cast(uint64_t, xb), d, 0, 0, 0);
-      }
-      const float scale = sx * P->sw[2];
-#pragma unroll
-      for (int i = 0; i < 4; i++) d[i] = d[i] * scale;
-    }
-    if (live && kb < 3)
-      *(float4 *)(out + r * 12 + 4 * kb) = make_float4(d[0] + P->b3[4 * kb], d[1] + P->b3[4 * kb + 1],
-                                                       d[2] + P->b3[4 * kb + 2], d[3] + P->b3[4 * kb + 3]);
   }
 }
 
@@ -1719,8 +1544,8 @@ hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s) 
   return hipGetLastError();
 }
 
-hipError_t launch_pack_mlp(const float *w, MfmaPack *p, float s1, float s2, float s3, hipStream_t s) {
-  hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, s, w, p, s1, s2, s3);
+hipError_t launch_pack_mlp(const float *w, MfmaPack *p, const float sw[3], const float sx[3], hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_mlp, dim3(1), dim3(64), 0, s, w, p, sw[0], sw[1], sw[2], sx[0], sx[1], sx[2]);
   return hipGetLastError();
 }
 

@@ -1164,7 +1164,8 @@ static hipError_t forward_t(hush_train_t *t, const T *w, T *out, hipStream_t s) 
   // the MLP runs over the feature surfaces in place, the activations follow; the backward stays FP32
   const bool mf = sizeof(T) == sizeof(float) && t->fwd_prec != HUSH_MLP_FP32;
   if (mf) {
-    hipError_t e = launch_pack_mlp((const float *)w, (MfmaPack *)t->wq, 1.0f, 1.0f, 1.0f, s);
+    const float one[3] = {1.0f, 1.0f, 1.0f};      // the binary16 / bfloat16 forms use no scales
+    hipError_t e = launch_pack_mlp((const float *)w, (MfmaPack *)t->wq, one, one, s);
     if (e != hipSuccess) return e;
   }
   auto mlp = [&](TexU F0, TexU F1, TexU F2, TexF GH, void *o, void *par) -> hipError_t {

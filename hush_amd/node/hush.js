@@ -16,6 +16,16 @@ const N_OUT = 12;
 const INPUTS = ['radiance', 'normalDepth', 'motion', 'material', 'emission'];
 const PRECISION = { fp32: 0, f16: 1, 'fp16-mfma': 2, 'bf16-mfma': 3, 'fp8-mfma': 4 };
 const DEFAULT_WEIGHTS_PATH = path.join(__dirname, '..', 'data', 'hush-weights.bin');
+const FP8_SCALES_PATH = path.join(__dirname, '..', 'data', 'fp8_scales.json');
+
+// the calibrated fp8 input scales stored for these weights (their SHA-256 must match), or null
+function storedFp8Scales(u16) {
+  let d;
+  try { d = JSON.parse(fs.readFileSync(FP8_SCALES_PATH, 'utf8')); } catch (e) { return null; }
+  const digest = require('crypto').createHash('sha256')
+    .update(Buffer.from(u16.buffer, u16.byteOffset, u16.byteLength)).digest('hex');
+  return d.weights_sha256 === digest && Array.isArray(d.scales) && d.scales.length === 3 ? d.scales : null;
+}
 
 // hiddenOf (hush.js:22-25): the MLP width h with h*h + (N_FEAT + N_OUT + 2) h + N_OUT == n, 0 if none
 function hiddenOf(n) {
@@ -76,7 +86,8 @@ function initDevice(ordinal = 0) {
 const REQUIRED_LIMITS = Object.freeze({ maxStorageTexturesPerShaderStage: 8 });
 
 class Hush {
-  // options: { width, height, exposure = 1, weights = built-in, f16 = false, precision = 'fp32', compensated = false }
+  // options: { width, height, exposure = 1, weights = built-in, f16 = false, precision = 'fp32', compensated = false,
+  //            fp8Scales = calibrated for the built-in weights }
   constructor(device, options = {}) {
     const { width, height, exposure = 1, weights, f16 = false } = options;
     for (const [k, v] of Object.entries(REQUIRED_LIMITS)) {
@@ -93,9 +104,16 @@ class Hush {
     this.f16 = precision === 'f16';
     this._h = native.create(device.ordinal, width, height, exposure, u16, PRECISION[precision]);
     if (options.compensated) native.setCompensated(this._h, true);   // compensated moment sums (a precision form)
+    // fp8 form: static per-layer input scales (powers of two); default the calibration stored for these weights
+    const fp8 = options.fp8Scales !== undefined ? options.fp8Scales : (this.hidden === 16 ? storedFp8Scales(u16) : null);
+    if (fp8) native.setFp8Scales(this._h, fp8);
     this._exposure = exposure;
     this._timed = false;
   }
+
+  // the fp8 form's static input scales of the three layers (each layer's input x enters as E4M3 of x / s)
+  get fp8Scales() { return native.getFp8Scales(this._h); }
+  set fp8Scales(s) { native.setFp8Scales(this._h, s); }
 
   get width() { return native.info(this._h).width; }
   get height() { return native.info(this._h).height; }

@@ -213,6 +213,41 @@ napi_value SetCompensated(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// setFp8Scales(h, [s0, s1, s2]): the fp8 form's static per-layer input scales (powers of two)
+napi_value SetFp8Scales(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return nullptr;
+  hush_t *h = ctx_of(env, a[0]);
+  bool arr = false;
+  napi_is_array(env, a[1], &arr);
+  uint32_t n = 0;
+  if (arr) napi_get_array_length(env, a[1], &n);
+  if (!arr || n != 3) return throw_err(env, "hush addon: fp8 scales must be an array of 3 numbers");
+  float sc[3];
+  for (uint32_t i = 0; i < 3; i++) {
+    napi_value v;
+    napi_get_element(env, a[1], i, &v);
+    sc[i] = (float)num(env, v, 0.0);
+  }
+  int r = hush_set_fp8_scales(h, sc);
+  if (r != HUSH_OK) return throw_status(env, h, r);
+  return nullptr;
+}
+
+// getFp8Scales(h) -> [s0, s1, s2]
+napi_value GetFp8Scales(napi_env env, napi_callback_info info) {
+  napi_value a[1];
+  if (!get_args(env, info, 1, a)) return nullptr;
+  hush_t *h = ctx_of(env, a[0]);
+  float sc[3];
+  int r = hush_get_fp8_scales(h, sc);
+  if (r != HUSH_OK) return throw_status(env, h, r);
+  napi_value o, v;
+  napi_create_array_with_length(env, 3, &o);
+  for (uint32_t i = 0; i < 3; i++) { napi_create_double(env, sc[i], &v); napi_set_element(env, o, i, v); }
+  return o;
+}
+
 napi_value Info(napi_env env, napi_callback_info info) {
   napi_value a[1];
   if (!get_args(env, info, 1, a)) return nullptr;
@@ -725,7 +760,8 @@ napi_value PoEncode(napi_env env, napi_callback_info info) {
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char *name; napi_callback fn; } fns[] = {
       {"hiddenOf", HiddenOf}, {"create", Create}, {"destroy", Destroy}, {"resize", Resize}, {"reset", Reset},
-      {"setExposure", SetExposure}, {"setPrecision", SetPrecision}, {"setCompensated", SetCompensated}, {"info", Info}, {"denoise", Denoise},
+      {"setExposure", SetExposure}, {"setPrecision", SetPrecision}, {"setCompensated", SetCompensated},
+      {"setFp8Scales", SetFp8Scales}, {"getFp8Scales", GetFp8Scales}, {"info", Info}, {"denoise", Denoise},
       {"output", Output}, {"passTimes", PassTimes}, {"alloc", Alloc}, {"free", Free}, {"upload", Upload},
       {"download", Download}, {"deviceCount", DeviceCount},
       {"svgfCreate", SvCreate}, {"svgfDestroy", SvDestroy}, {"svgfResize", SvResize}, {"svgfReset", SvReset},

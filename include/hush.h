@@ -46,11 +46,12 @@ enum hush_status {
 enum hush_precision {
   HUSH_MLP_FP32 = 0,          /* plain FP32 MLP: the drop-in (hush.js:683-704 with f16 = false) */
   HUSH_MLP_F16 = 1,           /* binary16 arithmetic, like shader-f16 (hush.js:690, 699-702) */
-  HUSH_MLP_FP16_MFMA = 2,     /* fp16 operands on MFMA (v_mfma_f32_16x16x16_f16), fp32 accumulation */
-  HUSH_MLP_BF16_MFMA = 3,     /* bf16 operands on MFMA, fp32 accumulation */
-  HUSH_MLP_FP8_MFMA = 4       /* fp8 E4M3 operands on MFMA (16x16x32), fp32 accumulation; power-of-two
-                                 scale per layer for the weights and per layer and 16-texel tile for the
-                                 activations */
+  HUSH_MLP_FP16_MFMA = 2,     /* fp16 operands on MFMA (layer 1 v_mfma_f32_16x16x32_f16, layers 2-3
+                                 16x16x16), fp32 accumulation */
+  HUSH_MLP_BF16_MFMA = 3,     /* bf16 operands on MFMA (same shapes), fp32 accumulation */
+  HUSH_MLP_FP8_MFMA = 4       /* fp8 E4M3 operands on MFMA (16x16x32), fp32 accumulation; static
+                                 power-of-two scales per layer for the weights (from the weights) and for
+                                 each layer's input (hush_set_fp8_scales) */
 };
 /* The MFMA forms need hidden width 16 (the built-in 20-16-16-12 network); other widths report
    HUSH_ERR_UNSUPPORTED for them. */
@@ -106,6 +107,15 @@ int64_t hush_frame(const hush_t *h);
 int32_t hush_set_precision(hush_t *h, int32_t precision);
 int32_t hush_get_precision(const hush_t *h);
 
+/* Static activation scales of the fp8 form (no counterpart in the reference): scales[L] divides the input
+   of layer L (features, hidden 1, hidden 2) before its E4M3 conversion, which saturates at +-448.  Each must
+   be a power of two in [2^-20, 2^20] (HUSH_ERR_ARGUMENT otherwise).  hush_create sets bounds derived from the
+   weights for features of magnitude at most 16; calibrated values (the largest input of each layer over
+   representative frames, as hush_amd/quant.py computes them) keep more of E4M3's range in use.  Takes
+   effect for later frames and hush_mlp_forward calls; hush_get_fp8_scales reads the current ones. */
+int32_t hush_set_fp8_scales(hush_t *h, const float *scales);
+int32_t hush_get_fp8_scales(const hush_t *h, float *scales);
+
 /* Math policy of later frames (no counterpart in the reference, whose arithmetic is the browser's):
    1 (default) = IEEE: correctly rounded division and square root, accurate exp / log throughout;
    0 = FAST: hardware reciprocal / square root / exp2 / log2 in the spatial filters, MLP activations and
@@ -131,7 +141,8 @@ int32_t hush_encode(hush_t *h, const hush_inputs *in, void *stream, hush_surface
 int32_t hush_output(const hush_t *h, hush_surface *out);
 
 /* The MLP alone (no counterpart in the reference): n rows of 20 binary16 features (row-major, 40 bytes a
-   row, 4-byte aligned device memory) -> n x 12 fp32 outputs (pre-activation), with the given precision.
+   row, 8-byte aligned device memory) -> n x 12 fp32 outputs (pre-activation; 16-byte aligned device memory),
+   with the given precision.  HUSH_ERR_ARGUMENT for misaligned pointers.
    For timing the MLP stage in isolation and for training tools. Hidden width 16 only. */
 int32_t hush_mlp_forward(hush_t *h, int32_t precision, const void *features, float *out, int64_t n, void *stream);
 

@@ -86,7 +86,8 @@ def test_node_svgf_sequence_matches_oracle(tmp_path):
 def test_node_options_match_oracle_and_python(tmp_path):
     """the JavaScript class's options through the N-API addon: an exposure change and reset() mid-sequence and
     weights as a base64 string (hidden width 8) against the oracle (tolerance of the end-to-end parity test);
-    the FP16-MFMA, shader-f16 and compensated forms bit-identical to the same library driven from Python.
+    the FP16-MFMA, shader-f16, compensated and FP8 forms (stored and explicit input scales) bit-identical to
+    the same library driven from Python.
     (The f16 MLP equals the oracle's bit for bit, tests/test_gpu_mlp.py; end to end on this scene one pixel
     of one frame moves 7% from single-ulp expf / logf differences between the GPU's math library and the
     host's, compounding through the history, which is beyond the end-to-end tolerance, so the option's
@@ -119,7 +120,8 @@ def test_node_options_match_oracle_and_python(tmp_path):
                 o.reset()
             st = compare(out(c, f).astype(np.float32)[..., :3], o.denoise(g)[..., :3])
             assert st["frac_rel_gt_1e-3"] <= 1e-3 and st["max_rel"] <= 1e-2, (c, f, st)
-    for c, opts in (("C", {"precision": "fp16-mfma"}), ("D", {"f16": True}), ("E", {"compensated": True})):
+    for c, opts in (("C", {"precision": "fp16-mfma"}), ("D", {"f16": True}), ("E", {"compensated": True}),
+                    ("F", {"precision": "fp8-mfma"}), ("G", {"precision": "fp8-mfma", "fp8_scales": (0.0625, 4, 16)})):
         hs = hush_amd.Hush(0, dict({"width": w, "height": h}, **opts))
         surfs = None
         for f, g in enumerate(gs):
