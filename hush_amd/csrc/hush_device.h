@@ -67,6 +67,23 @@ __device__ __forceinline__ void stu(const TexU &t, int x, int y, uint4 v) { t.p[
 
 __device__ __forceinline__ float4 ldf(const TexF &t, int x, int y) { return t.p[(size_t)y * t.pitch + x]; }
 __device__ __forceinline__ void stf(const TexF &t, int x, int y, float4 v) { t.p[(size_t)y * t.pitch + x] = v; }
+
+// Texel (x, y) of a surface.  O32: the surface spans less than 2 GiB (checked by the launcher), so the byte
+// offset from its base is a 32-bit value: one 24-bit multiply and a shift-add per access, and the
+// scalar-base + 32-bit-offset address form, instead of 64-bit address arithmetic (y < 2^14 and pitch < 2^23
+// fit the 24-bit multiply).
+template <bool O32, typename T>
+__device__ __forceinline__ T *texel(T *p, int pitch, int x, int y) {
+  if constexpr (O32)
+    return (T *)((char *)p + (__umul24((uint32_t)y, (uint32_t)pitch) + (uint32_t)x) * (uint32_t)sizeof(T));
+  else
+    return p + (size_t)y * pitch + x;
+}
+template <bool O32> __device__ __forceinline__ uint2 ldr(const Tex &t, int x, int y) { return *texel<O32>(t.p, t.pitch, x, y); }
+template <bool O32> __device__ __forceinline__ void st(const Tex &t, int x, int y, float4 v) { *texel<O32>(t.p, t.pitch, x, y) = pack4(v); }
+template <bool O32> __device__ __forceinline__ void stu(const TexU &t, int x, int y, uint4 v) { *texel<O32>(t.p, t.pitch, x, y) = v; }
+template <bool O32> __device__ __forceinline__ void stf(const TexF &t, int x, int y, float4 v) { *texel<O32>(t.p, t.pitch, x, y) = v; }
+template <bool O32> __device__ __forceinline__ float4 ldf(const TexF &t, int x, int y) { return *texel<O32>(t.p, t.pitch, x, y); }
 // low / high binary16 of a packed pair kept in a float4 lane
 __device__ __forceinline__ float2 pair_of(float w) { return unpack2(__float_as_uint(w)); }
 
@@ -82,7 +99,7 @@ __device__ __forceinline__ int f2i(float x) {
 }
 // i32 addition with WGSL's two's-complement wrap-around (no signed-overflow assumption)
 __device__ __forceinline__ int addw(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
-__device__ __forceinline__ bool inside(int x, int y, int w, int h) { return x >= 0 && y >= 0 && x < w && y < h; }
+__device__ __forceinline__ bool inside(int x, int y, int w, int h) { return (x >= 0) & (y >= 0) & (x < w) & (y < h); }
 
 // WGSL min/max on non-NaN operands
 __device__ __forceinline__ float mn(float a, float b) { return a < b ? a : b; }

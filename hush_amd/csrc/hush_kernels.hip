@@ -21,20 +21,24 @@ constexpr int TW = 32, TH = 8, NT = 256;
 // store A0 / mom / key; then per 2x2 block the 20 MLP features and the half-res guide (F0..F3).
 
 // check (hush.js:103-105) on a decoded key texel: xyz = octDec(k.xy) (exact), w = binary16 (k.z, k.w)
+// Every term is evaluated and the results combined without short-circuit: the test is branch-free, so a
+// caller's per-tap accumulation needs no nested divergent blocks.
 __device__ __forceinline__ bool check(float4 kd, float3 n, float zprev, float obj) {
   const float2 zi = pair_of(kd.w);
-  if (!(zi.x >= 0.0f)) return false;
-  if (!(zi.y == obj)) return false;
   float tol = 0.05f * zprev;
   tol = tol + 0.01f;
-  if (!(fabsf(zi.x - zprev) <= tol)) return false;
-  return dot3(make_float3(kd.x, kd.y, kd.z), n) >= 0.9f;
+  return (zi.x >= 0.0f) & (zi.y == obj) & (fabsf(zi.x - zprev) <= tol) & (dot3(make_float3(kd.x, kd.y, kd.z), n) >= 0.9f);
 }
 
 // per-pixel feature inputs kept in LDS for the block phase (structure-of-arrays, conflict-free).  P0.y
 // (E qf16(lum)) is the staged sQ of the pixel and is read from there; the material P3 is kept as its two
 // packed binary16 words (its exact input bits): 18 rows instead of 21, fewer LDS writes and reads.
 enum { P0X, P0Z, P0W, P1X, P1Y, P1Z, P1W, P2Z, P2W, NSX, NSY, NSZ, P3A, P3B, P4X, P4Y, P4Z, P4W, NP };
+// LDS slot of pixel t (t = 32 ly + lx) in the sP rows: bit 0 flipped by bit 6 of t and bit 5 by bit 7 (a
+// permutation of 0 .. 255).  The block phase's lane (bxl, byl) reads pixel 64 byl + 32 jy + 2 bxl + jx: the
+// 64 lanes then hit 64 distinct banks (4 lanes per bank without the flips); a wave's own writes stay
+// conflict free.
+__device__ __forceinline__ int pslot(int t) { return t ^ (((t >> 6) & 1) | (((t >> 7) & 1) << 5)); }
 
 // Temporal statistics (moments, variances, cancellation-prone differences such as m2 - m1^2) are
 // ill-conditioned, so this pass is evaluated with IEEE arithmetic under both math policies.
@@ -83,9 +87,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   // its unpacking, between it and the other three)
   uint2 ndr = make_uint2(0u, 0xbc000000u), mvr = make_uint2(0u, 0u), rdr = mvr, mraw = mvr;   // nd = (0, 0, 0, -1)
   if (in_img) {
-    const size_t o = (size_t)py;
-    ndr = a.nd.p[o * a.nd.pitch + px]; mvr = a.mv.p[o * a.mv.pitch + px]; rdr = a.rad.p[o * a.rad.pitch + px];
-    mraw = a.mat.p[o * a.mat.pitch + px];
+    ndr = ldr<BUF>(a.nd, px, py); mvr = ldr<BUF>(a.mv, px, py); rdr = ldr<BUF>(a.rad, px, py);
+    mraw = ldr<BUF>(a.mat, px, py);
     __builtin_amdgcn_sched_barrier(0);
   }
   // the 84 apron texels of the staged tile (threads 0..83): their normalDepth, radiance and material are
@@ -102,9 +105,9 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     const int qx = x0 - 1 + atx, qy = y0 - 1 + aty;
     a_in = inside(qx, qy, a.W, a.H);
     if (a_in) {
-      a_nd = a.nd.p[(size_t)qy * a.nd.pitch + qx];
-      a_rd = a.rad.p[(size_t)qy * a.rad.pitch + qx];
-      a_mt = a.mat.p[(size_t)qy * a.mat.pitch + qx];
+      a_nd = ldr<BUF>(a.nd, qx, qy);
+      a_rd = ldr<BUF>(a.rad, qx, qy);
+      a_mt = ldr<BUF>(a.mat, qx, qy);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -201,15 +204,18 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
 #pragma unroll
       for (int i = 0; i < 4; i++) { c4[i] = unpack4(c4r[i]); m4[i] = unpack4(m4r[i]); }
       const float2 pq = unpack2(pqr.x);
+      // a rejected tap enters the sums with weight 0: the stored history is finite (binary16 stores clamp to
+      // +-65504), so it adds a zero, which leaves every running sum unchanged bit for bit (each starts at +0
+      // and never becomes -0); branch-free, with no divergent block per tap
       float sumW = 0.0f;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        if (wts[i] > 0.0f && inside(addw(bix, i & 1), addw(biy, i >> 1), W, H) && check(k4[i], n, mv.w, mv.z)) {
-          mbits |= 1u << i;
-          sumW += wts[i];
-          hcx += wts[i] * c4[i].x; hcy += wts[i] * c4[i].y; hcz += wts[i] * c4[i].z;
-          hm.x += wts[i] * m4[i].x; hm.y += wts[i] * m4[i].y; hm.z += wts[i] * m4[i].z; hm.w += wts[i] * m4[i].w;
-        }
+        const bool ok = (wts[i] > 0.0f) & inside(addw(bix, i & 1), addw(biy, i >> 1), W, H) & check(k4[i], n, mv.w, mv.z);
+        mbits |= (uint32_t)ok << i;
+        const float w = ok ? wts[i] : 0.0f;
+        sumW += w;
+        hcx += w * c4[i].x; hcy += w * c4[i].y; hcz += w * c4[i].z;
+        hm.x += w * m4[i].x; hm.y += w * m4[i].y; hm.z += w * m4[i].z; hm.w += w * m4[i].w;
       }
       if (sumW >= 0.01f) {
         const float rs = F::rcpn(sumW);
@@ -225,18 +231,18 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         float4 kq[9];
 #pragma unroll
         for (int j = 0; j < 9; j++)
-          kq[j] = ldf(a.pkd, clampi(addw(cx, j % 3 - 1), 0, W - 1), clampi(addw(cy, j / 3 - 1), 0, H - 1));
+          kq[j] = ldf<BUF>(a.pkd, clampi(addw(cx, j % 3 - 1), 0, W - 1), clampi(addw(cy, j / 3 - 1), 0, H - 1));
         uint32_t okm = 0;
 #pragma unroll
         for (int j = 0; j < 9; j++)
-          if (inside(addw(cx, j % 3 - 1), addw(cy, j / 3 - 1), W, H) && check(kq[j], n, mv.w, mv.z)) okm |= 1u << j;
+          okm |= (uint32_t)(inside(addw(cx, j % 3 - 1), addw(cy, j / 3 - 1), W, H) & check(kq[j], n, mv.w, mv.z)) << j;
         uint2 cq[9], mq[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) {
           cq[j] = mq[j] = make_uint2(0u, 0u);
           if (okm >> j & 1u) {
             const int qx = addw(cx, j % 3 - 1), qy = addw(cy, j / 3 - 1);
-            cq[j] = a.pcol.p[(size_t)qy * a.pcol.pitch + qx]; mq[j] = a.pmom.p[(size_t)qy * a.pmom.pitch + qx];
+            cq[j] = ldr<BUF>(a.pcol, qx, qy); mq[j] = ldr<BUF>(a.pmom, qx, qy);
           }
         }
 #pragma unroll
@@ -263,16 +269,16 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   __syncthreads();
 
   // feature inputs go straight to LDS (sky / outside: zeros with depth -1, hush.js:112, 297)
-  float *P = &sP[0][tid];
+  float *P = &sP[0][pslot(tid)];
   bool p_written = false;
   if (in_img) {
     if (nd.w < 0.0f) {                                   // sky (hush.js:115-119)
-      st(a.outA, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-      st(a.outMom, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+      st<BUF>(a.outA, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+      st<BUF>(a.outMom, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
       const uint32_t zid = pack_rtz(-1.0f, mv.z);
-      a.outKey.p[(size_t)py * a.outKey.pitch + px] = make_uint2(0u, zid);
+      *texel<BUF>(a.outKey.p, a.outKey.pitch, px, py) = make_uint2(0u, zid);
       const float3 d0 = oct_dec(0.0f, 0.0f);
-      stf(a.outKD, px, py, make_float4(d0.x, d0.y, d0.z, __uint_as_float(zid)));
+      stf<BUF>(a.outKD, px, py, make_float4(d0.x, d0.y, d0.z, __uint_as_float(zid)));
       mbits = 0x80;
       if (AUX) a.aux[((size_t)py * a.W + px) * TEMPORAL_AUX] = -1.0f;
     } else {
@@ -338,14 +344,14 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         for (int k = 0; k < TEMPORAL_AUX; k++) x[k] = v[k];
       }
       float2 oc = oct_enc(n);
-      st(a.outA, px, py, A);
-      st(a.outMom, px, py, make_float4(nmx, nmy, nfast, len1));
+      st<BUF>(a.outA, px, py, A);
+      st<BUF>(a.outMom, px, py, make_float4(nmx, nmy, nfast, len1));
       // the key as stored, and its exact decode for the next frame's check and this frame's fine level
       const uint32_t okp = pack_rtz(oc.x, oc.y), zid = pack_rtz(nd.w, mv.z);
-      a.outKey.p[(size_t)py * a.outKey.pitch + px] = make_uint2(okp, zid);
+      *texel<BUF>(a.outKey.p, a.outKey.pitch, px, py) = make_uint2(okp, zid);
       const float2 ok = unpack2(okp);
       const float3 dec = oct_dec(ok.x, ok.y);
-      stf(a.outKD, px, py, make_float4(dec.x, dec.y, dec.z, __uint_as_float(zid)));
+      stf<BUF>(a.outKD, px, py, make_float4(dec.x, dec.y, dec.z, __uint_as_float(zid)));
       // feature inputs from the stored values (hush.js:212-222)
       float aqx = qf16(A.x), aqy = qf16(A.y), aqz = qf16(A.z), aqw = qf16(A.w);
       float m1 = E * qf16(nmx);
@@ -360,7 +366,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       P[P4X * NT] = aqx; P[P4Y * NT] = aqy; P[P4Z * NT] = aqz; P[P4W * NT] = aqw;
       p_written = true;
     }
-    if (a.mask) a.mask[(size_t)py * a.mask_pitch + px] = (uint8_t)mbits;
+    if (a.mask) *texel<BUF>(a.mask, a.mask_pitch, px, py) = (uint8_t)mbits;
   }
   if (!p_written) {
 #pragma unroll
@@ -378,39 +384,39 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   Sum<KAHAN> qS0[4], qS1[4], qS3[4], qAC[4], qNx, qNy, qNz, qMV, qZ;
   bool ok[4];
   float la[4], zs[4], nsx[4], nsy[4], nsz[4];
+  // A sky or outside pixel's feature inputs are stored as zeros (its depth as -1), so it enters every sum as
+  // +0, which leaves the running sums unchanged bit for bit (each starts at +0 and never becomes -0); only its
+  // depth and staged luminance are masked.  Straight-line code, no divergent block per pixel.
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int t = byl * 64 + (j >> 1) * 32 + 2 * bxl + (j & 1);
-    ok[j] = sP[P2Z][t] >= 0.0f;
-    la[j] = zs[j] = nsx[j] = nsy[j] = nsz[j] = 0.0f;
-    if (!ok[j]) continue;
-    nv += 1.0f;
-    la[j] = sP[P0X][t]; zs[j] = sP[P2Z][t];
+    const int t = pslot(byl * 64 + (j >> 1) * 32 + 2 * bxl + (j & 1));
+    const int tl = (2 * byl + (j >> 1) + 1) * SW + 2 * bxl + (j & 1) + 1;    // the pixel's staged tile index
+    const float z = sP[P2Z][t];
+    ok[j] = z >= 0.0f;
+    nv += ok[j] ? 1.0f : 0.0f;
+    la[j] = sP[P0X][t]; zs[j] = ok[j] ? z : 0.0f;
     nsx[j] = sP[NSX][t]; nsy[j] = sP[NSY][t]; nsz[j] = sP[NSZ][t];
-    {
-      const int tl = (2 * byl + (j >> 1) + 1) * SW + 2 * bxl + (j & 1) + 1;    // the pixel's staged tile index
-      const float2 m01 = unpack2(__float_as_uint(sP[P3A][t])), m23 = unpack2(__float_as_uint(sP[P3B][t]));
-      const float p0[4] = {sP[P0X][t], sQ[tl], sP[P0Z][t], sP[P0W][t]}, p3[4] = {m01.x, m01.y, m23.x, m23.y};
+    const float2 m01 = unpack2(__float_as_uint(sP[P3A][t])), m23 = unpack2(__float_as_uint(sP[P3B][t]));
+    const float p0[4] = {la[j], ok[j] ? sQ[tl] : 0.0f, sP[P0Z][t], sP[P0W][t]}, p3[4] = {m01.x, m01.y, m23.x, m23.y};
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        qS0[c].add(p0[c]); qS1[c].add(sP[P1X + c][t]); qS3[c].add(p3[c]); qAC[c].add(sP[P4X + c][t]);
-      }
+    for (int c = 0; c < 4; c++) {
+      qS0[c].add(p0[c]); qS1[c].add(sP[P1X + c][t]); qS3[c].add(p3[c]); qAC[c].add(sP[P4X + c][t]);
     }
     qNx.add(nsx[j]); qNy.add(nsy[j]); qNz.add(nsz[j]);
-    qMV.add(sP[P2W][t]); qZ.add(sP[P2Z][t]);
+    qMV.add(sP[P2W][t]); qZ.add(zs[j]);
   }
   float S0[4], S1[4], S3[4], AC[4];
 #pragma unroll
   for (int c = 0; c < 4; c++) { S0[c] = qS0[c].get(); S1[c] = qS1[c].get(); S3[c] = qS3[c].get(); AC[c] = qAC[c].get(); }
   float Nx = qNx.get(), Ny = qNy.get(), Nz = qNz.get(), MV = qMV.get(), Z = qZ.get();
   if (nv == 0.0f) {
-    stu(a.F0, X, Y, make_uint4(0, 0, 0, 0));
-    stu(a.F1, X, Y, make_uint4(0, 0, 0, 0));
-    stu(a.F2, X, Y, make_uint4(0, 0, 0, 0));
+    stu<BUF>(a.F0, X, Y, make_uint4(0, 0, 0, 0));
+    stu<BUF>(a.F1, X, Y, make_uint4(0, 0, 0, 0));
+    stu<BUF>(a.F2, X, Y, make_uint4(0, 0, 0, 0));
     const uint4 g3 = make_uint4(pk(0.0f, 0.0f), pk(-1.0f, 0.0f), 0u, 0u);
-    stu(a.F3, X, Y, g3);
+    stu<BUF>(a.F3, X, Y, g3);
     const float3 d0 = oct_dec(0.0f, 0.0f);
-    stf(a.outGH, X, Y, make_float4(d0.x, d0.y, d0.z, __uint_as_float(g3.y)));
+    stf<BUF>(a.outGH, X, Y, make_float4(d0.x, d0.y, d0.z, __uint_as_float(g3.y)));
     return;
   }
   const float k = F::rcpn(nv);
@@ -420,7 +426,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   const float Lacc = S0[0];
   Sum<KAHAN> qVA;
 #pragma unroll
-  for (int j = 0; j < 4; j++) if (ok[j]) qVA.add((la[j] - Lacc) * (la[j] - Lacc));
+  for (int j = 0; j < 4; j++) qVA.add(ok[j] ? (la[j] - Lacc) * (la[j] - Lacc) : 0.0f);
   float VA = qVA.get() * k;
   // 4x4 neighbourhood (block +-1) of the exposed, truncated luminance (hush.js:253-265)
   Sum<KAHAN> q1;
@@ -448,25 +454,25 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   const float3 Nn = make_float3(F::divsf(Nx, ln, rln), F::divsf(Ny, ln, rln), F::divsf(Nz, ln, rln));
   float zmax = -1e30f, zmin = 1e30f, mind = 1e30f;
 #pragma unroll
-  for (int j = 0; j < 4; j++)
-    if (ok[j]) {
-      zmax = mx(zmax, zs[j]); zmin = mn(zmin, zs[j]);
-      mind = mn(mind, dot3(make_float3(nsx[j], nsy[j], nsz[j]), Nn));
-    }
+  for (int j = 0; j < 4; j++) {
+    const float d = dot3(make_float3(nsx[j], nsy[j], nsz[j]), Nn);
+    zmax = ok[j] ? mx(zmax, zs[j]) : zmax; zmin = ok[j] ? mn(zmin, zs[j]) : zmin;
+    mind = ok[j] ? mn(mind, d) : mind;
+  }
   const float dz = mn(mx(F::div(zmax - zmin, mx(zmin, 1e-6f)), 0.0f), 1.0f);
   const float dn = mn(mx(1.0f - mind, 0.0f), 2.0f);
   const float2 oc = oct_enc_m<FAST>(Nn);
   float f[20] = {log1p_<FAST>(Lacc), log1p_<FAST>(S0[1]), log1p_<FAST>(S0[2]), log1p_<FAST>(var4), S0[3] / 32.0f, S1[0],
                  log1p_<FAST>(S1[1]), log1p_<FAST>(F::div(fabsf(S1[2] - S1[3]), S1[3] + 1e-3f)), oc.x, oc.y, dz, dn, S3[0],
                  S3[1], S3[2], S3[3], 1.0f - nv / 4.0f, log1p_<FAST>(MV), log1p_<FAST>(mu4), log1p_<FAST>(VA)};
-  stu(a.F0, X, Y, make_uint4(pk(f[0], f[1]), pk(f[2], f[3]), pk(f[4], f[5]), pk(f[6], f[7])));
-  stu(a.F1, X, Y, make_uint4(pk(f[8], f[9]), pk(f[10], f[11]), pk(f[12], f[13]), pk(f[14], f[15])));
-  stu(a.F2, X, Y, make_uint4(pk(f[16], f[17]), pk(f[18], f[19]), pk(AC[0], AC[1]), pk(AC[2], AC[3])));
+  stu<BUF>(a.F0, X, Y, make_uint4(pk(f[0], f[1]), pk(f[2], f[3]), pk(f[4], f[5]), pk(f[6], f[7])));
+  stu<BUF>(a.F1, X, Y, make_uint4(pk(f[8], f[9]), pk(f[10], f[11]), pk(f[12], f[13]), pk(f[14], f[15])));
+  stu<BUF>(a.F2, X, Y, make_uint4(pk(f[16], f[17]), pk(f[18], f[19]), pk(AC[0], AC[1]), pk(AC[2], AC[3])));
   const uint4 g3 = make_uint4(pk(oc.x, oc.y), pk(Z, nv), 0u, 0u);
-  stu(a.F3, X, Y, g3);
+  stu<BUF>(a.F3, X, Y, g3);
   const float2 go = unpack2(g3.x);
   const float3 gd = oct_dec(go.x, go.y);
-  stf(a.outGH, X, Y, make_float4(gd.x, gd.y, gd.z, __uint_as_float(g3.y)));
+  stf<BUF>(a.outGH, X, Y, make_float4(gd.x, gd.y, gd.z, __uint_as_float(g3.y)));
 }
 
 // ============================================================================ edge-aware a-trous
@@ -1513,11 +1519,17 @@ static void launch_temporal_t(const TemporalArgs &a, hipStream_t s) {
   }
 }
 hipError_t launch_temporal(const TemporalArgs &a, bool, hipStream_t s) {
-  // buffer descriptors take 32-bit offsets and record counts, passed as int: every history surface must stay
-  // below 2 GiB so that both stay non-negative (larger surfaces take the 64-bit address form)
+  // buffer descriptors take 32-bit offsets and record counts, passed as int, and the other accesses 32-bit
+  // byte offsets from the surface base: every surface the pass touches must stay below 2 GiB (larger
+  // surfaces take the 64-bit address form)
   const uint64_t lim = (uint64_t)1 << 31;
-  const bool buf = !a.wide && (uint64_t)a.pkd.pitch * a.H * 16 < lim && (uint64_t)a.pcol.pitch * a.H * 8 < lim &&
-                   (uint64_t)a.pmom.pitch * a.H * 8 < lim && (uint64_t)a.ppar0.pitch * a.H2 * 8 < lim;
+  auto small = [&](int pitch, int rows, int texel) { return (uint64_t)pitch * (uint64_t)rows * (uint64_t)texel < lim; };
+  const bool buf = !a.wide && small(a.pkd.pitch, a.H, 16) && small(a.pcol.pitch, a.H, 8) && small(a.pmom.pitch, a.H, 8) &&
+                   small(a.ppar0.pitch, a.H2, 8) && small(a.rad.pitch, a.H, 8) && small(a.nd.pitch, a.H, 8) &&
+                   small(a.mv.pitch, a.H, 8) && small(a.mat.pitch, a.H, 8) && small(a.outA.pitch, a.H, 8) &&
+                   small(a.outMom.pitch, a.H, 8) && small(a.outKey.pitch, a.H, 8) && small(a.outKD.pitch, a.H, 16) &&
+                   small(a.F0.pitch, a.H2, 16) && small(a.F1.pitch, a.H2, 16) && small(a.F2.pitch, a.H2, 16) &&
+                   small(a.F3.pitch, a.H2, 16) && small(a.outGH.pitch, a.H2, 16) && (!a.mask || small(a.mask_pitch, a.H, 1));
   if (buf) launch_temporal_t<true>(a, s);
   else launch_temporal_t<false>(a, s);
   return hipGetLastError();
