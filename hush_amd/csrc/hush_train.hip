@@ -928,21 +928,24 @@ __global__ __launch_bounds__(NTH) void tk_blend(const float *aux, const T *pprev
   st4(a0, i, V4<T>{col[0], col[1], col[2], svar + (tvar - svar) * tmin(len1 / T(4), T(1))});
 }
 
-// dL/dalpha, dL/dgamma of the previous frame's half-res texels from dL/dA0 (added atomically: the
-// reprojected texel of a pixel follows its motion vector)
+// dL/dalpha, dL/dgamma of the previous frame's half-res texels from dL/dA0.  A pixel's reprojected texel
+// follows its motion vector, so several pixels may feed one texel in any order; the sums are made
+// order-independent (and so bit-reproducible) with 64-bit fixed point: pass 1 finds the largest magnitude of
+// any contribution (an exact maximum over the float bit patterns), pass 2 adds every contribution, scaled by
+// a power of two that keeps the sum of all of them below 2^62, as an integer, and a per-texel pass converts
+// the two sums back.  The fixed point keeps 2^-62 of (pixels x the largest contribution).
 template <class T>
-__global__ __launch_bounds__(NTH) void tk_blend_bwd(const float *aux, const T *pprev, const T *ga0, int n, T *gprev) {
-  const int i = tid1();
-  if (i >= n) return;
+__device__ __forceinline__ bool blend_bwd_terms(const float *aux, const T *pprev, const T *ga0, int i, int &q, T &daa,
+                                                T &dga) {
   const float *x = aux + (size_t)i * TEMPORAL_AUX;
-  if (!(x[0] > 0.0f)) return;
+  daa = dga = T(0);
+  if (!(x[0] > 0.0f)) return false;
   const V4<T> g = ld4(ga0, i);
   const T gv[3] = {g.x, g.y, g.z};
-  const int q = (int)x[18];
+  q = (int)x[18];
   const T al = pprev[12 * q], ga = pprev[12 * q + 1];
   const T l = x[16], len1 = tmin((T)x[15] + T(1), T(32));
   const T aa = tmax(al, T(1) / len1);
-  T daa = T(0), dga = T(0);
   for (int k = 0; k < 3; k++) {
     const T sd = x[7 + k], mu = x[4 + k], hc = x[10 + k];
     const T lo = mu - ga * sd, hi = mu + ga * sd;
@@ -955,8 +958,59 @@ __global__ __launch_bounds__(NTH) void tk_blend_bwd(const float *aux, const T *p
   }
   const T nmx = (T)x[13] + (l - (T)x[13]) * aa, nmy = (T)x[14] + (l * l - (T)x[14]) * aa;
   if (nmy - nmx * nmx > T(0)) daa += g.w * tmin(len1 / T(4), T(1)) * ((l * l - (T)x[14]) - T(2) * nmx * (l - (T)x[13]));
-  if (al > T(1) / len1) atomicAdd(&gprev[12 * q], daa);
-  atomicAdd(&gprev[12 * q + 1], dga);
+  if (!(al > T(1) / len1)) daa = T(0);          // alpha below 1 / len1 does not reach the blend
+  return true;
+}
+__device__ __forceinline__ unsigned long long mag_bits(double v) { return (unsigned long long)__double_as_longlong(fabs(v)); }
+// pass 1: the largest |contribution| (non-negative doubles order like their bit patterns)
+template <class T>
+__global__ __launch_bounds__(NTH) void tk_blend_bwd_max(const float *aux, const T *pprev, const T *ga0, int n,
+                                                        unsigned long long *mx) {
+  const int i = tid1();
+  if (i >= n) return;
+  int q;
+  T daa, dga;
+  if (!blend_bwd_terms(aux, pprev, ga0, i, q, daa, dga)) return;
+  const unsigned long long m = max(mag_bits((double)daa), mag_bits((double)dga));
+  if (m) atomicMax(mx, m);
+}
+// the power of two s with n x max|contribution| x s < 2^62 (at most 2^1000); 0 when every contribution is
+// zero, -1 when one is not finite
+__device__ __forceinline__ double fixed_scale(unsigned long long mxb, int n) {
+  if (!mxb) return 0.0;
+  if (mxb >= 0x7ff0000000000000ull) return -1.0;
+  int e;
+  (void)frexp(__longlong_as_double((long long)mxb), &e);     // max < 2^e
+  const int L = 32 - __clz(n);                                // n < 2^L
+  return ldexp(1.0, min(62 - e - L, 1000));
+}
+// pass 2: the contributions as integers, summed exactly
+template <class T>
+__global__ __launch_bounds__(NTH) void tk_blend_bwd_add(const float *aux, const T *pprev, const T *ga0, int n,
+                                                        const unsigned long long *mx, long long *acc) {
+  const int i = tid1();
+  if (i >= n) return;
+  int q;
+  T daa, dga;
+  if (!blend_bwd_terms(aux, pprev, ga0, i, q, daa, dga)) return;
+  const double sc = fixed_scale(*mx, n);
+  if (!(sc > 0.0)) return;
+  const long long ia = (long long)rint((double)daa * sc), ig = (long long)rint((double)dga * sc);
+  if (ia) atomicAdd((unsigned long long *)&acc[2 * q], (unsigned long long)ia);
+  if (ig) atomicAdd((unsigned long long *)&acc[2 * q + 1], (unsigned long long)ig);
+}
+// pass 3: the sums into the previous frame's parameter gradient (the other ten entries zero)
+template <class T>
+__global__ __launch_bounds__(NTH) void tk_blend_bwd_store(const long long *acc, const unsigned long long *mx, int n, int n2,
+                                                          T *gprev) {
+  const int q = tid1();
+  if (q >= n2) return;
+  const double sc = fixed_scale(*mx, n);
+  T *g = gprev + 12 * (size_t)q;
+  const T bad = sc < 0.0 ? (T)NAN : T(0);                    // a non-finite contribution poisons the gradient
+  g[0] = sc > 0.0 ? (T)((double)acc[2 * q] / sc) : bad;
+  g[1] = sc > 0.0 ? (T)((double)acc[2 * q + 1] / sc) : bad;
+  for (int k = 2; k < 12; k++) g[k] = T(0);
 }
 
 // (alpha, gamma, beta, kappa) of a stored binary16 par0 texel into the 12-wide parameter layout
@@ -1024,6 +1078,7 @@ struct hush_train_ctx {
   uint4 *Fprev[3] = {};                 // the previous frame's features F0..F2 (half res)
   float4 *GHprev = nullptr;             // and its half-res guide (sky blocks)
   void *oprev = nullptr, *pprev = nullptr, *gpprev = nullptr, *ga0 = nullptr;
+  long long *bacc = nullptr;            // fixed-point blend-gradient sums (2 per half-res texel) + the maximum
   bool has_prev = false;                // Fprev holds the committed previous frame
   int prev_mode = 0;                    // this frame: 0 no history, 1 alpha/gamma from the stored par0, 2 recomputed
   bool a0_fixed = false;
@@ -1057,9 +1112,10 @@ extern "C" int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out
   const size_t N = (size_t)t->W * t->H, N2 = (size_t)t->W2 * t->H2, ts = t->tsz;
   t->nwb = (int)((N2 + (size_t)MB * (fp64 ? 1 : MT) - 1) / ((size_t)MB * (fp64 ? 1 : MT)));
   t->nlb = (int)((N + NTH - 1) / NTH);
-  // half: o, par, gpar, coefH (12 each), mid x3, gmid x3 (4 each); full: a0, fine, X, prev, gX, gfine, gill, gpf
-  // (4 each), taac, coefF (12 each); FP64 partials
-  const size_t half = N2 * ts * (7 * 12 + 6 * 4) + N2 * 16 * 4, full = N * ts * (9 * 4 + 2 * 12) + N * 4 * TEMPORAL_AUX;
+  // half: o, par, gpar, coefH (12 each), mid x3, gmid x3 (4 each), the fixed-point blend sums; full: a0, fine,
+  // X, prev, gX, gfine, gill, gpf (4 each), taac, coefF (12 each); FP64 partials
+  const size_t half = N2 * ts * (7 * 12 + 6 * 4) + N2 * 16 * 4 + (N2 * 2 + 1) * 8,
+               full = N * ts * (9 * 4 + 2 * 12) + N * 4 * TEMPORAL_AUX;
   const size_t parts = ((size_t)2 * t->nwb * NW + (size_t)WSL * NW + t->nlb + 64) * sizeof(double);
   const size_t total = half + full + parts + 64 * 256;
   e = hipMalloc(&t->arena, total);
@@ -1079,6 +1135,7 @@ extern "C" int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out
   t->taac = take(N * ts * 12); t->coefF = take(N * ts * 12);
   t->oprev = take(N2 * ts * 12); t->pprev = take(N2 * ts * 12); t->gpprev = take(N2 * ts * 12); t->ga0 = take(N * ts * 4);
   t->aux = (float *)take(N * 4 * TEMPORAL_AUX);
+  t->bacc = (long long *)take((N2 * 2 + 1) * 8);
   for (int k = 0; k < 3; k++) t->Fprev[k] = (uint4 *)take(N2 * 16);
   t->GHprev = (float4 *)take(N2 * 16);
   t->wpart = (double *)take(((size_t)2 * t->nwb + WSL) * NW * sizeof(double));   // partials, then the slice sums
@@ -1369,9 +1426,14 @@ static hipError_t backward_t(hush_train_t *t, const T *w, const T *gout, T *gw, 
   int nb = t->nwb;
   if (chain) {
     // temporal blend -> the previous frame's alpha / gamma -> its MLP (features of that frame)
-    hipLaunchKernelGGL(tk_zero<T>, grid1(N2 * 12), dim3(NTH), 0, s, (T *)t->gpprev, N2 * 12);
-    hipLaunchKernelGGL(tk_blend_bwd<T>, grid1(N), dim3(NTH), 0, s, (const float *)t->aux, (const T *)t->pprev, (const T *)ga, (int)N,
-                       (T *)t->gpprev);
+    hipLaunchKernelGGL(tk_zero<long long>, grid1(N2 * 2 + 1), dim3(NTH), 0, s, t->bacc, (long)(N2 * 2 + 1));
+    unsigned long long *mxp = (unsigned long long *)(t->bacc + N2 * 2);
+    hipLaunchKernelGGL(tk_blend_bwd_max<T>, grid1(N), dim3(NTH), 0, s, (const float *)t->aux, (const T *)t->pprev, (const T *)ga,
+                       (int)N, mxp);
+    hipLaunchKernelGGL(tk_blend_bwd_add<T>, grid1(N), dim3(NTH), 0, s, (const float *)t->aux, (const T *)t->pprev, (const T *)ga,
+                       (int)N, (const unsigned long long *)mxp, t->bacc);
+    hipLaunchKernelGGL(tk_blend_bwd_store<T>, grid1(N2), dim3(NTH), 0, s, (const long long *)t->bacc,
+                       (const unsigned long long *)mxp, (int)N, (int)N2, (T *)t->gpprev);
     launch_mlp_bwd<T>(t, fprev_tex(t, 0), fprev_tex(t, 1), fprev_tex(t, 2), TexF{t->GHprev, t->W2}, w, (const T *)t->oprev,
                       (const T *)t->pprev, (const T *)t->gpprev, t->wpart + (size_t)t->nwb * NW, s);
     nb *= 2;
