@@ -930,10 +930,11 @@ __global__ __launch_bounds__(NTH) void tk_blend(const float *aux, const T *pprev
 
 // dL/dalpha, dL/dgamma of the previous frame's half-res texels from dL/dA0.  A pixel's reprojected texel
 // follows its motion vector, so several pixels may feed one texel in any order; the sums are made
-// order-independent (and so bit-reproducible) with 64-bit fixed point: pass 1 finds the largest magnitude of
-// any contribution (an exact maximum over the float bit patterns), pass 2 adds every contribution, scaled by
-// a power of two that keeps the sum of all of them below 2^62, as an integer, and a per-texel pass converts
-// the two sums back.  The fixed point keeps 2^-62 of (pixels x the largest contribution).
+// order-independent (and so bit-reproducible) with 64-bit fixed point: pass 1 stores every pixel's
+// contributions and finds their largest magnitude (an exact maximum over the float bit patterns), pass 2
+// adds every contribution as an integer, scaled by a power of two that keeps the sum of all of them below
+// 2^62, and a per-texel pass converts the two sums back.  The fixed point keeps 2^-62 of (pixels x the
+// largest contribution).
 template <class T>
 __device__ __forceinline__ bool blend_bwd_terms(const float *aux, const T *pprev, const T *ga0, int i, int &q, T &daa,
                                                 T &dga) {
@@ -962,17 +963,46 @@ __device__ __forceinline__ bool blend_bwd_terms(const float *aux, const T *pprev
   return true;
 }
 __device__ __forceinline__ unsigned long long mag_bits(double v) { return (unsigned long long)__double_as_longlong(fabs(v)); }
-// pass 1: the largest |contribution| (non-negative doubles order like their bit patterns)
+// pass 1: each pixel's two contributions and target texel (q = -1: none), and each workgroup's largest
+// |contribution| into mx[blockIdx] (non-negative doubles order like their bit patterns)
 template <class T>
-__global__ __launch_bounds__(NTH) void tk_blend_bwd_max(const float *aux, const T *pprev, const T *ga0, int n,
-                                                        unsigned long long *mx) {
+__global__ __launch_bounds__(NTH) void tk_blend_bwd_max(const float *aux, const T *pprev, const T *ga0, int n, T *terms,
+                                                        int *tq, unsigned long long *mx) {
   const int i = tid1();
-  if (i >= n) return;
-  int q;
-  T daa, dga;
-  if (!blend_bwd_terms(aux, pprev, ga0, i, q, daa, dga)) return;
-  const unsigned long long m = max(mag_bits((double)daa), mag_bits((double)dga));
-  if (m) atomicMax(mx, m);
+  unsigned long long m = 0;
+  if (i < n) {
+    int q = -1;
+    T daa, dga;
+    if (!blend_bwd_terms(aux, pprev, ga0, i, q, daa, dga)) q = -1;
+    terms[2 * (size_t)i] = daa; terms[2 * (size_t)i + 1] = dga; tq[i] = q;
+    m = max(mag_bits((double)daa), mag_bits((double)dga));
+  }
+  // the workgroup's maximum (waves by lane shuffles, every lane taking part, then across the four waves)
+  // into its partial; tk_max_bits reduces the partials
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned lo = __shfl_xor((unsigned)m, off), hi = __shfl_xor((unsigned)(m >> 32), off);
+    m = max(m, ((unsigned long long)hi << 32) | lo);
+  }
+  __shared__ unsigned long long wm[NTH / 64];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < NTH / 64; k++) m = max(m, wm[k]);
+    mx[blockIdx.x] = m;
+  }
+}
+// the largest of n partial maxima, by one workgroup: mx[n] = max(mx[0 .. n-1])
+__global__ __launch_bounds__(NTH) void tk_max_bits(unsigned long long *mx, int n) {
+  unsigned long long m = 0;
+  for (int k = threadIdx.x; k < n; k += NTH) m = max(m, mx[k]);
+  __shared__ unsigned long long sm[NTH];
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = NTH / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sm[threadIdx.x] = max(sm[threadIdx.x], sm[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mx[n] = sm[0];
 }
 // the power of two s with n x max|contribution| x s < 2^62 (at most 2^1000); 0 when every contribution is
 // zero, -1 when one is not finite
@@ -986,13 +1016,13 @@ __device__ __forceinline__ double fixed_scale(unsigned long long mxb, int n) {
 }
 // pass 2: the contributions as integers, summed exactly
 template <class T>
-__global__ __launch_bounds__(NTH) void tk_blend_bwd_add(const float *aux, const T *pprev, const T *ga0, int n,
-                                                        const unsigned long long *mx, long long *acc) {
+__global__ __launch_bounds__(NTH) void tk_blend_bwd_add(const T *terms, const int *tq, int n, const unsigned long long *mx,
+                                                        long long *acc) {
   const int i = tid1();
   if (i >= n) return;
-  int q;
-  T daa, dga;
-  if (!blend_bwd_terms(aux, pprev, ga0, i, q, daa, dga)) return;
+  const int q = tq[i];
+  if (q < 0) return;
+  const T daa = terms[2 * (size_t)i], dga = terms[2 * (size_t)i + 1];
   const double sc = fixed_scale(*mx, n);
   if (!(sc > 0.0)) return;
   const long long ia = (long long)rint((double)daa * sc), ig = (long long)rint((double)dga * sc);
@@ -1078,7 +1108,10 @@ struct hush_train_ctx {
   uint4 *Fprev[3] = {};                 // the previous frame's features F0..F2 (half res)
   float4 *GHprev = nullptr;             // and its half-res guide (sky blocks)
   void *oprev = nullptr, *pprev = nullptr, *gpprev = nullptr, *ga0 = nullptr;
-  long long *bacc = nullptr;            // fixed-point blend-gradient sums (2 per half-res texel) + the maximum
+  long long *bacc = nullptr;            // fixed-point blend-gradient sums (2 per half-res texel)
+  unsigned long long *bmax = nullptr;   // per-workgroup maxima of the blend-gradient contributions, then the maximum
+  void *bterm = nullptr;                // per pixel: the two blend-gradient contributions (element type)
+  int *bq = nullptr;                    // per pixel: their half-res texel, -1 for none
   bool has_prev = false;                // Fprev holds the committed previous frame
   int prev_mode = 0;                    // this frame: 0 no history, 1 alpha/gamma from the stored par0, 2 recomputed
   bool a0_fixed = false;
@@ -1114,8 +1147,8 @@ extern "C" int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out
   t->nlb = (int)((N + NTH - 1) / NTH);
   // half: o, par, gpar, coefH (12 each), mid x3, gmid x3 (4 each), the fixed-point blend sums; full: a0, fine,
   // X, prev, gX, gfine, gill, gpf (4 each), taac, coefF (12 each); FP64 partials
-  const size_t half = N2 * ts * (7 * 12 + 6 * 4) + N2 * 16 * 4 + (N2 * 2 + 1) * 8,
-               full = N * ts * (9 * 4 + 2 * 12) + N * 4 * TEMPORAL_AUX;
+  const size_t half = N2 * ts * (7 * 12 + 6 * 4) + N2 * 16 * 4 + N2 * 2 * 8,
+               full = N * ts * (9 * 4 + 2 * 12 + 2) + N * 4 * TEMPORAL_AUX + N * 4 + ((N + NTH - 1) / NTH + 1) * 8;
   const size_t parts = ((size_t)2 * t->nwb * NW + (size_t)WSL * NW + t->nlb + 64) * sizeof(double);
   const size_t total = half + full + parts + 64 * 256;
   e = hipMalloc(&t->arena, total);
@@ -1135,7 +1168,9 @@ extern "C" int32_t hush_train_create(hush_t *h, int32_t fp64, hush_train_t **out
   t->taac = take(N * ts * 12); t->coefF = take(N * ts * 12);
   t->oprev = take(N2 * ts * 12); t->pprev = take(N2 * ts * 12); t->gpprev = take(N2 * ts * 12); t->ga0 = take(N * ts * 4);
   t->aux = (float *)take(N * 4 * TEMPORAL_AUX);
-  t->bacc = (long long *)take((N2 * 2 + 1) * 8);
+  t->bacc = (long long *)take(N2 * 2 * 8);
+  t->bmax = (unsigned long long *)take(((N + NTH - 1) / NTH + 1) * 8);
+  t->bterm = take(N * ts * 2); t->bq = (int *)take(N * 4);
   for (int k = 0; k < 3; k++) t->Fprev[k] = (uint4 *)take(N2 * 16);
   t->GHprev = (float4 *)take(N2 * 16);
   t->wpart = (double *)take(((size_t)2 * t->nwb + WSL) * NW * sizeof(double));   // partials, then the slice sums
@@ -1426,12 +1461,14 @@ static hipError_t backward_t(hush_train_t *t, const T *w, const T *gout, T *gw, 
   int nb = t->nwb;
   if (chain) {
     // temporal blend -> the previous frame's alpha / gamma -> its MLP (features of that frame)
-    hipLaunchKernelGGL(tk_zero<long long>, grid1(N2 * 2 + 1), dim3(NTH), 0, s, t->bacc, (long)(N2 * 2 + 1));
-    unsigned long long *mxp = (unsigned long long *)(t->bacc + N2 * 2);
+    hipLaunchKernelGGL(tk_zero<long long>, grid1(N2 * 2), dim3(NTH), 0, s, t->bacc, (long)(N2 * 2));
+    const int nblk = (int)grid1(N).x;
+    unsigned long long *mxp = t->bmax + nblk;          // the maximum, after the nblk workgroup partials
     hipLaunchKernelGGL(tk_blend_bwd_max<T>, grid1(N), dim3(NTH), 0, s, (const float *)t->aux, (const T *)t->pprev, (const T *)ga,
-                       (int)N, mxp);
-    hipLaunchKernelGGL(tk_blend_bwd_add<T>, grid1(N), dim3(NTH), 0, s, (const float *)t->aux, (const T *)t->pprev, (const T *)ga,
-                       (int)N, (const unsigned long long *)mxp, t->bacc);
+                       (int)N, (T *)t->bterm, t->bq, t->bmax);
+    hipLaunchKernelGGL(tk_max_bits, dim3(1), dim3(NTH), 0, s, t->bmax, nblk);
+    hipLaunchKernelGGL(tk_blend_bwd_add<T>, grid1(N), dim3(NTH), 0, s, (const T *)t->bterm, (const int *)t->bq, (int)N,
+                       (const unsigned long long *)mxp, t->bacc);
     hipLaunchKernelGGL(tk_blend_bwd_store<T>, grid1(N2), dim3(NTH), 0, s, (const long long *)t->bacc,
                        (const unsigned long long *)mxp, (int)N, (int)N2, (T *)t->gpprev);
     launch_mlp_bwd<T>(t, fprev_tex(t, 0), fprev_tex(t, 1), fprev_tex(t, 2), TexF{t->GHprev, t->W2}, w, (const T *)t->oprev,
