@@ -374,105 +374,130 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   }
   __syncthreads();
 
-  // block phase (hush.js:226-282, 301-306): one wave computes the 16x4 half-res texels of this tile; which
-  // wave rotates with the workgroup index, so that the phase's extra work spreads over the CU's SIMDs
-  if ((tid >> 6) != (int)(blockIdx.x & 3)) return;
+  // block phase (hush.js:226-282, 301-306): the 16x4 half-res texels of this tile, split over two waves by
+  // output surface, each over all 64 texels: wave A (F0, F2: the luminance, variance and motion features)
+  // and wave B (F1, F3 and the decoded guide: normal, depth and material features).  Each computes the 2x2
+  // sums it needs in the same order, so every feature is the one-wave result bit for bit; which two waves
+  // rotates with the workgroup index, so that the phase's extra work spreads over the CU's SIMDs.
+  const int role = ((tid >> 6) - (int)blockIdx.x) & 3;
+  if (role != 0 && role != 2) return;
+  const bool wA = role == 0;
   const int bxl = tid & 15, byl = (tid & 63) >> 4;
   const int X = (x0 >> 1) + bxl, Y = (y0 >> 1) + byl;
   if (X >= a.W2 || Y >= a.H2) return;
-  float nv = 0.0f;
-  Sum<KAHAN> qS0[4], qS1[4], qS3[4], qAC[4], qNx, qNy, qNz, qMV, qZ;
-  bool ok[4];
-  float la[4], zs[4], nsx[4], nsy[4], nsz[4];
   // A sky or outside pixel's feature inputs are stored as zeros (its depth as -1), so it enters every sum as
   // +0, which leaves the running sums unchanged bit for bit (each starts at +0 and never becomes -0); only its
   // depth and staged luminance are masked.  Straight-line code, no divergent block per pixel.
+  float nv = 0.0f;
+  bool ok[4];
+  int t4[4], tl4[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const int t = pslot(byl * 64 + (j >> 1) * 32 + 2 * bxl + (j & 1));
-    const int tl = (2 * byl + (j >> 1) + 1) * SW + 2 * bxl + (j & 1) + 1;    // the pixel's staged tile index
-    const float z = sP[P2Z][t];
-    ok[j] = z >= 0.0f;
+    t4[j] = pslot(byl * 64 + (j >> 1) * 32 + 2 * bxl + (j & 1));
+    tl4[j] = (2 * byl + (j >> 1) + 1) * SW + 2 * bxl + (j & 1) + 1;        // the pixel's staged tile index
+    ok[j] = sP[P2Z][t4[j]] >= 0.0f;
     nv += ok[j] ? 1.0f : 0.0f;
-    la[j] = sP[P0X][t]; zs[j] = ok[j] ? z : 0.0f;
-    nsx[j] = sP[NSX][t]; nsy[j] = sP[NSY][t]; nsz[j] = sP[NSZ][t];
-    const float2 m01 = unpack2(__float_as_uint(sP[P3A][t])), m23 = unpack2(__float_as_uint(sP[P3B][t]));
-    const float p0[4] = {la[j], ok[j] ? sQ[tl] : 0.0f, sP[P0Z][t], sP[P0W][t]}, p3[4] = {m01.x, m01.y, m23.x, m23.y};
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      qS0[c].add(p0[c]); qS1[c].add(sP[P1X + c][t]); qS3[c].add(p3[c]); qAC[c].add(sP[P4X + c][t]);
-    }
-    qNx.add(nsx[j]); qNy.add(nsy[j]); qNz.add(nsz[j]);
-    qMV.add(sP[P2W][t]); qZ.add(zs[j]);
   }
-  float S0[4], S1[4], S3[4], AC[4];
-#pragma unroll
-  for (int c = 0; c < 4; c++) { S0[c] = qS0[c].get(); S1[c] = qS1[c].get(); S3[c] = qS3[c].get(); AC[c] = qAC[c].get(); }
-  float Nx = qNx.get(), Ny = qNy.get(), Nz = qNz.get(), MV = qMV.get(), Z = qZ.get();
   if (nv == 0.0f) {
-    stu<BUF>(a.F0, X, Y, make_uint4(0, 0, 0, 0));
-    stu<BUF>(a.F1, X, Y, make_uint4(0, 0, 0, 0));
-    stu<BUF>(a.F2, X, Y, make_uint4(0, 0, 0, 0));
-    const uint4 g3 = make_uint4(pk(0.0f, 0.0f), pk(-1.0f, 0.0f), 0u, 0u);
-    stu<BUF>(a.F3, X, Y, g3);
-    const float3 d0 = oct_dec(0.0f, 0.0f);
-    stf<BUF>(a.outGH, X, Y, make_float4(d0.x, d0.y, d0.z, __uint_as_float(g3.y)));
+    if (wA) {
+      stu<BUF>(a.F0, X, Y, make_uint4(0, 0, 0, 0));
+      stu<BUF>(a.F2, X, Y, make_uint4(0, 0, 0, 0));
+    } else {
+      stu<BUF>(a.F1, X, Y, make_uint4(0, 0, 0, 0));
+      const uint4 g3 = make_uint4(pk(0.0f, 0.0f), pk(-1.0f, 0.0f), 0u, 0u);
+      stu<BUF>(a.F3, X, Y, g3);
+      const float3 d0 = oct_dec(0.0f, 0.0f);
+      stf<BUF>(a.outGH, X, Y, make_float4(d0.x, d0.y, d0.z, __uint_as_float(g3.y)));
+    }
     return;
   }
   const float k = F::rcpn(nv);
+  if (wA) {
+    Sum<KAHAN> qS0[4], qS1[4], qAC[4], qMV;
+    float la[4];
 #pragma unroll
-  for (int c = 0; c < 4; c++) { S0[c] *= k; S1[c] *= k; S3[c] *= k; AC[c] *= k; }
-  Nx *= k; Ny *= k; Nz *= k; MV *= k; Z *= k;
-  const float Lacc = S0[0];
-  Sum<KAHAN> qVA;
+    for (int j = 0; j < 4; j++) {
+      const int t = t4[j];
+      la[j] = sP[P0X][t];
+      const float p0[4] = {la[j], ok[j] ? sQ[tl4[j]] : 0.0f, sP[P0Z][t], sP[P0W][t]};
 #pragma unroll
-  for (int j = 0; j < 4; j++) qVA.add(ok[j] ? (la[j] - Lacc) * (la[j] - Lacc) : 0.0f);
-  float VA = qVA.get() * k;
-  // 4x4 neighbourhood (block +-1) of the exposed, truncated luminance (hush.js:253-265)
-  Sum<KAHAN> q1;
-  float n4 = 0.0f;
-  const int b0 = (2 * byl) * SW + 2 * bxl;      // tile index of pixel (2*bxl - 1, 2*byl - 1) incl. apron
+      for (int c = 0; c < 4; c++) { qS0[c].add(p0[c]); qS1[c].add(sP[P1X + c][t]); qAC[c].add(sP[P4X + c][t]); }
+      qMV.add(sP[P2W][t]);
+    }
+    float S0[4], S1[4], AC[4];
 #pragma unroll
-  for (int j = 0; j < 16; j++) {
-    int idx = b0 + (j >> 2) * SW + (j & 3);
-    float w = sI[idx].w;
-    // w * E * qf16(lum) = sQ: w is exactly 1 or 0, and an invalid texel's sQ is E * qf16(0) = 0
-    q1.add(sQ[idx]);
-    n4 += w;
+    for (int c = 0; c < 4; c++) { S0[c] = qS0[c].get() * k; S1[c] = qS1[c].get() * k; AC[c] = qAC[c].get() * k; }
+    const float MV = qMV.get() * k;
+    const float Lacc = S0[0];
+    Sum<KAHAN> qVA;
+#pragma unroll
+    for (int j = 0; j < 4; j++) qVA.add(ok[j] ? (la[j] - Lacc) * (la[j] - Lacc) : 0.0f);
+    const float VA = qVA.get() * k;
+    // 4x4 neighbourhood (block +-1) of the exposed, truncated luminance (hush.js:253-265)
+    Sum<KAHAN> q1;
+    float n4 = 0.0f;
+    const int b0 = (2 * byl) * SW + 2 * bxl;      // tile index of pixel (2*bxl - 1, 2*byl - 1) incl. apron
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      int idx = b0 + (j >> 2) * SW + (j & 3);
+      float w = sI[idx].w;
+      // w * E * qf16(lum) = sQ: w is exactly 1 or 0, and an invalid texel's sQ is E * qf16(0) = 0
+      q1.add(sQ[idx]);
+      n4 += w;
+    }
+    const float mu4 = F::div(q1.get(), mx(n4, 1.0f));
+    Sum<KAHAN> q4;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      int idx = b0 + (j >> 2) * SW + (j & 3);
+      float d = sQ[idx] - mu4;
+      q4.add(sI[idx].w * d * d);
+    }
+    const float var4 = F::div(q4.get(), mx(n4, 1.0f));
+    stu<BUF>(a.F0, X, Y, make_uint4(pk(log1p_<FAST>(Lacc), log1p_<FAST>(S0[1])), pk(log1p_<FAST>(S0[2]), log1p_<FAST>(var4)),
+                                    pk(S0[3] / 32.0f, S1[0]),
+                                    pk(log1p_<FAST>(S1[1]), log1p_<FAST>(F::div(fabsf(S1[2] - S1[3]), S1[3] + 1e-3f)))));
+    stu<BUF>(a.F2, X, Y, make_uint4(pk(1.0f - nv / 4.0f, log1p_<FAST>(MV)), pk(log1p_<FAST>(mu4), log1p_<FAST>(VA)),
+                                    pk(AC[0], AC[1]), pk(AC[2], AC[3])));
+  } else {
+    Sum<KAHAN> qS3[4], qNx, qNy, qNz, qZ;
+    float zs[4], nsx[4], nsy[4], nsz[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int t = t4[j];
+      zs[j] = ok[j] ? sP[P2Z][t] : 0.0f;
+      nsx[j] = sP[NSX][t]; nsy[j] = sP[NSY][t]; nsz[j] = sP[NSZ][t];
+      const float2 m01 = unpack2(__float_as_uint(sP[P3A][t])), m23 = unpack2(__float_as_uint(sP[P3B][t]));
+      const float p3[4] = {m01.x, m01.y, m23.x, m23.y};
+#pragma unroll
+      for (int c = 0; c < 4; c++) qS3[c].add(p3[c]);
+      qNx.add(nsx[j]); qNy.add(nsy[j]); qNz.add(nsz[j]);
+      qZ.add(zs[j]);
+    }
+    float S3[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) S3[c] = qS3[c].get() * k;
+    const float Nx = qNx.get() * k, Ny = qNy.get() * k, Nz = qNz.get() * k, Z = qZ.get() * k;
+    const float ln = mx(F::sqrt(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
+    const float rln = F::rcpn(ln);
+    const float3 Nn = make_float3(F::divsf(Nx, ln, rln), F::divsf(Ny, ln, rln), F::divsf(Nz, ln, rln));
+    float zmax = -1e30f, zmin = 1e30f, mind = 1e30f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const float d = dot3(make_float3(nsx[j], nsy[j], nsz[j]), Nn);
+      zmax = ok[j] ? mx(zmax, zs[j]) : zmax; zmin = ok[j] ? mn(zmin, zs[j]) : zmin;
+      mind = ok[j] ? mn(mind, d) : mind;
+    }
+    const float dz = mn(mx(F::div(zmax - zmin, mx(zmin, 1e-6f)), 0.0f), 1.0f);
+    const float dn = mn(mx(1.0f - mind, 0.0f), 2.0f);
+    const float2 oc = oct_enc_m<FAST>(Nn);
+    stu<BUF>(a.F1, X, Y, make_uint4(pk(oc.x, oc.y), pk(dz, dn), pk(S3[0], S3[1]), pk(S3[2], S3[3])));
+    const uint4 g3 = make_uint4(pk(oc.x, oc.y), pk(Z, nv), 0u, 0u);
+    stu<BUF>(a.F3, X, Y, g3);
+    const float2 go = unpack2(g3.x);
+    const float3 gd = oct_dec(go.x, go.y);
+    stf<BUF>(a.outGH, X, Y, make_float4(gd.x, gd.y, gd.z, __uint_as_float(g3.y)));
   }
-  const float mu4 = F::div(q1.get(), mx(n4, 1.0f));
-  Sum<KAHAN> q4;
-#pragma unroll
-  for (int j = 0; j < 16; j++) {
-    int idx = b0 + (j >> 2) * SW + (j & 3);
-    float d = sQ[idx] - mu4;
-    q4.add(sI[idx].w * d * d);
-  }
-  float var4 = F::div(q4.get(), mx(n4, 1.0f));
-  const float ln = mx(F::sqrt(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
-  const float rln = F::rcpn(ln);
-  const float3 Nn = make_float3(F::divsf(Nx, ln, rln), F::divsf(Ny, ln, rln), F::divsf(Nz, ln, rln));
-  float zmax = -1e30f, zmin = 1e30f, mind = 1e30f;
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const float d = dot3(make_float3(nsx[j], nsy[j], nsz[j]), Nn);
-    zmax = ok[j] ? mx(zmax, zs[j]) : zmax; zmin = ok[j] ? mn(zmin, zs[j]) : zmin;
-    mind = ok[j] ? mn(mind, d) : mind;
-  }
-  const float dz = mn(mx(F::div(zmax - zmin, mx(zmin, 1e-6f)), 0.0f), 1.0f);
-  const float dn = mn(mx(1.0f - mind, 0.0f), 2.0f);
-  const float2 oc = oct_enc_m<FAST>(Nn);
-  float f[20] = {log1p_<FAST>(Lacc), log1p_<FAST>(S0[1]), log1p_<FAST>(S0[2]), log1p_<FAST>(var4), S0[3] / 32.0f, S1[0],
-                 log1p_<FAST>(S1[1]), log1p_<FAST>(F::div(fabsf(S1[2] - S1[3]), S1[3] + 1e-3f)), oc.x, oc.y, dz, dn, S3[0],
-                 S3[1], S3[2], S3[3], 1.0f - nv / 4.0f, log1p_<FAST>(MV), log1p_<FAST>(mu4), log1p_<FAST>(VA)};
-  stu<BUF>(a.F0, X, Y, make_uint4(pk(f[0], f[1]), pk(f[2], f[3]), pk(f[4], f[5]), pk(f[6], f[7])));
-  stu<BUF>(a.F1, X, Y, make_uint4(pk(f[8], f[9]), pk(f[10], f[11]), pk(f[12], f[13]), pk(f[14], f[15])));
-  stu<BUF>(a.F2, X, Y, make_uint4(pk(f[16], f[17]), pk(f[18], f[19]), pk(AC[0], AC[1]), pk(AC[2], AC[3])));
-  const uint4 g3 = make_uint4(pk(oc.x, oc.y), pk(Z, nv), 0u, 0u);
-  stu<BUF>(a.F3, X, Y, g3);
-  const float2 go = unpack2(g3.x);
-  const float3 gd = oct_dec(go.x, go.y);
-  stf<BUF>(a.outGH, X, Y, make_float4(gd.x, gd.y, gd.z, __uint_as_float(g3.y)));
 }
 
 // ============================================================================ edge-aware a-trous
