@@ -65,6 +65,39 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t hist_rsrc(const void *p, int p
                                            (int)((uint32_t)pitch * (uint32_t)rows * (uint32_t)texel), 0x00020000);
 }
 
+// 4x4 neighbourhood (block +-1) of the exposed, truncated luminance of a half-res texel's block
+// (hush.js:253-265) over the staged tile (row pitch TW + 2): its mean, and its variance about that mean.
+// sI[.].w is exactly 1 or 0 and an invalid texel's sQ is E * qf16(0) = 0, so w * E * qf16(lum) is sQ.
+template <bool KAHAN>
+__device__ __forceinline__ float mean4x4(const float4 *sI, const float *sQ, int byl, int bxl) {
+  constexpr int SW = TW + 2;
+  Sum<KAHAN> q1;
+  float n4 = 0.0f;
+  const int b0 = (2 * byl) * SW + 2 * bxl;      // tile index of pixel (2*bxl - 1, 2*byl - 1) incl. apron
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int idx = b0 + (j >> 2) * SW + (j & 3);
+    q1.add(sQ[idx]);
+    n4 += sI[idx].w;
+  }
+  return Math<false>::div(q1.get(), mx(n4, 1.0f));
+}
+template <bool KAHAN>
+__device__ __forceinline__ float var4x4(const float4 *sI, const float *sQ, int byl, int bxl, float mu4) {
+  constexpr int SW = TW + 2;
+  Sum<KAHAN> q4;
+  float n4 = 0.0f;
+  const int b0 = (2 * byl) * SW + 2 * bxl;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int idx = b0 + (j >> 2) * SW + (j & 3);
+    const float w = sI[idx].w, d = sQ[idx] - mu4;
+    q4.add(w * d * d);
+    n4 += w;
+  }
+  return Math<false>::div(q4.get(), mx(n4, 1.0f));
+}
+
 template <bool AUX, bool KAHAN, bool BUF>
 __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   constexpr bool FAST = false;
@@ -374,14 +407,13 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   }
   __syncthreads();
 
-  // block phase (hush.js:226-282, 301-306): the 16x4 half-res texels of this tile, split over two waves by
-  // output surface, each over all 64 texels: wave A (F0, F2: the luminance, variance and motion features)
-  // and wave B (F1, F3 and the decoded guide: normal, depth and material features).  Each computes the 2x2
-  // sums it needs in the same order, so every feature is the one-wave result bit for bit; which two waves
-  // rotates with the workgroup index, so that the phase's extra work spreads over the CU's SIMDs.
+  // block phase (hush.js:226-282, 301-306): the 16x4 half-res texels of this tile, split over three waves
+  // by output surface, each over all 64 texels: F0 (luminance and variance features), F2 (block count,
+  // motion, 4x4 mean, block variance, albedo) and F1 + F3 + the decoded guide (normal, depth and material
+  // features).  Each forms the 2x2 sums it needs in the same order, so every feature is the one-wave result
+  // bit for bit; which waves rotates with the workgroup index, so that the extra work spreads over the SIMDs.
   const int role = ((tid >> 6) - (int)blockIdx.x) & 3;
-  if (role != 0 && role != 2) return;
-  const bool wA = role == 0;
+  if (role == 3) return;
   const int bxl = tid & 15, byl = (tid & 63) >> 4;
   const int X = (x0 >> 1) + bxl, Y = (y0 >> 1) + byl;
   if (X >= a.W2 || Y >= a.H2) return;
@@ -399,9 +431,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     nv += ok[j] ? 1.0f : 0.0f;
   }
   if (nv == 0.0f) {
-    if (wA) {
-      stu<BUF>(a.F0, X, Y, make_uint4(0, 0, 0, 0));
-      stu<BUF>(a.F2, X, Y, make_uint4(0, 0, 0, 0));
+    if (role != 2) {
+      stu<BUF>(role == 0 ? a.F0 : a.F2, X, Y, make_uint4(0, 0, 0, 0));
     } else {
       stu<BUF>(a.F1, X, Y, make_uint4(0, 0, 0, 0));
       const uint4 g3 = make_uint4(pk(0.0f, 0.0f), pk(-1.0f, 0.0f), 0u, 0u);
@@ -412,51 +443,45 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     return;
   }
   const float k = F::rcpn(nv);
-  if (wA) {
-    Sum<KAHAN> qS0[4], qS1[4], qAC[4], qMV;
+  if (role == 0) {
+    Sum<KAHAN> qS0[4], qS1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int t = t4[j];
+      const float p0[4] = {sP[P0X][t], ok[j] ? sQ[tl4[j]] : 0.0f, sP[P0Z][t], sP[P0W][t]};
+#pragma unroll
+      for (int c = 0; c < 4; c++) { qS0[c].add(p0[c]); qS1[c].add(sP[P1X + c][t]); }
+    }
+    float S0[4], S1[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) { S0[c] = qS0[c].get() * k; S1[c] = qS1[c].get() * k; }
+    const float mu4 = mean4x4<KAHAN>(sI, sQ, byl, bxl);
+    const float var4 = var4x4<KAHAN>(sI, sQ, byl, bxl, mu4);
+    stu<BUF>(a.F0, X, Y, make_uint4(pk(log1p_<FAST>(S0[0]), log1p_<FAST>(S0[1])), pk(log1p_<FAST>(S0[2]), log1p_<FAST>(var4)),
+                                    pk(S0[3] / 32.0f, S1[0]),
+                                    pk(log1p_<FAST>(S1[1]), log1p_<FAST>(F::div(fabsf(S1[2] - S1[3]), S1[3] + 1e-3f)))));
+  } else if (role == 1) {
+    Sum<KAHAN> qL, qAC[4], qMV;
     float la[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int t = t4[j];
       la[j] = sP[P0X][t];
-      const float p0[4] = {la[j], ok[j] ? sQ[tl4[j]] : 0.0f, sP[P0Z][t], sP[P0W][t]};
+      qL.add(la[j]);
 #pragma unroll
-      for (int c = 0; c < 4; c++) { qS0[c].add(p0[c]); qS1[c].add(sP[P1X + c][t]); qAC[c].add(sP[P4X + c][t]); }
+      for (int c = 0; c < 4; c++) qAC[c].add(sP[P4X + c][t]);
       qMV.add(sP[P2W][t]);
     }
-    float S0[4], S1[4], AC[4];
+    float AC[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) { S0[c] = qS0[c].get() * k; S1[c] = qS1[c].get() * k; AC[c] = qAC[c].get() * k; }
+    for (int c = 0; c < 4; c++) AC[c] = qAC[c].get() * k;
     const float MV = qMV.get() * k;
-    const float Lacc = S0[0];
+    const float Lacc = qL.get() * k;
     Sum<KAHAN> qVA;
 #pragma unroll
     for (int j = 0; j < 4; j++) qVA.add(ok[j] ? (la[j] - Lacc) * (la[j] - Lacc) : 0.0f);
     const float VA = qVA.get() * k;
-    // 4x4 neighbourhood (block +-1) of the exposed, truncated luminance (hush.js:253-265)
-    Sum<KAHAN> q1;
-    float n4 = 0.0f;
-    const int b0 = (2 * byl) * SW + 2 * bxl;      // tile index of pixel (2*bxl - 1, 2*byl - 1) incl. apron
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-      int idx = b0 + (j >> 2) * SW + (j & 3);
-      float w = sI[idx].w;
-      // w * E * qf16(lum) = sQ: w is exactly 1 or 0, and an invalid texel's sQ is E * qf16(0) = 0
-      q1.add(sQ[idx]);
-      n4 += w;
-    }
-    const float mu4 = F::div(q1.get(), mx(n4, 1.0f));
-    Sum<KAHAN> q4;
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-      int idx = b0 + (j >> 2) * SW + (j & 3);
-      float d = sQ[idx] - mu4;
-      q4.add(sI[idx].w * d * d);
-    }
-    const float var4 = F::div(q4.get(), mx(n4, 1.0f));
-    stu<BUF>(a.F0, X, Y, make_uint4(pk(log1p_<FAST>(Lacc), log1p_<FAST>(S0[1])), pk(log1p_<FAST>(S0[2]), log1p_<FAST>(var4)),
-                                    pk(S0[3] / 32.0f, S1[0]),
-                                    pk(log1p_<FAST>(S1[1]), log1p_<FAST>(F::div(fabsf(S1[2] - S1[3]), S1[3] + 1e-3f)))));
+    const float mu4 = mean4x4<KAHAN>(sI, sQ, byl, bxl);
     stu<BUF>(a.F2, X, Y, make_uint4(pk(1.0f - nv / 4.0f, log1p_<FAST>(MV)), pk(log1p_<FAST>(mu4), log1p_<FAST>(VA)),
                                     pk(AC[0], AC[1]), pk(AC[2], AC[3])));
   } else {
