@@ -407,13 +407,13 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   }
   __syncthreads();
 
-  // block phase (hush.js:226-282, 301-306): the 16x4 half-res texels of this tile, split over three waves
-  // by output surface, each over all 64 texels: F0 (luminance and variance features), F2 (block count,
-  // motion, 4x4 mean, block variance, albedo) and F1 + F3 + the decoded guide (normal, depth and material
-  // features).  Each forms the 2x2 sums it needs in the same order, so every feature is the one-wave result
-  // bit for bit; which waves rotates with the workgroup index, so that the extra work spreads over the SIMDs.
+  // block phase (hush.js:226-282, 301-306): the 16x4 half-res texels of this tile, split over the four
+  // waves by output, each over all 64 texels: F0.xy (luminance features and the 4x4 variance), F0.zw (the
+  // remaining luminance, lag and fast-moment features), F2 (block count, motion, 4x4 mean, block variance,
+  // albedo) and F1 + F3 + the decoded guide (normal, depth and material features).  Each forms the 2x2
+  // sums it needs in the same order, so every feature is the one-wave result bit for bit; which wave takes
+  // which part rotates with the workgroup index, so that the heavier parts spread over the SIMDs.
   const int role = ((tid >> 6) - (int)blockIdx.x) & 3;
-  if (role == 3) return;
   const int bxl = tid & 15, byl = (tid & 63) >> 4;
   const int X = (x0 >> 1) + bxl, Y = (y0 >> 1) + byl;
   if (X >= a.W2 || Y >= a.H2) return;
@@ -431,8 +431,10 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     nv += ok[j] ? 1.0f : 0.0f;
   }
   if (nv == 0.0f) {
-    if (role != 2) {
-      stu<BUF>(role == 0 ? a.F0 : a.F2, X, Y, make_uint4(0, 0, 0, 0));
+    if (role == 0 || role == 3) {
+      texel<BUF>((uint2 *)a.F0.p, 2 * a.F0.pitch, 2 * X + (role == 3), Y)[0] = make_uint2(0u, 0u);
+    } else if (role == 1) {
+      stu<BUF>(a.F2, X, Y, make_uint4(0, 0, 0, 0));
     } else {
       stu<BUF>(a.F1, X, Y, make_uint4(0, 0, 0, 0));
       const uint4 g3 = make_uint4(pk(0.0f, 0.0f), pk(-1.0f, 0.0f), 0u, 0u);
@@ -443,23 +445,37 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
     return;
   }
   const float k = F::rcpn(nv);
+  // F0 as two 8-byte halves of its 16-byte texel
+  uint2 *const f0h = texel<BUF>((uint2 *)a.F0.p, 2 * a.F0.pitch, 2 * X + (role == 3), Y);
   if (role == 0) {
-    Sum<KAHAN> qS0[4], qS1[4];
+    Sum<KAHAN> qS0[3];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int t = t4[j];
-      const float p0[4] = {sP[P0X][t], ok[j] ? sQ[tl4[j]] : 0.0f, sP[P0Z][t], sP[P0W][t]};
+      const float p0[3] = {sP[P0X][t], ok[j] ? sQ[tl4[j]] : 0.0f, sP[P0Z][t]};
 #pragma unroll
-      for (int c = 0; c < 4; c++) { qS0[c].add(p0[c]); qS1[c].add(sP[P1X + c][t]); }
+      for (int c = 0; c < 3; c++) qS0[c].add(p0[c]);
     }
-    float S0[4], S1[4];
+    float S0[3];
 #pragma unroll
-    for (int c = 0; c < 4; c++) { S0[c] = qS0[c].get() * k; S1[c] = qS1[c].get() * k; }
+    for (int c = 0; c < 3; c++) S0[c] = qS0[c].get() * k;
     const float mu4 = mean4x4<KAHAN>(sI, sQ, byl, bxl);
     const float var4 = var4x4<KAHAN>(sI, sQ, byl, bxl, mu4);
-    stu<BUF>(a.F0, X, Y, make_uint4(pk(log1p_<FAST>(S0[0]), log1p_<FAST>(S0[1])), pk(log1p_<FAST>(S0[2]), log1p_<FAST>(var4)),
-                                    pk(S0[3] / 32.0f, S1[0]),
-                                    pk(log1p_<FAST>(S1[1]), log1p_<FAST>(F::div(fabsf(S1[2] - S1[3]), S1[3] + 1e-3f)))));
+    f0h[0] = make_uint2(pk(log1p_<FAST>(S0[0]), log1p_<FAST>(S0[1])), pk(log1p_<FAST>(S0[2]), log1p_<FAST>(var4)));
+  } else if (role == 3) {
+    Sum<KAHAN> qS03, qS1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int t = t4[j];
+      qS03.add(sP[P0W][t]);
+#pragma unroll
+      for (int c = 0; c < 4; c++) qS1[c].add(sP[P1X + c][t]);
+    }
+    float S1[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) S1[c] = qS1[c].get() * k;
+    const float S03 = qS03.get() * k;
+    f0h[0] = make_uint2(pk(S03 / 32.0f, S1[0]), pk(log1p_<FAST>(S1[1]), log1p_<FAST>(F::div(fabsf(S1[2] - S1[3]), S1[3] + 1e-3f))));
   } else if (role == 1) {
     Sum<KAHAN> qL, qAC[4], qMV;
     float la[4];
