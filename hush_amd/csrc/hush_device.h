@@ -243,9 +243,12 @@ __device__ __forceinline__ float pow128(float x) {                            //
 template <bool FAST> __device__ __forceinline__ float sigmoid(float x) { return Math<FAST>::rcp(1.0f + Math<FAST>::exp(-x)); }
 
 // octEnc (hush.js:34-39)
+// The shared reciprocal RN(1 / d) comes from rcpn (exact for normal d below 2^126, which d >= 1e-20 is
+// unless the normal is not finite: then rcpn's NaN reaches only the quotients' special-case path, where
+// v_div_fixup_f32 returns IEEE's result from d and the numerator alone).
 __device__ __forceinline__ float2 oct_enc(float3 n) {
   float d = mx(fabsf(n.x) + fabsf(n.y) + fabsf(n.z), 1e-20f);
-  const float r = 1.0f / d;
+  const float r = Math<false>::rcpn(d);
   float vx = Math<false>::divs(n.x, d, r), vy = Math<false>::divs(n.y, d, r), vz = Math<false>::divs(n.z, d, r);
   float sx = vx >= 0.0f ? 1.0f : -1.0f, sy = vy >= 0.0f ? 1.0f : -1.0f;
   if (vz < 0.0f) return make_float2((1.0f - fabsf(vy)) * sx, (1.0f - fabsf(vx)) * sy);
