@@ -64,7 +64,7 @@ static const int FP8_SCALE_EXP = 20;
 static float pow2_over_448(double m) {
   if (!(m > 0.0)) return 1.0f;
   int e;
-  std::frexp(m / 448.0, &e);
+  if (std::frexp(m / 448.0, &e) == 0.5) e -= 1;       // m / 448 = f 2^e, f in [0.5, 1): 2^(e-1) suffices at f = 1/2
   return std::ldexp(1.0f, std::min(std::max(e, -FP8_SCALE_EXP), FP8_SCALE_EXP));
 }
 static float fp8_scale(const float *w, int n) {

@@ -28,8 +28,8 @@ def pow2_at_least(m: float) -> float:
     """the smallest power of two s with m / s <= 448 (1 for m == 0)"""
     if not m > 0.0:
         return 1.0
-    e = int(np.frexp(m / E4M3_MAX)[1])
-    return float(2.0 ** e)
+    f, e = np.frexp(m / E4M3_MAX)            # m / 448 = f 2^e, f in [0.5, 1)
+    return float(2.0 ** (int(e) - 1 if f == 0.5 else int(e)))
 
 
 def layer_input_maxima(weights16: np.ndarray, rows: np.ndarray) -> list:
