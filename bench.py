@@ -52,27 +52,50 @@ def measured_traffic(kernel: str, grid: int, precision: str):
     return None, None
 
 
-def mlp_stage(hs, hip, W, H, reps=20):
-    """The MLP alone (hush_mlp_forward) over this frame's real half-res feature rows, per precision: mean
-    duration over `reps` launches with hipEvents, and achieved TFLOP/s at 1536 FLOP per row."""
+def mlp_stage(hs, hip, W, H, reps=20, replays=3):
+    """The MLP alone (hush_mlp_forward) over this frame's real half-res feature rows, per precision: `reps`
+    launches captured in one HIP graph and replayed `replays` times between hipEvents (a ~10 us kernel
+    launched one call at a time from Python would be timed with the host's issue gaps), the mean duration
+    per launch, and achieved TFLOP/s at 1536 FLOP per row.  Every replay's output is checked against an
+    eager launch's, bit for bit."""
     F = np.concatenate([hs.surface(f"F{k}").download().view(np.float16) for k in range(3)], -1)
     rows = np.ascontiguousarray(F.reshape(-1, 24)[:, :20])
     n = rows.shape[0]
     fin, fout = hip.DeviceBuffer(rows.nbytes), hip.DeviceBuffer(n * 48)
     hip.check(hip.lib().hipMemcpy(hip.ctypes.c_void_p(fin.ptr), rows.ctypes.data_as(hip.ctypes.c_void_p), rows.nbytes,
                                   hip.hipMemcpyHostToDevice), "features")
+
+    def download():
+        out = np.empty(n * 12, np.float32)
+        hip.check(hip.lib().hipMemcpy(out.ctypes.data_as(hip.ctypes.c_void_p), hip.ctypes.c_void_p(fout.ptr), out.nbytes,
+                                      hip.hipMemcpyDeviceToHost), "outputs")
+        return out
+
     res = {}
+    stream = hip.Stream()
     e0, e1 = hip.Event(), hip.Event()
     for p in ("fp32", "f16", "fp16-mfma", "bf16-mfma", "fp8-mfma"):
-        hs.mlp_forward(fin, fout, n, p)
-        hip.synchronize()
-        e0.record()
-        for _ in range(reps):
-            hs.mlp_forward(fin, fout, n, p)
-        e1.record()
-        e1.synchronize()
-        us = e0.elapsed_ms(e1) / reps * 1e3
+        hs.mlp_forward(fin, fout, n, p, stream=stream.handle)
+        stream.synchronize()
+        ref = download()
+        g = hip.Graph(stream, lambda s: [hs.mlp_forward(fin, fout, n, p, stream=s) for _ in range(reps)])
+        g.launch()                                          # untimed replay
+        stream.synchronize()
+        tot = 0.0
+        for _ in range(replays):
+            hip.check(hip.lib().hipMemset(hip.ctypes.c_void_p(fout.ptr), 0, n * 48), "clear outputs")
+            hip.synchronize()
+            e0.record(stream.handle)
+            g.launch()
+            e1.record(stream.handle)
+            e1.synchronize()
+            tot += e0.elapsed_ms(e1)
+            if not np.array_equal(download().view(np.uint32), ref.view(np.uint32)):
+                raise RuntimeError(f"MLP stage {p}: a graph replay differs from the eager launch")
+        g.destroy()
+        us = tot / (reps * replays) * 1e3
         res[p] = {"us": round(us, 2), "TFLOP/s": round(1536.0 * n / (us * 1e-6) / 1e12, 2)}
+    stream.destroy()
     res["rows"] = n
     res["fp16_mfma_speedup_vs_fp32"] = round(res["fp32"]["us"] / res["fp16-mfma"]["us"], 2)
     return res
@@ -413,6 +436,9 @@ def main():
     ap.add_argument("--math", default=os.environ.get("HUSH_BENCH_MATH", "ieee"), choices=["ieee", "fast"],
                     help="math policy: ieee (the oracle's operation-exact mirror) or fast")
     ap.add_argument("--cpu-frames", type=int, default=6)
+    ap.add_argument("--ramp-s", type=float, default=0.5,
+                    help="untimed frames before the warm-up steps: at least one pass over the sequence and this many "
+                         "seconds, so the GPU reaches its steady clock (reported as clock_ramp)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the MLP-stage, FP16-MFMA frame, SVGF, training and PSNR legs")
     ap.add_argument("--mlp-stage-only", action="store_true",
@@ -481,6 +507,15 @@ def main():
         for _ in range(n):
             hs.encode(dev[next_frame(hs)], stream=stream.handle)
 
+    # clock ramp: the GPU leaves its idle clock only under sustained load, and a few warm-up frames (~1 ms of
+    # work) end before it gets there; play untimed frames for one pass over the sequence and at least
+    # --ramp-s seconds, then the --warmup steps, then the timed steps
+    ramp_frames, t_r = 0, time.perf_counter()
+    while ramp_frames < F or time.perf_counter() - t_r < args.ramp_s:
+        run(16)
+        stream.synchronize()
+        ramp_frames += 16
+    ramp = {"frames": ramp_frames, "s": round(time.perf_counter() - t_r, 3)}
     run(args.warmup)
     stream.synchronize()
     if args.mlp_stage_only:
@@ -598,7 +633,7 @@ def main():
                    "width": W, "height": H, "frames": F, "mlp": args.precision, "math": args.math},
         "frame_GB/s_algorithmic": round(total_bytes / (ms_frame * 1e-3) / 1e9, 1),
         "frame_hbm_frac": round(total_bytes / (ms_frame * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "passes": passes, "roofline": roofline, "cpu_baseline": cpu, **extras,
+        "clock_ramp": ramp, "passes": passes, "roofline": roofline, "cpu_baseline": cpu, **extras,
     }
     print(json.dumps(line), flush=True)
     hs.destroy()

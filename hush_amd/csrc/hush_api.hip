@@ -118,9 +118,13 @@ static void free_surfaces(hush_t *h) {
 // allocates the 22 surfaces of hush.js:780-793 (plus the debug mask) in one HBM arena
 static int alloc_surfaces(hush_t *h, int W, int H) {
   const int W2 = (W + 1) / 2, H2 = (H + 1) / 2, W4 = (W2 + 1) / 2, H4 = (H2 + 1) / 2;
-  struct Spec { int id, w, h, bpt; };
+  // pad: texels the row must hold beyond the width.  The colour and moment histories keep at least one: the
+  // temporal pass reads a row's two horizontal reprojection taps as one 16-byte load, which at the right
+  // edge covers texel W (its value is never used) and must stay inside the surface.
+  struct Spec { int id, w, h, bpt, pad = 0; };
   std::vector<Spec> specs;
-  for (int id : {S_A0, S_X, S_MOM0, S_MOM1, S_KEY0, S_KEY1, S_HIST0, S_HIST1, S_TAA0, S_TAA1}) specs.push_back({id, W, H, 8});
+  for (int id : {S_A0, S_X, S_KEY0, S_KEY1, S_TAA0, S_TAA1}) specs.push_back({id, W, H, 8});
+  for (int id : {S_MOM0, S_MOM1, S_HIST0, S_HIST1}) specs.push_back({id, W, H, 8, 1});
   for (int id : {S_BH0, S_BH1, S_PAR0_0, S_PAR0_1, S_PAR1, S_PAR2}) specs.push_back({id, W2, H2, 8});
   for (int id : {S_F0, S_F1, S_F2, S_F3}) specs.push_back({id, W2, H2, 16});
   for (int id : {S_C0, S_C1, S_CG}) specs.push_back({id, W4, H4, 8});
@@ -131,7 +135,7 @@ static int alloc_surfaces(hush_t *h, int W, int H) {
   size_t total = 0;
   std::vector<size_t> offs;
   for (auto &s : specs) {
-    size_t pitch = (size_t)round_up(s.w, 64) * s.bpt;
+    size_t pitch = (size_t)round_up(s.w + s.pad, 64) * s.bpt;
     offs.push_back(total);
     total += (pitch * s.h + 4095) / 4096 * 4096;
   }
@@ -144,7 +148,7 @@ static int alloc_surfaces(hush_t *h, int W, int H) {
   for (size_t i = 0; i < specs.size(); i++) {
     auto &s = specs[i];
     char *p = (char *)arena + offs[i];
-    int pitch = round_up(s.w, 64);
+    int pitch = round_up(s.w + s.pad, 64);
     if (s.id >= S_F0 && s.id <= S_F3) h->F[s.id - S_F0] = TexU{(uint4 *)p, pitch};
     else if (s.id == S_MASK) { h->mask = (uint8_t *)p; h->mask_pitch = pitch; }
     else if (s.id == S_KD0 || s.id == S_KD1) h->kd[s.id - S_KD0] = TexF{(float4 *)p, pitch};

@@ -167,16 +167,30 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       const __amdgpu_buffer_rsrc_t rc = hist_rsrc(a.pcol.p, a.pcol.pitch, a.H, 8);
       const __amdgpu_buffer_rsrc_t rm = hist_rsrc(a.pmom.p, a.pmom.pitch, a.H, 8);
       const __amdgpu_buffer_rsrc_t rp = hist_rsrc(a.ppar0.p, a.ppar0.pitch, a.H2, 8);
+      // colour and moments: a row's two horizontal taps in one 16-byte load at qx0 = clamp(bix, 0, W - 1) (the
+      // launcher has checked that both surfaces hold at least one texel of padding per row, so texel qx0 + 1
+      // lies inside the row).  Tap 0 is the low half.  Tap 1 is the high half when 0 <= bix <= W - 2, else the
+      // low half: bix < 0 puts tap 0 outside, and texel 0 is tap 1's clamped texel; bix >= W - 1 puts tap 1
+      // outside, where the clamped tap is texel W - 1.  So every tap holds exactly the clamped texel that a
+      // separate load would read, and the padding's contents are never used.
+      const uint32_t qx0 = clampi(bix, 0, a.W - 1);
+      const bool lo1 = (bix < 0) | (bix > a.W - 2);
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
+      for (int r = 0; r < 2; r++) {
         // qy < 16384 and pitch < 2^23 fit the 24-bit multiply; the product is below 2^28 texels
-        const uint32_t qx = clampi(addw(bix, i & 1), 0, a.W - 1), qy = clampi(addw(biy, i >> 1), 0, a.H - 1);
-        const auto kv = __builtin_amdgcn_raw_buffer_load_b128(rk, (int)((__umul24(qy, a.pkd.pitch) + qx) * 16u), 0, 0);
-        k4[i] = make_float4(__uint_as_float(kv[0]), __uint_as_float(kv[1]), __uint_as_float(kv[2]), __uint_as_float(kv[3]));
-        const auto cv = __builtin_amdgcn_raw_buffer_load_b64(rc, (int)((__umul24(qy, a.pcol.pitch) + qx) * 8u), 0, 0);
-        c4r[i] = make_uint2(cv[0], cv[1]);
-        const auto mv2 = __builtin_amdgcn_raw_buffer_load_b64(rm, (int)((__umul24(qy, a.pmom.pitch) + qx) * 8u), 0, 0);
-        m4r[i] = make_uint2(mv2[0], mv2[1]);
+        const uint32_t qy = clampi(addw(biy, r), 0, a.H - 1);
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const uint32_t qx = clampi(addw(bix, c), 0, a.W - 1);
+          const auto kv = __builtin_amdgcn_raw_buffer_load_b128(rk, (int)((__umul24(qy, a.pkd.pitch) + qx) * 16u), 0, 0);
+          k4[2 * r + c] = make_float4(__uint_as_float(kv[0]), __uint_as_float(kv[1]), __uint_as_float(kv[2]), __uint_as_float(kv[3]));
+        }
+        const auto cv = __builtin_amdgcn_raw_buffer_load_b128(rc, (int)((__umul24(qy, a.pcol.pitch) + qx0) * 8u), 0, 0);
+        const auto mv4 = __builtin_amdgcn_raw_buffer_load_b128(rm, (int)((__umul24(qy, a.pmom.pitch) + qx0) * 8u), 0, 0);
+        c4r[2 * r] = make_uint2(cv[0], cv[1]);
+        c4r[2 * r + 1] = lo1 ? make_uint2(cv[0], cv[1]) : make_uint2(cv[2], cv[3]);
+        m4r[2 * r] = make_uint2(mv4[0], mv4[1]);
+        m4r[2 * r + 1] = lo1 ? make_uint2(mv4[0], mv4[1]) : make_uint2(mv4[2], mv4[3]);
       }
       const uint32_t hx = clampi(cx, 0, a.W - 1) >> 1, hy = clampi(cy, 0, a.H - 1) >> 1;
       const auto pv = __builtin_amdgcn_raw_buffer_load_b64(rp, (int)((__umul24(hy, a.ppar0.pitch) + hx) * 8u), 0, 0);
@@ -1590,7 +1604,9 @@ hipError_t launch_temporal(const TemporalArgs &a, bool, hipStream_t s) {
   // surfaces take the 64-bit address form)
   const uint64_t lim = (uint64_t)1 << 31;
   auto small = [&](int pitch, int rows, int texel) { return (uint64_t)pitch * (uint64_t)rows * (uint64_t)texel < lim; };
-  const bool buf = !a.wide && small(a.pkd.pitch, a.H, 16) && small(a.pcol.pitch, a.H, 8) && small(a.pmom.pitch, a.H, 8) &&
+  // the paired colour / moment taps also need a texel of padding per row of those two surfaces
+  const bool buf = !a.wide && a.pcol.pitch > a.W && a.pmom.pitch > a.W &&
+                   small(a.pkd.pitch, a.H, 16) && small(a.pcol.pitch, a.H, 8) && small(a.pmom.pitch, a.H, 8) &&
                    small(a.ppar0.pitch, a.H2, 8) && small(a.rad.pitch, a.H, 8) && small(a.nd.pitch, a.H, 8) &&
                    small(a.mv.pitch, a.H, 8) && small(a.mat.pitch, a.H, 8) && small(a.outA.pitch, a.H, 8) &&
                    small(a.outMom.pitch, a.H, 8) && small(a.outKey.pitch, a.H, 8) && small(a.outKD.pitch, a.H, 16) &&

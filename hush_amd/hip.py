@@ -52,6 +52,13 @@ def lib() -> ctypes.CDLL:
         _hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         _hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        _hip.hipStreamBeginCapture.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        _hip.hipStreamEndCapture.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        _hip.hipGraphInstantiate.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_char_p, ctypes.c_size_t]
+        _hip.hipGraphLaunch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _hip.hipGraphExecDestroy.argtypes = [ctypes.c_void_p]
+        _hip.hipGraphDestroy.argtypes = [ctypes.c_void_p]
     return _hip
 
 
@@ -165,6 +172,35 @@ class Event:
         check(lib().hipEventElapsedTime(ctypes.byref(ms), ctypes.c_void_p(self.handle), ctypes.c_void_p(end.handle)),
               "hipEventElapsedTime")
         return ms.value
+
+
+class Graph:
+    """The launches that `record(stream_handle)` enqueues on `stream`, captured once (hipStreamBeginCapture,
+    relaxed mode) and replayed by `launch()` as one hipGraphLaunch: the device runs them back to back, with
+    no gap for the host to issue each one.  Only kernel launches belong in it (no memsets or copies)."""
+
+    def __init__(self, stream: Stream, record):
+        L = lib()
+        s = ctypes.c_void_p(stream.handle)
+        self.stream = stream
+        self.graph, self.exec = ctypes.c_void_p(), ctypes.c_void_p()
+        check(L.hipStreamBeginCapture(s, 2), "hipStreamBeginCapture")
+        try:
+            record(stream.handle)
+        finally:
+            check(L.hipStreamEndCapture(s, ctypes.byref(self.graph)), "hipStreamEndCapture")
+        check(L.hipGraphInstantiate(ctypes.byref(self.exec), self.graph, None, None, 0), "hipGraphInstantiate")
+
+    def launch(self) -> None:
+        check(lib().hipGraphLaunch(self.exec, ctypes.c_void_p(self.stream.handle)), "hipGraphLaunch")
+
+    def destroy(self) -> None:
+        if self.exec:
+            lib().hipGraphExecDestroy(self.exec)
+            self.exec = ctypes.c_void_p()
+        if self.graph:
+            lib().hipGraphDestroy(self.graph)
+            self.graph = ctypes.c_void_p()
 
 
 def gpu_available() -> bool:
