@@ -114,6 +114,9 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   const int lx = tid & 31, ly = tid >> 5;
   const int px = x0 + lx, py = y0 + ly;
   const bool in_img = inside(px, py, a.W, a.H);
+  // the wave runs at a raised priority while it issues its loads (G-buffer, apron, reprojection gathers), so
+  // that other waves' arithmetic on the SIMD does not hold them back; back to the base priority after them
+  __builtin_amdgcn_s_setprio(1);
 
   // this pixel's G-buffer (also its own staged texel)
   // all four loads leave before any of them is waited for (the scheduler had put the first one's wait, and
@@ -210,6 +213,7 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
 
   // the staging's arithmetic runs while the gathers are in flight (its inputs arrived with the first loads)
   __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_setprio(0);
   // tile staging (hush.js:287-294): each thread its own pixel, threads 0..83 also an apron texel.
   // radiance / max(albedo, 1e-3): the quotient through the exact reciprocal (binary16 numerators, divisors in
   // [1e-3, 65504]: the domain where divs equals IEEE division, special operands included)
@@ -1295,6 +1299,8 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   // Every load of the tile is issued before the first wait: the staged A0 / key texels (two per thread for
   // threads 0..83), one window texel (threads 0..211), and the pixel's own material and emission, which the
   // pixel phase after the barrier reads.
+  // The wave issues them at a raised priority (as in the temporal pass).
+  __builtin_amdgcn_s_setprio(1);
   uint2 a0r[2];
   float4 kdr[2];
 #pragma unroll
@@ -1331,6 +1337,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   const int cpx = clampi(px, 0, a.W - 1), cpy = clampi(py, 0, a.H - 1);
   const uint2 albr = a.mat.p[(size_t)cpy * a.mat.pitch + cpx], emir = a.emi.p[(size_t)cpy * a.emi.pitch + cpx];
   __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_setprio(0);
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     const int i = tid + k * NT;
