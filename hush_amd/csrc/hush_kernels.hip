@@ -969,28 +969,42 @@ template <int MODE>
 constexpr int infer_mfma_lds() { return infer_lds() + 4 * 64 * 3; }
 template <int MODE, bool FAST>
 __device__ __forceinline__ void infer_mfma_tile(const InferArgs &a, int n, float4 *lds) {
-  constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
+  constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH, NI = (SN + NT - 1) / NT;
   float4 *sA = lds, *sG = lds + SN;
   const int tid = threadIdx.x;
   const int2 org = tile_origin<TW, TH>(a.W2, a.H2, n);
   const int x0 = org.x, y0 = org.y;
-  for (int i = tid; i < SN; i += NT) {
-    int qx = clampi(x0 - AP + i % SW, 0, a.W2 - 1), qy = clampi(y0 - AP + i / SW, 0, a.H2 - 1);
-    uint4 g2 = ldu(a.F2, qx, qy);
-    float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
-    sA[i] = make_float4(a01.x, a01.y, a23.x, a23.y);
-    sG[i] = guide_gh(ldf(a.GH, qx, qy));
-  }
   const int lx = tid & 31, ly = tid >> 5;
   const int X = x0 + lx, Y = y0 + ly;
   const int wv = tid >> 6, lane = tid & 63, kb = lane >> 4, tl = lane & 15;
   const MfmaPack *__restrict__ P = (const MfmaPack *)a.wq;
   float4 (*sO)[64 * 3] = (float4 (*)[64 * 3])(lds + 2 * SN);
+  // every load before the first wait: this lane's B fragments, then this thread's staged texels (clamped
+  // addresses past the staged block: loaded and discarded)
   uint4 xf[4];
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     const int gx = x0 + 16 * (g & 1) + tl, gy = y0 + 2 * wv + (g >> 1);
     xf[g] = feature_frag(a.F0, a.F1, a.F2, gx, gy, kb, inside(gx, gy, a.W2, a.H2));
+  }
+  uint4 g2r[NI];
+  float4 ghr[NI];
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    const int i = mn(tid + k * NT, SN - 1);
+    const int qx = clampi(x0 - AP + i % SW, 0, a.W2 - 1), qy = clampi(y0 - AP + i / SW, 0, a.H2 - 1);
+    g2r[k] = ldu(a.F2, qx, qy);
+    ghr[k] = ldf(a.GH, qx, qy);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    const int i = tid + k * NT;
+    if (i < SN) {
+      const float2 a01 = unpack2(g2r[k].z), a23 = unpack2(g2r[k].w);
+      sA[i] = make_float4(a01.x, a01.y, a23.x, a23.y);
+      sG[i] = guide_gh(ghr[k]);
+    }
   }
   f32x4 d[4];
   mlp_mfma_x4<MODE>(P, xf, d, lane);
@@ -1113,27 +1127,48 @@ constexpr int atrous_lds() {
 template <int R, int STEP, bool PRE, bool GUIDE_PACKED, int COMP, bool FAST>
 __device__ __forceinline__ void atrous_tile(const AtrousArgs &a, int n, float4 *lds) {
   constexpr int AP = R * STEP > 1 ? R * STEP : 1;
-  constexpr int SW = TW + 2 * AP, SH = TH + 2 * AP, SN = SW * SH;
+  constexpr int SW = TW + 2 * AP, SH = TH + 2 * AP, SN = SW * SH, NI = (SN + NT - 1) / NT;
   float4 *sA = lds, *sG = lds + SN;
   const int tid = threadIdx.x;
   const int2 org = tile_origin<TW, TH>(a.w, a.h, n);
   const int x0 = org.x, y0 = org.y;
-  for (int i = tid; i < SN; i += NT) {
-    int qx = clampi(x0 - AP + i % SW, 0, a.w - 1), qy = clampi(y0 - AP + i / SW, 0, a.h - 1);
-    sA[i] = ld(a.inA, qx, qy);
-    const float4 g = ldf(a.guide, qx, qy);
-    sG[i] = GUIDE_PACKED ? guide_gh(g) : g;
-  }
-  __syncthreads();
   const int lx = tid & 31, ly = tid >> 5;
   const int X = x0 + lx, Y = y0 + ly;
+  // Every load before the first wait: the pixel's own parameters, then the NI staged texels of this thread.
+  // Addresses are clamped (a lane outside the level, or past the staged block, loads a texel it discards),
+  // so no branch surrounds a load.
+  uint2 p1r = make_uint2(0u, 0u), p2r = p1r;
+  if (COMP >= 0) {
+    const int cX = clampi(X, 0, a.w - 1), cY = clampi(Y, 0, a.h - 1);
+    p1r = a.par1.p[(size_t)cY * a.par1.pitch + cX];
+    p2r = a.par2.p[(size_t)cY * a.par2.pitch + cX];
+  }
+  uint2 ar[NI];
+  float4 gr[NI];
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    const int i = mn(tid + k * NT, SN - 1);
+    const int qx = clampi(x0 - AP + i % SW, 0, a.w - 1), qy = clampi(y0 - AP + i / SW, 0, a.h - 1);
+    ar[k] = a.inA.p[(size_t)qy * a.inA.pitch + qx];
+    gr[k] = a.guide.p[(size_t)qy * a.guide.pitch + qx];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    const int i = tid + k * NT;
+    if (i < SN) {
+      sA[i] = unpack4(ar[k]);
+      sG[i] = GUIDE_PACKED ? guide_gh(gr[k]) : gr[k];
+    }
+  }
+  __syncthreads();
   if (!inside(X, Y, a.w, a.h)) return;
   const int tx = lx + AP, ty = ly + AP;
   if (sG[ty * SW + tx].w < 0.0f) { st(a.out, X, Y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
   float s, c;
   if (COMP < 0) { s = a.s; c = 1.0f; }
   else {
-    float4 p1 = ld(a.par1, X, Y), p2 = ld(a.par2, X, Y);
+    const float4 p1 = unpack4(p1r), p2 = unpack4(p2r);
     s = COMP == 0 ? p1.x : COMP == 1 ? p1.y : COMP == 2 ? p1.z : p1.w;
     c = COMP == 0 ? p2.x : COMP == 1 ? p2.y : COMP == 2 ? p2.z : p2.w;
   }
