@@ -1422,18 +1422,20 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   const float3 n = make_float3(gp.x, gp.y, gp.z);
   const float z = gp.w;
   const float jd = 0.05f * z + 1e-3f, rjd = F::rcpn(jd);
-  // half-res mid level (hush.js:585-602)
+  // half-res mid level (hush.js:585-602).  Both upsampling loops are branch-free: a tap outside the level
+  // (its window slot holds a clamped, finite copy) or on a sky texel (staged level values zero) gets weight
+  // 0 and adds exact zeros, which leave the sums unchanged bit for bit (each starts at +0)
   const float hpx = ((float)px + 0.5f) / 2.0f - 0.5f, hpy = ((float)py + 0.5f) / 2.0f - 0.5f;
   const float hbx = floorf(hpx), hby = floorf(hpy);
   float uBx = 0.0f, uBy = 0.0f, uBz = 0.0f, uAx = 0.0f, uAy = 0.0f, uAz = 0.0f, swh = 0.0f;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     int qx = (int)hbx + (j & 1), qy = (int)hby + (j >> 1);
-    if (CHK && (qx < 0 || qy < 0 || qx >= a.W2 || qy >= a.H2)) continue;
     const int wi = (qy - hy0) * HWW + (qx - hx0);
     const float4 gh = sHG[wi];
-    if (gh.w < 0.0f) continue;
-    float w = jbw<FAST>(hpx - hbx, hpy - hby, j, make_float3(gh.x, gh.y, gh.z), gh.w, n, z, jd, rjd);
+    const bool ok = (!CHK || ((qx >= 0) & (qy >= 0) & (qx < a.W2) & (qy < a.H2))) & (gh.w >= 0.0f);
+    const float wj = jbw<FAST>(hpx - hbx, hpy - hby, j, make_float3(gh.x, gh.y, gh.z), gh.w, n, z, jd, rjd);
+    const float w = ok ? wj : 0.0f;
     const float4 m = sHM[wi];
     const float2 aa = sHA[wi];
     swh += w;
@@ -1454,11 +1456,11 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     int qx = (int)cbx + (j & 1), qy = (int)cby + (j >> 1);
-    if (CHK && (qx < 0 || qy < 0 || qx >= a.W4 || qy >= a.H4)) continue;
     const int wi = (qy - qy0) * QWW + (qx - qx0);
     const float4 g = sQG[wi];
-    if (g.w < 0.0f) continue;
-    float w = jbw<FAST>(cpx - cbx, cpy - cby, j, make_float3(g.x, g.y, g.z), g.w, n, z, jd, rjd);
+    const bool ok = (!CHK || ((qx >= 0) & (qy >= 0) & (qx < a.W4) & (qy < a.H4))) & (g.w >= 0.0f);
+    const float wj = jbw<FAST>(cpx - cbx, cpy - cby, j, make_float3(g.x, g.y, g.z), g.w, n, z, jd, rjd);
+    const float w = ok ? wj : 0.0f;
     swq += w;
     const float4 cc = sQC[wi];
     uCx += w * cc.x; uCy += w * cc.y; uCz += w * cc.z;
