@@ -358,6 +358,7 @@ static TemporalArgs temporal_args(const hush_t *h, const hush_inputs *in) {
   a.mask = h->mask_on ? h->mask : nullptr; a.mask_pitch = h->mask_pitch; a.aux = h->train_aux; a.compensated = h->compensated ? 1 : 0;
   a.wide = h->wide_gathers ? 1 : 0;
   a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.reset = h->reset ? 1 : 0; a.E = h->exposure;
+  a.nz = -0.0f;
   return a;
 }
 static InferArgs infer_args(const hush_t *h) {
@@ -407,6 +408,7 @@ static TaaArgs taa_args(const hush_t *h, const hush_inputs *in) {
   a.X = t[S_X]; a.mv = in_tex(in->motion); a.prev = t[S_TAA0 + prev]; a.out = t[S_TAA0 + cur];
   a.W = h->W; a.H = h->H; a.reset = h->reset ? 1 : 0;
   a.rW = 1.0f / (float)h->W; a.rH = 1.0f / (float)h->H;
+  a.nz = -0.0f;
   return a;
 }
 

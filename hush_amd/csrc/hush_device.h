@@ -233,6 +233,13 @@ template <> struct Math<true> {
   static __device__ __forceinline__ float logn(float x) { return log(x); }
 };
 
+// w * x for a binary16 value x widened to fp32, evaluated as RN(w x + (-0)): the exact product rounded once,
+// the same bits as w * x (adding -0 changes neither a non-zero product nor the sign of a zero one, and
+// infinities and NaN pass through).  nz must be -0.0f supplied at run time (a kernel argument): with a
+// literal -0 the compiler folds the FMA back into a multiply, while an opaque addend keeps it, and then the
+// widening and the multiply are one v_fma_mix_f32 that reads the binary16 operand as stored.
+__device__ __forceinline__ float mulh(float w, float x, float nz) { return __builtin_fmaf(w, x, nz); }
+
 template <bool FAST> __device__ __forceinline__ float log1p_(float x) { return Math<FAST>::logn(1.0f + x); }  // hush.js:47
 
 __device__ __forceinline__ float pow128(float x) {                            // hush.js:70-73

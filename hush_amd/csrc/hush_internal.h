@@ -41,6 +41,7 @@ struct TemporalArgs {
   int wide;                          // history gathers on 64-bit addresses even below 4 GiB (debug)
   int W, H, W2, H2, reset;
   float E;                           // exposure
+  float nz;                          // -0.0f, the addend of mulh (exact products with binary16 history texels)
 };
 
 struct InferArgs {
@@ -85,6 +86,7 @@ struct TaaArgs {
   Tex X, mv, prev, out;
   int W, H, reset;
   float rW, rH;                      // RN(1 / W), RN(1 / H)
+  float nz;                          // -0.0f, the addend of mulh (exact products with binary16 history texels)
 };
 
 hipError_t launch_temporal(const TemporalArgs &a, bool fast, hipStream_t s);

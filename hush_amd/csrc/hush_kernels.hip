@@ -265,8 +265,9 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         mbits |= (uint32_t)ok << i;
         const float w = ok ? wts[i] : 0.0f;
         sumW += w;
-        hcx += w * c4[i].x; hcy += w * c4[i].y; hcz += w * c4[i].z;
-        hm.x += w * m4[i].x; hm.y += w * m4[i].y; hm.z += w * m4[i].z; hm.w += w * m4[i].w;
+        hcx += mulh(w, c4[i].x, a.nz); hcy += mulh(w, c4[i].y, a.nz); hcz += mulh(w, c4[i].z, a.nz);
+        hm.x += mulh(w, m4[i].x, a.nz); hm.y += mulh(w, m4[i].y, a.nz); hm.z += mulh(w, m4[i].z, a.nz);
+        hm.w += mulh(w, m4[i].w, a.nz);
       }
       if (sumW >= 0.01f) {
         const float rs = F::rcpn(sumW);
@@ -1594,7 +1595,7 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
       int ox = j & 1, oy = j >> 1;
       float wb = (ox == 1 ? fx : 1.0f - fx) * (oy == 1 ? fy : 1.0f - fy);
       const float4 t = unpack4(hr4[j]);
-      hx += wb * t.x; hy += wb * t.y; hz += wb * t.z;
+      hx += mulh(wb, t.x, a.nz); hy += mulh(wb, t.y, a.nz); hz += mulh(wb, t.z, a.nz);
     }
     float3 hh = to_ycocg(tm(make_float3(hx, hy, hz)));
     float kx = 1.25f * sdx, ky = 1.25f * sdy, kz = 1.25f * sdz;
