@@ -533,7 +533,8 @@ def main():
     print(f"[bench] timed {args.steps} frames: {ms_frame:.4f} ms/frame (wall {wall * 1e3 / args.steps:.4f})",
           file=sys.stderr, flush=True)
 
-    # per-pass durations with hipEvents on the launch stream (separate loop, one frame at a time)
+    # per-pass durations with hipEvents on the launch stream: the context's per-pass timer (each pass's kernel
+    # records a start and a stop event of its own execution on the launch stream), averaged over 64 frames
     timer = hush_amd.HushTimer()
     acc = np.zeros(8)
     n_t = min(args.steps, 64)

@@ -222,7 +222,7 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
       return HUSH_ERR_DEVICE;
     }
   }
-  for (int i = 0; i < 9 && e == hipSuccess; i++) e = hipEventCreate(&h->ev[i]);
+  for (int i = 0; i < 16 && e == hipSuccess; i++) e = hipEventCreate(&h->ev[i]);
   if (e != hipSuccess) {
     hip_fail(nullptr, e, "timing events");
     hush_destroy(h);
@@ -246,7 +246,7 @@ extern "C" void hush_destroy(hush_t *h) {
   if (h->d_w) (void)hipFree(h->d_w);
   if (h->d_wq) (void)hipFree(h->d_wq);
   if (h->d_wscale) (void)hipFree(h->d_wscale);
-  for (int i = 0; i < 9; i++)
+  for (int i = 0; i < 16; i++)
     if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
   delete h;
 }
@@ -453,13 +453,14 @@ extern "C" int32_t hush_encode(hush_t *h, const hush_inputs *in, void *stream, h
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(h->device);
   if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
-  if (h->timing) {                       // timed frames run the passes one after another
+  if (h->timing) {
+    // timed frames run the eight passes as eight launches, each kernel recording its own start and stop
     for (int p = 0; p < 8; p++) {
-      (void)hipEventRecord(h->ev[p], s);
+      set_pass_events(h->ev[2 * p], h->ev[2 * p + 1]);
       e = hush_run_pass(h, p, in, s);
+      set_pass_events(nullptr, nullptr);
       if (e != hipSuccess) return hip_fail(h, e, kPassLabel[p]);
     }
-    (void)hipEventRecord(h->ev[8], s);
     h->ev_ready = true;
   } else {
     // six launches on the caller's stream: temporal; coarse1 beside infer + mid1; mid2; coarse2 beside mid3;
@@ -521,10 +522,10 @@ extern "C" int32_t hush_pass_times(hush_t *h, float *ms, const char **labels) {
     for (int i = 0; i < 8; i++) labels[i] = kPassLabel[i];
   if (!ms) return HUSH_OK;
   if (!h->ev_ready) return fail(h, HUSH_ERR_ARGUMENT, "hush_pass_times: no timed frame (enable timing first)");
-  hipError_t e = hipEventSynchronize(h->ev[8]);
+  hipError_t e = hipEventSynchronize(h->ev[15]);
   if (e != hipSuccess) return hip_fail(h, e, "hipEventSynchronize");
   for (int i = 0; i < 8; i++) {
-    e = hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]);
+    e = hipEventElapsedTime(&ms[i], h->ev[2 * i], h->ev[2 * i + 1]);
     if (e != hipSuccess) return hip_fail(h, e, "hipEventElapsedTime");
   }
   return HUSH_OK;

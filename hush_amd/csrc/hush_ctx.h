@@ -40,7 +40,7 @@ struct hush_ctx {
   bool compensated = false;         // compensated (Neumaier) moment sums in the temporal statistics
   bool timing = false;              // timed frames: the eight passes one after another, events between them
   bool ev_ready = false;
-  hipEvent_t ev[9] = {};
+  hipEvent_t ev[16] = {};            // per-pass timing: start and stop of each pass's kernel
   char err[512] = {0};
 };
 

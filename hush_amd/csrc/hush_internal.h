@@ -89,6 +89,8 @@ struct TaaArgs {
   float nz;                          // -0.0f, the addend of mulh (exact products with binary16 history texels)
 };
 
+// while set (non-null) on the calling thread, the next pass kernels record start / stop on these events
+void set_pass_events(hipEvent_t start, hipEvent_t stop);
 hipError_t launch_temporal(const TemporalArgs &a, bool fast, hipStream_t s);
 hipError_t launch_mlp_tex(int mode, TexU F0, TexU F1, TexU F2, int W2, int H2, float *out, const void *wq, hipStream_t s);
 hipError_t launch_math_selftest(int which, uint32_t base, uint32_t blocks, unsigned long long *bad, uint32_t *first,
