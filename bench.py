@@ -161,14 +161,20 @@ def uhd_leg(hip, stream, steps=32, warmup=4, frames=32):
     e0, e1 = hip.Event(), hip.Event()
     for p in ("fp32", "fp16-mfma"):
         hs = hush_amd.Hush(0, {"width": W, "height": H, "precision": p})
-        for i in range(warmup):
-            hs.encode(dev[i % frames], stream=stream.handle)
-        stream.synchronize()
-        e0.record(stream.handle)
-        for i in range(steps):
+        t_r, i = time.perf_counter(), 0
+        while i < warmup or time.perf_counter() - t_r < 0.5:   # warm-up and clock ramp, untimed
             if i % frames == 0:
                 hs.reset()
             hs.encode(dev[i % frames], stream=stream.handle)
+            i += 1
+            if i % 16 == 0:
+                stream.synchronize()
+        stream.synchronize()
+        e0.record(stream.handle)
+        for j in range(i, i + steps):                          # the sequence plays on from the warm-up
+            if j % frames == 0:
+                hs.reset()
+            hs.encode(dev[j % frames], stream=stream.handle)
         e1.record(stream.handle)
         e1.synchronize()
         res[f"{p}_ms_per_frame"] = round(e0.elapsed_ms(e1) / steps, 5)
@@ -510,12 +516,16 @@ def main():
     # clock ramp: the GPU leaves its idle clock only under sustained load, and a few warm-up frames (~1 ms of
     # work) end before it gets there; play untimed frames for one pass over the sequence and at least
     # --ramp-s seconds, then the --warmup steps, then the timed steps
-    ramp_frames, t_r = 0, time.perf_counter()
-    while ramp_frames < F or time.perf_counter() - t_r < args.ramp_s:
-        run(16)
-        stream.synchronize()
-        ramp_frames += 16
-    ramp = {"frames": ramp_frames, "s": round(time.perf_counter() - t_r, 3)}
+    def clock_ramp(h):
+        n, t_r = 0, time.perf_counter()
+        while n < F or time.perf_counter() - t_r < args.ramp_s:
+            for _ in range(16):
+                h.encode(dev[next_frame(h)], stream=stream.handle)
+            stream.synchronize()
+            n += 16
+        return {"frames": n, "s": round(time.perf_counter() - t_r, 3)}
+
+    ramp = clock_ramp(hs)
     run(args.warmup)
     stream.synchronize()
     if args.mlp_stage_only:
@@ -577,6 +587,7 @@ def main():
         def frame_ms(opts):
             hm = hush_amd.Hush(0, dict({"width": W, "height": H}, **opts))
             pos[0] = 0
+            clock_ramp(hm)
             for i in range(args.warmup):
                 hm.encode(dev[next_frame(hm)], stream=stream.handle)
             stream.synchronize()
