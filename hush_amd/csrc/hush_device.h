@@ -84,6 +84,20 @@ template <bool O32> __device__ __forceinline__ void st(const Tex &t, int x, int 
 template <bool O32> __device__ __forceinline__ void stu(const TexU &t, int x, int y, uint4 v) { *texel<O32>(t.p, t.pitch, x, y) = v; }
 template <bool O32> __device__ __forceinline__ void stf(const TexF &t, int x, int y, float4 v) { *texel<O32>(t.p, t.pitch, x, y) = v; }
 template <bool O32> __device__ __forceinline__ float4 ldf(const TexF &t, int x, int y) { return *texel<O32>(t.p, t.pitch, x, y); }
+// Non-temporal (streaming) accesses: the nt cache policy for a texel that this pass reads, or writes, and no
+// later access in the frame needs from the caches (the G-buffer's own texels, outputs read only by the next
+// frame), so that it competes less with the reused texels (history taps, staged neighbourhoods) in the L2.
+typedef unsigned int u32x2v __attribute__((ext_vector_type(2)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 ld_nt(const uint2 *p) {
+  const u32x2v v = __builtin_nontemporal_load((const u32x2v *)p);
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(uint2 *p, uint2 v) { __builtin_nontemporal_store(u32x2v{v.x, v.y}, (u32x2v *)p); }
+__device__ __forceinline__ void st_nt(float4 *p, float4 v) {
+  __builtin_nontemporal_store(f32x4v{v.x, v.y, v.z, v.w}, (f32x4v *)p);
+}
+
 // low / high binary16 of a packed pair kept in a float4 lane
 __device__ __forceinline__ float2 pair_of(float w) { return unpack2(__float_as_uint(w)); }
 

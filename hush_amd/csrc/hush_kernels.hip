@@ -122,10 +122,11 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   // this pixel's G-buffer (also its own staged texel)
   // all four loads leave before any of them is waited for (the scheduler had put the first one's wait, and
   // its unpacking, between it and the other three)
+  // (streaming: non-temporal loads, the pass reads each of these texels once)
   uint2 ndr = make_uint2(0u, 0xbc000000u), mvr = make_uint2(0u, 0u), rdr = mvr, mraw = mvr;   // nd = (0, 0, 0, -1)
   if (in_img) {
-    ndr = ldr<BUF>(a.nd, px, py); mvr = ldr<BUF>(a.mv, px, py); rdr = ldr<BUF>(a.rad, px, py);
-    mraw = ldr<BUF>(a.mat, px, py);
+    ndr = ld_nt(texel<BUF>(a.nd.p, a.nd.pitch, px, py)); mvr = ld_nt(texel<BUF>(a.mv.p, a.mv.pitch, px, py));
+    rdr = ld_nt(texel<BUF>(a.rad.p, a.rad.pitch, px, py)); mraw = ld_nt(texel<BUF>(a.mat.p, a.mat.pitch, px, py));
     __builtin_amdgcn_sched_barrier(0);
   }
   // the 84 apron texels of the staged tile (threads 0..83): their normalDepth, radiance and material are
@@ -327,11 +328,11 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
   if (in_img) {
     if (nd.w < 0.0f) {                                   // sky (hush.js:115-119)
       st<BUF>(a.outA, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-      st<BUF>(a.outMom, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+      st_nt(texel<BUF>(a.outMom.p, a.outMom.pitch, px, py), make_uint2(0u, 0u));
       const uint32_t zid = pack_rtz(-1.0f, mv.z);
-      *texel<BUF>(a.outKey.p, a.outKey.pitch, px, py) = make_uint2(0u, zid);
+      st_nt(texel<BUF>(a.outKey.p, a.outKey.pitch, px, py), make_uint2(0u, zid));
       const float3 d0 = oct_dec(0.0f, 0.0f);
-      stf<BUF>(a.outKD, px, py, make_float4(d0.x, d0.y, d0.z, __uint_as_float(zid)));
+      st_nt(texel<BUF>(a.outKD.p, a.outKD.pitch, px, py), make_float4(d0.x, d0.y, d0.z, __uint_as_float(zid)));
       mbits = 0x80;
       if (AUX) a.aux[((size_t)py * a.W + px) * TEMPORAL_AUX] = -1.0f;
     } else {
@@ -398,13 +399,14 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       }
       float2 oc = oct_enc(n);
       st<BUF>(a.outA, px, py, A);
-      st<BUF>(a.outMom, px, py, make_float4(nmx, nmy, nfast, len1));
-      // the key as stored, and its exact decode for the next frame's check and this frame's fine level
+      st_nt(texel<BUF>(a.outMom.p, a.outMom.pitch, px, py), pack4(make_float4(nmx, nmy, nfast, len1)));
+      // the key as stored, and its exact decode for the next frame's check and this frame's fine level; mom,
+      // key and KD are stored non-temporal (only the next frame reads them, after this frame's other passes)
       const uint32_t okp = pack_rtz(oc.x, oc.y), zid = pack_rtz(nd.w, mv.z);
-      *texel<BUF>(a.outKey.p, a.outKey.pitch, px, py) = make_uint2(okp, zid);
+      st_nt(texel<BUF>(a.outKey.p, a.outKey.pitch, px, py), make_uint2(okp, zid));
       const float2 ok = unpack2(okp);
       const float3 dec = oct_dec(ok.x, ok.y);
-      stf<BUF>(a.outKD, px, py, make_float4(dec.x, dec.y, dec.z, __uint_as_float(zid)));
+      st_nt(texel<BUF>(a.outKD.p, a.outKD.pitch, px, py), make_float4(dec.x, dec.y, dec.z, __uint_as_float(zid)));
       // feature inputs from the stored values (hush.js:212-222)
       float aqx = qf16(A.x), aqy = qf16(A.y), aqz = qf16(A.z), aqw = qf16(A.w);
       float m1 = E * qf16(nmx);
@@ -1372,7 +1374,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
     wp2 = a.par2.p[(size_t)qy * a.par2.pitch + qx];
   }
   const int cpx = clampi(px, 0, a.W - 1), cpy = clampi(py, 0, a.H - 1);
-  const uint2 albr = a.mat.p[(size_t)cpy * a.mat.pitch + cpx], emir = a.emi.p[(size_t)cpy * a.emi.pitch + cpx];
+  const uint2 albr = ld_nt(a.mat.p + (size_t)cpy * a.mat.pitch + cpx), emir = ld_nt(a.emi.p + (size_t)cpy * a.emi.pitch + cpx);
   __builtin_amdgcn_sched_barrier(0);
   __builtin_amdgcn_s_setprio(0);
 #pragma unroll
@@ -1406,7 +1408,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   const float4 alb = unpack4(albr), emi = unpack4(emir);
   const float4 gp = sG[ty * SW + tx];
   if (gp.w < 0.0f) {
-    st(a.outHist, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    st_nt(a.outHist.p + (size_t)py * a.outHist.pitch + px, make_uint2(0u, 0u));
     st(a.outX, px, py, make_float4(emi.x, emi.y, emi.z, 0.0f));
     return;
   }
@@ -1419,7 +1421,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   constexpr bool CHK = !decltype(in_tile)::value;
   const float4 pp = sPar[(ly >> 1) * (TW / 2) + (lx >> 1)];       // par1.x, par2.x, par0.z, par0.w
   const float4 A1 = filt<1, 1, true, SW, FAST, CHK, true>(sA, sG, tx, ty, px, py, a.W, a.H, pp.x, pp.y, sL);
-  st(a.outHist, px, py, A1);
+  st_nt(a.outHist.p + (size_t)py * a.outHist.pitch + px, pack4(A1));
   const float fx = qf16(A1.x), fy = qf16(A1.y), fz = qf16(A1.z);
   const float3 n = make_float3(gp.x, gp.y, gp.z);
   const float z = gp.w;
@@ -1567,7 +1569,7 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
   if (!own) return;
   if (a.reset) {
     float3 r = itm(cur);
-    st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
+    st_nt(a.out.p + (size_t)py * a.out.pitch + px, pack4(make_float4(r.x, r.y, r.z, 1.0f)));
     return;
   }
   const int c0 = (ly + AP) * SW + lx + AP;
@@ -1607,7 +1609,7 @@ __global__ __launch_bounds__(NT) void k_taa(TaaArgs a) {
     res = make_float3(hr.x + (cur.x - hr.x) * 0.1f, hr.y + (cur.y - hr.y) * 0.1f, hr.z + (cur.z - hr.z) * 0.1f);
   }
   float3 r = itm(res);
-  st(a.out, px, py, make_float4(r.x, r.y, r.z, 1.0f));
+  st_nt(a.out.p + (size_t)py * a.out.pitch + px, pack4(make_float4(r.x, r.y, r.z, 1.0f)));
 }
 
 // ============================================================================ launchers
