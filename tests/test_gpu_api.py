@@ -114,6 +114,44 @@ def test_wide_history_gathers_give_the_same_bits(w, h):
     b.destroy()
 
 
+@pytest.mark.parametrize("w,h", [(64, 24), (63, 17), (65, 9), (1, 5)])
+def test_paired_history_taps_at_the_image_edges(w, h):
+    """the temporal pass reads a row's two horizontal colour / moment history taps as one 16-byte load (the
+    histories keep a texel of padding per row; widths 64 and 63 put that padding in a new 64-texel block or
+    fill the row's last texel): a motion field of -4 .. 4 px sends taps past both image edges (bix = -1 and
+    bix = W - 1 among them), and every output must equal the 64-bit form's, which loads each clamped tap
+    separately"""
+    gen = _scene(w, h, 43)
+    rng = np.random.default_rng(7)
+    frames = []
+    for f in range(5):
+        g = gen.render(f)
+        mv = g["motion"].view(np.float16).astype(np.float32)       # binary16 bits as generated
+        mv[..., 0] = rng.uniform(-4.0, 4.0, (h, w)).round(2)
+        mv[..., 1] = rng.uniform(-2.0, 2.0, (h, w)).round(2)
+        g["motion"] = mv.astype(np.float16).view(np.uint16)
+        frames.append(g)
+    a = hush_amd.Hush(0, {"width": w, "height": h})
+    b = hush_amd.Hush(0, {"width": w, "height": h})
+    b.set_wide_gathers(True)
+    a.set_debug_mask(True)
+    b.set_debug_mask(True)
+    assert a.surface("mom0").pitch > w * 8 and a.surface("hist0").pitch > w * 8
+    surfs, used = None, 0
+    for g in frames:
+        surfs = upload_frame(g, surfs)
+        out_a = a.denoise(surfs).download()
+        out_b = b.denoise(surfs).download()
+        assert np.array_equal(out_a.view(np.uint16), out_b.view(np.uint16))
+        for name in ("A0", "X", "F0", "F1", "F2", "F3", "mom0", "mom1", "mask"):
+            assert np.array_equal(a.surface(name).download(), b.surface(name).download()), name
+        used += int(np.count_nonzero(a.surface("mask").download()[..., 0] & 0x0f))
+    if w > 1:
+        assert used > 0                  # some bilinear taps passed their checks (at 1 x 5 none can)
+    a.destroy()
+    b.destroy()
+
+
 def test_exposure_change_matches_oracle():
     hs, o = _pair(96, 64, exposure=1.0)
     gen = _scene(96, 64, 34, "spheres")
