@@ -594,7 +594,10 @@ __device__ __forceinline__ float4 filt(const float4 *sA, const float4 *sG, int t
     gv = center.w;
   }
   const float lc = LUM ? sL[c] : lum(center.x, center.y, center.z);
-  const float sl = F::div(s, F::sqrtn(mx(gv, 1e-12f)) + 1e-6f);
+  // s / sd: sd lies in [2e-6, 256] (a square root of at least 1e-12 of binary16 variances, plus 1e-6) and s is
+  // 0.25 exp(clamp(o, +-5)) or the coarse level's 0.25, so the shared-reciprocal quotient is IEEE's
+  const float sd = F::sqrtn(mx(gv, 1e-12f)) + 1e-6f;
+  const float sl = FAST ? F::div(s, sd) : Math<false>::divs(s, sd, Math<false>::rcpn(sd));
   const float zl = sG[c - 1].w, zr = sG[c + 1].w, zu = sG[c - SW].w, zd = sG[c + SW].w;
   const float gx = mn(zr > 0.0f ? fabsf(zr - gp.w) : 1e9f, zl > 0.0f ? fabsf(zl - gp.w) : 1e9f);
   const float gy = mn(zd > 0.0f ? fabsf(zd - gp.w) : 1e9f, zu > 0.0f ? fabsf(zu - gp.w) : 1e9f);
@@ -1489,17 +1492,20 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
 
 // Karis tone map and its inverse, IEEE under both policies: itm(tm(x)) lands on x exactly for most texels
 // (the reset frame), where any negative error would drop a round-toward-zero store by a full binary16 ulp.
-// tm's divisor 1 + lum(c) is 0 for a colour with negative channels and luminance exactly -1; the quotient
-// then follows IEEE (+-inf, or NaN for a zero channel), as the shared-reciprocal quotient does for zero and
-// infinite operands.
+// The three quotients share one reciprocal, rcpn's (exact for a normal divisor).  tm's divisor 1 + lum(c) is
+// 0 or normal: near lum = -1 the sum is exact, a multiple of 2^-24.  It is 0 for a colour with negative
+// channels and luminance exactly -1; the quotient then follows IEEE (+-inf, or NaN for a zero channel) through
+// v_div_fixup, which gives the special cases from the divisor and the numerator alone.  itm's divisor is at
+// least 1e-4, or +inf when the colour is -inf after such a zero divisor (the fixup then gives IEEE's 0 or
+// NaN).  Numerators are 0 or far above 2^-102 (binary16 texels and their tone-mapped values).
 __device__ __forceinline__ float3 tm(float3 c) {
   using M = Math<false>;
-  const float d = 1.0f + lum(c.x, c.y, c.z), r = 1.0f / d;
+  const float d = 1.0f + lum(c.x, c.y, c.z), r = M::rcpn(d);
   return make_float3(M::divs(c.x, d, r), M::divs(c.y, d, r), M::divs(c.z, d, r));
 }
 __device__ __forceinline__ float3 itm(float3 c) {
   using M = Math<false>;
-  const float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f), r = 1.0f / d;
+  const float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f), r = M::rcpn(d);
   return make_float3(M::divs(c.x, d, r), M::divs(c.y, d, r), M::divs(c.z, d, r));
 }
 __device__ __forceinline__ float3 to_ycocg(float3 c) {
