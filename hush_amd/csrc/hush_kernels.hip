@@ -69,6 +69,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t hist_rsrc(const void *p, int p
 // 4x4 neighbourhood (block +-1) of the exposed, truncated luminance of a half-res texel's block
 // (hush.js:253-265) over the staged tile (row pitch TW + 2): its mean, and its variance about that mean.
 // sI[.].w is exactly 1 or 0 and an invalid texel's sQ is E * qf16(0) = 0, so w * E * qf16(lum) is sQ.
+// The count divides through rcpn's exact reciprocal (a count of 1 .. 16); a sum below 2^-102, where the
+// shared-reciprocal quotient could miss by an ulp, only occurs at exposures below 2^-58, and its quotient
+// then vanishes in the features' log(1 + x) and in the variance's squares.
 template <bool KAHAN>
 __device__ __forceinline__ float mean4x4(const float4 *sI, const float *sQ, int byl, int bxl) {
   constexpr int SW = TW + 2;
@@ -81,7 +84,8 @@ __device__ __forceinline__ float mean4x4(const float4 *sI, const float *sQ, int 
     q1.add(sQ[idx]);
     n4 += sI[idx].w;
   }
-  return Math<false>::div(q1.get(), mx(n4, 1.0f));
+  const float n = mx(n4, 1.0f);
+  return Math<false>::divs(q1.get(), n, Math<false>::rcpn(n));
 }
 template <bool KAHAN>
 __device__ __forceinline__ float var4x4(const float4 *sI, const float *sQ, int byl, int bxl, float mu4) {
@@ -96,7 +100,8 @@ __device__ __forceinline__ float var4x4(const float4 *sI, const float *sQ, int b
     q4.add(w * d * d);
     n4 += w;
   }
-  return Math<false>::div(q4.get(), mx(n4, 1.0f));
+  const float n = mx(n4, 1.0f);
+  return Math<false>::divs(q4.get(), n, Math<false>::rcpn(n));
 }
 
 template <bool AUX, bool KAHAN, bool BUF>
@@ -382,7 +387,10 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
         nmy = hm.y + (l * l - hm.y) * aa;
         nfast = hm.z + (l - hm.z) * mx(1.0f / 6.0f, F::rcpn(len1));
         float lh = lum(hcx, hcy, hcz);
-        lag = F::div(fabsf(slm - lh), F::sqrtn(mx(F::divs(svar, cnt, rcnt), 1e-12f)) + 0.02f * lh + 1e-3f);
+        // the divisor is 0 or normal (near cancellation the sum is exact, a multiple of at least 2^-44) and the
+        // numerator 0 or at least 2^-96 (luminances of demodulated binary16 texels and of their averages)
+        const float ld = F::sqrtn(mx(F::divs(svar, cnt, rcnt), 1e-12f)) + 0.02f * lh + 1e-3f;
+        lag = F::divs(fabsf(slm - lh), ld, F::rcpn(ld));
       } else {
         v = 0.0f;
       }
@@ -497,7 +505,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
 #pragma unroll
     for (int c = 0; c < 4; c++) S1[c] = qS1[c].get() * k;
     const float S03 = qS03.get() * k;
-    f0h[0] = make_uint2(pk(S03 / 32.0f, S1[0]), pk(log1p_<FAST>(S1[1]), log1p_<FAST>(F::div(fabsf(S1[2] - S1[3]), S1[3] + 1e-3f))));
+    const float fd = S1[3] + 1e-3f;        // 0 or normal (a sum with 1e-3 is exact near cancellation)
+    f0h[0] = make_uint2(pk(S03 / 32.0f, S1[0]), pk(log1p_<FAST>(S1[1]), log1p_<FAST>(F::divs(fabsf(S1[2] - S1[3]), fd, F::rcpn(fd)))));
   } else if (role == 1) {
     Sum<KAHAN> qL, qAC[4], qMV;
     float la[4];
@@ -541,7 +550,9 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
 #pragma unroll
     for (int c = 0; c < 4; c++) S3[c] = qS3[c].get() * k;
     const float Nx = qNx.get() * k, Ny = qNy.get() * k, Nz = qNz.get() * k, Z = qZ.get() * k;
-    const float ln = mx(F::sqrt(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
+    // sqrtn is sqrtf for |N|^2 >= 2^-96 or 0; a smaller |N|^2 has a root below 2^-48, and so has sqrtn's
+    // result there, which the 1e-6 floor replaces either way
+    const float ln = mx(F::sqrtn(Nx * Nx + Ny * Ny + Nz * Nz), 1e-6f);
     const float rln = F::rcpn(ln);
     const float3 Nn = make_float3(F::divsf(Nx, ln, rln), F::divsf(Ny, ln, rln), F::divsf(Nz, ln, rln));
     float zmax = -1e30f, zmin = 1e30f, mind = 1e30f;
@@ -551,7 +562,8 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       zmax = ok[j] ? mx(zmax, zs[j]) : zmax; zmin = ok[j] ? mn(zmin, zs[j]) : zmin;
       mind = ok[j] ? mn(mind, d) : mind;
     }
-    const float dz = mn(mx(F::div(zmax - zmin, mx(zmin, 1e-6f)), 0.0f), 1.0f);
+    const float zd = mx(zmin, 1e-6f);     // in [1e-6, 65504]; the numerator is a difference of binary16 depths
+    const float dz = mn(mx(F::divs(zmax - zmin, zd, F::rcpn(zd)), 0.0f), 1.0f);
     const float dn = mn(mx(1.0f - mind, 0.0f), 2.0f);
     const float2 oc = oct_enc_m<FAST>(Nn);
     stu<BUF>(a.F1, X, Y, make_uint4(pk(oc.x, oc.y), pk(dz, dn), pk(S3[0], S3[1]), pk(S3[2], S3[3])));
@@ -1212,7 +1224,7 @@ __device__ __forceinline__ void down(const TexU &F2, const TexF &GH, int QX, int
   }
   if (N == 0.0f) { col = make_float4(0.0f, 0.0f, 0.0f, 0.0f); g = make_float4(0.0f, 0.0f, -1.0f, 0.0f); return; }
   using F = Math<false>;   // stored level values: IEEE under both policies (they set the slope sl)
-  float ln = mx(F::sqrt(nx * nx + ny * ny + nz * nz), 1e-6f);
+  float ln = mx(F::sqrtn(nx * nx + ny * ny + nz * nz), 1e-6f);   // below 2^-96 the 1e-6 floor decides, as in block()
   const float rln = F::rcpn(ln), rN = F::rcpn(N);
   float2 o = oct_enc_m<FAST>(make_float3(F::divsf(nx, ln, rln), F::divsf(ny, ln, rln), F::divsf(nz, ln, rln)));
   col = make_float4(qf16(F::divsf(clx, N, rN)), qf16(F::divsf(cly, N, rN)), qf16(F::divsf(clz, N, rN)),
