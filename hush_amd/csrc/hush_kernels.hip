@@ -360,15 +360,27 @@ __global__ __launch_bounds__(NT) void k_temporal(TemporalArgs a) {
       const float mux = F::divs(s1x, cnt, rcnt), muy = F::divs(s1y, cnt, rcnt), muz = F::divs(s1z, cnt, rcnt);
       const float slm = lum(mux, muy, muz);
       Sum<KAHAN> avx, avy, avz, avl;
+      // centred sums, each term weighted by its validity t.w; when all nine texels of every active lane are
+      // valid (t.w == 1), t.w * d * d is d * d bit for bit (1 * x is x, infinities and NaN included), and the
+      // wave takes the form without the weights (a wave-uniform branch)
+      auto centred = [&](auto weighted) {
 #pragma unroll
-      for (int j = 0; j < 9; j++) {
-        int idx = c0 + (j / 3 - 1) * SW + (j % 3 - 1);
-        float4 t = sI[idx];
-        float dx = t.x - mux, dy = t.y - muy, dz = t.z - muz;
-        float dl = sL[idx] - slm;
-        avx.add(t.w * dx * dx); avy.add(t.w * dy * dy); avz.add(t.w * dz * dz);
-        avl.add(t.w * dl * dl);
-      }
+        for (int j = 0; j < 9; j++) {
+          int idx = c0 + (j / 3 - 1) * SW + (j % 3 - 1);
+          float4 t = sI[idx];
+          float dx = t.x - mux, dy = t.y - muy, dz = t.z - muz;
+          float dl = sL[idx] - slm;
+          if constexpr (decltype(weighted)::value) {
+            avx.add(t.w * dx * dx); avy.add(t.w * dy * dy); avz.add(t.w * dz * dz);
+            avl.add(t.w * dl * dl);
+          } else {
+            avx.add(dx * dx); avy.add(dy * dy); avz.add(dz * dz);
+            avl.add(dl * dl);
+          }
+        }
+      };
+      if (__all(cnt == 9.0f)) centred(std::false_type{});
+      else centred(std::true_type{});
       const float svx = avx.get(), svy = avy.get(), svz = avz.get();
       float svar = avl.get();
       const float sdx = F::sqrtn(mx(F::divs(svx, cnt, rcnt), 1e-12f)), sdy = F::sqrtn(mx(F::divs(svy, cnt, rcnt), 1e-12f)),
