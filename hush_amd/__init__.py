@@ -318,7 +318,7 @@ class Hush:
         self._check(self._L.hush_set_debug_mask(self._h, int(bool(on))))
 
     def set_wide_gathers(self, on: bool) -> None:
-        """debug: the temporal pass's history gathers on 64-bit addresses (their form for surfaces of 4 GiB)"""
+        """debug: the temporal and fine+final passes on 64-bit addresses (their form for surfaces of 2 GiB and more)"""
         self._check(self._L.hush_debug_wide_gathers(self._h, int(bool(on))))
 
 

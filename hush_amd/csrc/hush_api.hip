@@ -399,6 +399,7 @@ static FineArgs fine_args(const hush_t *h, const hush_inputs *in) {
   a.mat = in_tex(in->material); a.emi = in_tex(in->emission);
   a.outHist = t[S_HIST0 + cur]; a.outX = t[S_X];
   a.W = h->W; a.H = h->H; a.W2 = h->W2; a.H2 = h->H2; a.W4 = h->W4; a.H4 = h->H4;
+  a.wide = h->wide_gathers ? 1 : 0;
   return a;
 }
 static TaaArgs taa_args(const hush_t *h, const hush_inputs *in) {

@@ -80,6 +80,7 @@ struct FineArgs {
   Tex mid, coarse, mat, emi;
   Tex outHist, outX;
   int W, H, W2, H2, W4, H4;
+  int wide;                          // 64-bit addresses even below 2 GiB per surface (debug)
 };
 
 struct TaaArgs {

@@ -1346,7 +1346,10 @@ constexpr int HWW = TW / 2 + 2, HWH = TH / 2 + 2, HWN = HWW * HWH;      // 18 x 
 constexpr int QWW = TW / 4 + 2, QWH = TH / 4 + 2, QWN = QWW * QWH;      // 10 x 4
 constexpr int PWN = (TW / 2) * (TH / 2);                                // 16 x 4
 
-template <bool FAST>
+// O32: every surface the pass touches spans less than 2 GiB (checked by the launcher), so each access is a
+// 32-bit byte offset from the surface base (the scalar-base address form) in place of 64-bit address
+// arithmetic; larger surfaces take the 64-bit form.
+template <bool FAST, bool O32>
 __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   using F = Math<FAST>;
   constexpr int AP = 1, SW = TW + 2, SH = TH + 2, SN = SW * SH;
@@ -1375,8 +1378,8 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
     a0r[k] = make_uint2(0u, 0u); kdr[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (i < SN) {
       const int qx = clampi(x0 - AP + i % SW, 0, a.W - 1), qy = clampi(y0 - AP + i / SW, 0, a.H - 1);
-      a0r[k] = a.A0.p[(size_t)qy * a.A0.pitch + qx];
-      kdr[k] = a.kd.p[(size_t)qy * a.kd.pitch + qx];
+      a0r[k] = *texel<O32>(a.A0.p, a.A0.pitch, qx, qy);
+      kdr[k] = *texel<O32>(a.kd.p, a.kd.pitch, qx, qy);
     }
   }
   // windows: texels outside the level are staged clamped and never read (the taps skip them)
@@ -1385,23 +1388,23 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   uint2 wm = make_uint2(0u, 0u), wp1 = wm, wp2 = wm;
   if (tid < HWN) {
     const int qx = clampi(hx0 + tid % HWW, 0, a.W2 - 1), qy = clampi(hy0 + tid / HWW, 0, a.H2 - 1);
-    w4 = a.GH.p[(size_t)qy * a.GH.pitch + qx];
-    wu = a.F2.p[(size_t)qy * a.F2.pitch + qx];
-    wm = a.mid.p[(size_t)qy * a.mid.pitch + qx];
+    w4 = *texel<O32>(a.GH.p, a.GH.pitch, qx, qy);
+    wu = *texel<O32>(a.F2.p, a.F2.pitch, qx, qy);
+    wm = *texel<O32>(a.mid.p, a.mid.pitch, qx, qy);
   } else if (tid < HWN + QWN) {
     const int i = tid - HWN;
     const int qx = clampi(qx0 + i % QWW, 0, a.W4 - 1), qy = clampi(qy0 + i / QWW, 0, a.H4 - 1);
-    w4 = a.cgd.p[(size_t)qy * a.cgd.pitch + qx];
-    wm = a.coarse.p[(size_t)qy * a.coarse.pitch + qx];
+    w4 = *texel<O32>(a.cgd.p, a.cgd.pitch, qx, qy);
+    wm = *texel<O32>(a.coarse.p, a.coarse.pitch, qx, qy);
   } else if (tid < HWN + QWN + PWN) {
     const int i = tid - HWN - QWN;
     const int qx = clampi((x0 >> 1) + i % (TW / 2), 0, a.W2 - 1), qy = clampi((y0 >> 1) + i / (TW / 2), 0, a.H2 - 1);
-    wm = a.par0.p[(size_t)qy * a.par0.pitch + qx];
-    wp1 = a.par1.p[(size_t)qy * a.par1.pitch + qx];
-    wp2 = a.par2.p[(size_t)qy * a.par2.pitch + qx];
+    wm = *texel<O32>(a.par0.p, a.par0.pitch, qx, qy);
+    wp1 = *texel<O32>(a.par1.p, a.par1.pitch, qx, qy);
+    wp2 = *texel<O32>(a.par2.p, a.par2.pitch, qx, qy);
   }
   const int cpx = clampi(px, 0, a.W - 1), cpy = clampi(py, 0, a.H - 1);
-  const uint2 albr = ld_nt(a.mat.p + (size_t)cpy * a.mat.pitch + cpx), emir = ld_nt(a.emi.p + (size_t)cpy * a.emi.pitch + cpx);
+  const uint2 albr = ld_nt(texel<O32>(a.mat.p, a.mat.pitch, cpx, cpy)), emir = ld_nt(texel<O32>(a.emi.p, a.emi.pitch, cpx, cpy));
   __builtin_amdgcn_sched_barrier(0);
   __builtin_amdgcn_s_setprio(0);
 #pragma unroll
@@ -1435,8 +1438,8 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   const float4 alb = unpack4(albr), emi = unpack4(emir);
   const float4 gp = sG[ty * SW + tx];
   if (gp.w < 0.0f) {
-    st_nt(a.outHist.p + (size_t)py * a.outHist.pitch + px, make_uint2(0u, 0u));
-    st(a.outX, px, py, make_float4(emi.x, emi.y, emi.z, 0.0f));
+    st_nt(texel<O32>(a.outHist.p, a.outHist.pitch, px, py), make_uint2(0u, 0u));
+    *texel<O32>(a.outX.p, a.outX.pitch, px, py) = pack4(make_float4(emi.x, emi.y, emi.z, 0.0f));
     return;
   }
   // interior tiles (every tap of the fine filter and of both upsampling windows inside the image: all but
@@ -1448,7 +1451,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   constexpr bool CHK = !decltype(in_tile)::value;
   const float4 pp = sPar[(ly >> 1) * (TW / 2) + (lx >> 1)];       // par1.x, par2.x, par0.z, par0.w
   const float4 A1 = filt<1, 1, true, SW, FAST, CHK, true>(sA, sG, tx, ty, px, py, a.W, a.H, pp.x, pp.y, sL);
-  st_nt(a.outHist.p + (size_t)py * a.outHist.pitch + px, pack4(A1));
+  st_nt(texel<O32>(a.outHist.p, a.outHist.pitch, px, py), pack4(A1));
   const float fx = qf16(A1.x), fy = qf16(A1.y), fz = qf16(A1.z);
   const float3 n = make_float3(gp.x, gp.y, gp.z);
   const float z = gp.w;
@@ -1504,7 +1507,7 @@ __global__ __launch_bounds__(NT) void k_fine_final(FineArgs a) {
   ix = ix + (upx - ix) * pp.z;
   iy = iy + (upy - iy) * pp.z;
   iz = iz + (upz - iz) * pp.z;
-  st(a.outX, px, py, make_float4(ix * alb.x + emi.x, iy * alb.y + emi.y, iz * alb.z + emi.z, 0.0f));
+  *texel<O32>(a.outX.p, a.outX.pitch, px, py) = pack4(make_float4(ix * alb.x + emi.x, iy * alb.y + emi.y, iz * alb.z + emi.z, 0.0f));
   };
   if (interior) pixel(std::true_type{});
   else pixel(std::false_type{});
@@ -1674,7 +1677,8 @@ static void launch_pass_kernel(K kernel, dim3 grid, hipStream_t s, Args... args)
 #define K_MID3(F) k_atrous<1, 4, false, true, 3, F>
 #define K_COARSE2(F) k_atrous<2, 2, true, false, -1, F>
 #define K_COARSE1(F) k_coarse1<F>
-#define K_FINE(F) k_fine_final<F>
+#define K_FINE(F) k_fine_final<F, false>
+#define K_FINE32(F) k_fine_final<F, true>
 #define K_TAA(F) k_taa<F>
 
 template <bool BUF>
@@ -1810,7 +1814,15 @@ hipError_t launch_coarse2_mid3(const AtrousArgs &c, const AtrousArgs &m, bool fa
   return hipGetLastError();
 }
 hipError_t launch_fine_final(const FineArgs &a, bool fast, hipStream_t s) {
-  HUSH_LAUNCH(K_FINE, grid_for(a.W, a.H), a);
+  const uint64_t lim = (uint64_t)1 << 31;
+  auto small = [&](int pitch, int rows, int texel) { return (uint64_t)pitch * (uint64_t)rows * (uint64_t)texel < lim; };
+  const bool o32 = !a.wide && small(a.A0.pitch, a.H, 8) && small(a.kd.pitch, a.H, 16) && small(a.par0.pitch, a.H2, 8) &&
+                   small(a.par1.pitch, a.H2, 8) && small(a.par2.pitch, a.H2, 8) && small(a.F2.pitch, a.H2, 16) &&
+                   small(a.GH.pitch, a.H2, 16) && small(a.cgd.pitch, a.H4, 16) && small(a.mid.pitch, a.H2, 8) &&
+                   small(a.coarse.pitch, a.H4, 8) && small(a.mat.pitch, a.H, 8) && small(a.emi.pitch, a.H, 8) &&
+                   small(a.outHist.pitch, a.H, 8) && small(a.outX.pitch, a.H, 8);
+  if (o32) HUSH_LAUNCH(K_FINE32, grid_for(a.W, a.H), a);
+  else HUSH_LAUNCH(K_FINE, grid_for(a.W, a.H), a);
   return hipGetLastError();
 }
 hipError_t launch_taa(const TaaArgs &a, bool fast, hipStream_t s) {

@@ -172,9 +172,9 @@ int32_t hush_debug_run_pass(hush_t *h, int32_t pass, const hush_inputs *in, void
 /* enable the per-pixel reprojection mask: bit i (0..3) = bilinear tap i valid, bit 4 = 3x3 fallback
    used, bit 5 = history used (has), bit 7 = sky */
 int32_t hush_set_debug_mask(hush_t *h, int32_t enabled);
-/* force the temporal pass's history gathers onto 64-bit addresses (the form taken when a history surface
-   reaches 4 GiB; smaller ones use buffer descriptors with 32-bit offsets), so that tests can check that both
-   forms give the same bits */
+/* force the temporal and fine+final passes onto 64-bit addresses (the form taken when a surface they touch
+   reaches 2 GiB; smaller ones use 32-bit byte offsets, the temporal pass's history gathers through buffer
+   descriptors), so that tests can check that both forms give the same bits */
 int32_t hush_debug_wide_gathers(hush_t *h, int32_t enabled);
 /* self-test of the reduced-range square root, exponential, reciprocal and logarithm that the kernels use
    where their arguments allow it, against the toolchain's own sqrtf / expf / IEEE division / logf evaluated

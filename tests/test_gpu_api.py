@@ -95,8 +95,9 @@ def test_reset_equals_fresh_first_frame():
 
 @pytest.mark.parametrize("w,h", [(96, 64), (333, 197)])
 def test_wide_history_gathers_give_the_same_bits(w, h):
-    """the temporal pass gathers its history through 32-bit buffer offsets below 4 GiB per surface and through
-    64-bit addresses above; forced onto the 64-bit form, a sequence gives the same bits in every output"""
+    """the temporal pass gathers its history through 32-bit buffer offsets below 2 GiB per surface, and the
+    temporal and fine+final passes address their surfaces with 32-bit byte offsets there, and through 64-bit
+    addresses above; forced onto the 64-bit forms, a sequence gives the same bits in every output"""
     gen = _scene(w, h, 41)
     frames = [gen.render(f) for f in range(5)]
     a = hush_amd.Hush(0, {"width": w, "height": h})
@@ -108,7 +109,7 @@ def test_wide_history_gathers_give_the_same_bits(w, h):
         out_a = a.denoise(surfs).download()
         out_b = b.denoise(surfs).download()
         assert np.array_equal(out_a.view(np.uint16), out_b.view(np.uint16))
-        for name in ("A0", "X", "F0", "F3", "key0", "key1"):
+        for name in ("A0", "X", "F0", "F3", "key0", "key1", "hist0", "hist1"):
             assert np.array_equal(a.surface(name).download(), b.surface(name).download()), name
     a.destroy()
     b.destroy()
