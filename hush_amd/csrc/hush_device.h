@@ -84,6 +84,9 @@ template <bool O32> __device__ __forceinline__ void st(const Tex &t, int x, int 
 template <bool O32> __device__ __forceinline__ void stu(const TexU &t, int x, int y, uint4 v) { *texel<O32>(t.p, t.pitch, x, y) = v; }
 template <bool O32> __device__ __forceinline__ void stf(const TexF &t, int x, int y, float4 v) { *texel<O32>(t.p, t.pitch, x, y) = v; }
 template <bool O32> __device__ __forceinline__ float4 ldf(const TexF &t, int x, int y) { return *texel<O32>(t.p, t.pitch, x, y); }
+// ld / ldu on 32-bit byte offsets (surfaces below 2 GiB)
+__device__ __forceinline__ float4 ld32(const Tex &t, int x, int y) { return unpack4(*texel<true>(t.p, t.pitch, x, y)); }
+__device__ __forceinline__ uint4 ldu32(const TexU &t, int x, int y) { return *texel<true>(t.p, t.pitch, x, y); }
 // Non-temporal (streaming) accesses: the nt cache policy for a texel that this pass reads, or writes, and no
 // later access in the frame needs from the caches (the G-buffer's own texels, outputs read only by the next
 // frame), so that it competes less with the reused texels (history taps, staged neighbourhoods) in the L2.

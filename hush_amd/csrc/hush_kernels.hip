@@ -7,6 +7,7 @@
 //
 // The arithmetic follows the reference's operation order with no FMA contraction (build flag
 // -ffp-contract=off); the MLP's multiply-adds are fused explicitly (__builtin_fmaf).
+#include "hush.h"
 #include "hush_device.h"
 #include <hip/hip_ext.h>
 #include <type_traits>
@@ -16,6 +17,12 @@
 namespace hush {
 
 constexpr int TW = 32, TH = 8, NT = 256;
+
+// The half- and quarter-resolution surfaces are the context's own, and at the largest frame side they span
+// at most (HUSH_MAX_SIDE / 2)^2 texels of 16 B, 1 GiB: the passes that touch only them (infer + mid1, mid2,
+// mid3, coarse1, coarse2) address them with 32-bit byte offsets from the surface base at every size.
+static_assert((uint64_t)((HUSH_MAX_SIDE + 1) / 2 + 63) / 64 * 64 * ((HUSH_MAX_SIDE + 1) / 2) * 16 < ((uint64_t)1 << 31),
+              "half-resolution surfaces must stay below 2 GiB for 32-bit byte offsets");
 
 // ============================================================================ pass 1: temporal
 // TEMPORAL (hush.js:79-308): demodulate, 3x3 stats, reproject (bilinear + 3x3 fallback), clamp and blend,
@@ -737,10 +744,10 @@ __device__ __forceinline__ void infer_outputs(const InferArgs &a, const float4 *
     s[k] = 0.25f * F::expm(mn(mx(o[4 + k], -5.0f), 5.0f));
     c[k] = F::expm(mn(mx(o[8 + k], -4.0f), 4.0f));
   }
-  st(a.out0, X, Y, make_float4(alpha, gamma, beta, kappa));
-  st(a.out1, X, Y, make_float4(s[0], s[1], s[2], s[3]));
-  st(a.out2, X, Y, make_float4(c[0], c[1], c[2], c[3]));
-  st(a.outMid, X, Y, filt<1, 1, true, SW, FAST>(sA, sG, tx, ty, X, Y, a.W2, a.H2, qf16(s[1]), qf16(c[1])));
+  st<true>(a.out0, X, Y, make_float4(alpha, gamma, beta, kappa));
+  st<true>(a.out1, X, Y, make_float4(s[0], s[1], s[2], s[3]));
+  st<true>(a.out2, X, Y, make_float4(c[0], c[1], c[2], c[3]));
+  st<true>(a.outMid, X, Y, filt<1, 1, true, SW, FAST>(sA, sG, tx, ty, X, Y, a.W2, a.H2, qf16(s[1]), qf16(c[1])));
 }
 
 __device__ __forceinline__ void unpack_features(uint4 f0, uint4 f1, uint4 f2, float *f) {
@@ -765,10 +772,10 @@ __device__ __forceinline__ void infer_tile(const InferArgs &a, int n, float4 *ld
   const int x0 = org.x, y0 = org.y;
   for (int i = tid; i < SN; i += NT) {
     int qx = clampi(x0 - AP + i % SW, 0, a.W2 - 1), qy = clampi(y0 - AP + i / SW, 0, a.H2 - 1);
-    uint4 g2 = ldu(a.F2, qx, qy);
+    uint4 g2 = ldu32(a.F2, qx, qy);
     float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
     sA[i] = make_float4(a01.x, a01.y, a23.x, a23.y);
-    sG[i] = guide_gh(ldf(a.GH, qx, qy));
+    sG[i] = guide_gh(ldf<true>(a.GH, qx, qy));
   }
   __syncthreads();
   const int lx = tid & 31, ly = tid >> 5;
@@ -777,11 +784,11 @@ __device__ __forceinline__ void infer_tile(const InferArgs &a, int n, float4 *ld
   const int tx = lx + AP, ty = ly + AP;
   if (sG[ty * SW + tx].w < 0.0f) {
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    st(a.out0, X, Y, z); st(a.out1, X, Y, z); st(a.out2, X, Y, z); st(a.outMid, X, Y, z);
+    st<true>(a.out0, X, Y, z); st<true>(a.out1, X, Y, z); st<true>(a.out2, X, Y, z); st<true>(a.outMid, X, Y, z);
     return;
   }
   float f[20];
-  unpack_features(ldu(a.F0, X, Y), ldu(a.F1, X, Y), ldu(a.F2, X, Y), f);
+  unpack_features(ldu32(a.F0, X, Y), ldu32(a.F1, X, Y), ldu32(a.F2, X, Y), f);
   float o[12];
   if (MODE == 0) mlp_fp32<HID>(a.w, a.hidden, f, o);
   else mlp_f16(a.w, a.hidden, f, o);
@@ -984,11 +991,11 @@ __device__ __forceinline__ void mlp_mfma_x4(const MfmaPack *__restrict__ P, cons
 __device__ __forceinline__ uint4 feature_frag(const TexU &F0, const TexU &F1, const TexU &F2, int x, int y, int kb, bool live) {
   if (!live || kb == 3) return make_uint4(0u, 0u, 0u, 0u);
   if (kb == 2) {
-    const uint2 v = *(const uint2 *)(F2.p + (size_t)y * F2.pitch + x);
+    const uint2 v = *(const uint2 *)texel<true>(F2.p, F2.pitch, x, y);
     return make_uint4(v.x, v.y, 0u, 0u);
   }
   const TexU &F = kb == 0 ? F0 : F1;
-  return F.p[(size_t)y * F.pitch + x];
+  return *texel<true>(F.p, F.pitch, x, y);
 }
 
 // MFMA forms in the frame's inference pass.  Lane (kb, tl) of wave v loads its B fragments of the wave's four
@@ -1024,8 +1031,8 @@ __device__ __forceinline__ void infer_mfma_tile(const InferArgs &a, int n, float
   for (int k = 0; k < NI; k++) {
     const int i = mn(tid + k * NT, SN - 1);
     const int qx = clampi(x0 - AP + i % SW, 0, a.W2 - 1), qy = clampi(y0 - AP + i / SW, 0, a.H2 - 1);
-    g2r[k] = ldu(a.F2, qx, qy);
-    ghr[k] = ldf(a.GH, qx, qy);
+    g2r[k] = ldu32(a.F2, qx, qy);
+    ghr[k] = ldf<true>(a.GH, qx, qy);
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1049,7 +1056,7 @@ __device__ __forceinline__ void infer_mfma_tile(const InferArgs &a, int n, float
   const int tx = lx + AP, ty = ly + AP;
   if (sG[ty * SW + tx].w < 0.0f) {
     const float4 zr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    st(a.out0, X, Y, zr); st(a.out1, X, Y, zr); st(a.out2, X, Y, zr); st(a.outMid, X, Y, zr);
+    st<true>(a.out0, X, Y, zr); st<true>(a.out1, X, Y, zr); st<true>(a.out2, X, Y, zr); st<true>(a.outMid, X, Y, zr);
     return;
   }
   float o[12];
@@ -1171,8 +1178,8 @@ __device__ __forceinline__ void atrous_tile(const AtrousArgs &a, int n, float4 *
   uint2 p1r = make_uint2(0u, 0u), p2r = p1r;
   if (COMP >= 0) {
     const int cX = clampi(X, 0, a.w - 1), cY = clampi(Y, 0, a.h - 1);
-    p1r = a.par1.p[(size_t)cY * a.par1.pitch + cX];
-    p2r = a.par2.p[(size_t)cY * a.par2.pitch + cX];
+    p1r = *texel<true>(a.par1.p, a.par1.pitch, cX, cY);
+    p2r = *texel<true>(a.par2.p, a.par2.pitch, cX, cY);
   }
   uint2 ar[NI];
   float4 gr[NI];
@@ -1180,8 +1187,8 @@ __device__ __forceinline__ void atrous_tile(const AtrousArgs &a, int n, float4 *
   for (int k = 0; k < NI; k++) {
     const int i = mn(tid + k * NT, SN - 1);
     const int qx = clampi(x0 - AP + i % SW, 0, a.w - 1), qy = clampi(y0 - AP + i / SW, 0, a.h - 1);
-    ar[k] = a.inA.p[(size_t)qy * a.inA.pitch + qx];
-    gr[k] = a.guide.p[(size_t)qy * a.guide.pitch + qx];
+    ar[k] = *texel<true>(a.inA.p, a.inA.pitch, qx, qy);
+    gr[k] = *texel<true>(a.guide.p, a.guide.pitch, qx, qy);
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1195,7 +1202,7 @@ __device__ __forceinline__ void atrous_tile(const AtrousArgs &a, int n, float4 *
   __syncthreads();
   if (!inside(X, Y, a.w, a.h)) return;
   const int tx = lx + AP, ty = ly + AP;
-  if (sG[ty * SW + tx].w < 0.0f) { st(a.out, X, Y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
+  if (sG[ty * SW + tx].w < 0.0f) { st<true>(a.out, X, Y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
   float s, c;
   if (COMP < 0) { s = a.s; c = 1.0f; }
   else {
@@ -1203,7 +1210,7 @@ __device__ __forceinline__ void atrous_tile(const AtrousArgs &a, int n, float4 *
     s = COMP == 0 ? p1.x : COMP == 1 ? p1.y : COMP == 2 ? p1.z : p1.w;
     c = COMP == 0 ? p2.x : COMP == 1 ? p2.y : COMP == 2 ? p2.z : p2.w;
   }
-  st(a.out, X, Y, filt<R, STEP, PRE, SW, FAST>(sA, sG, tx, ty, X, Y, a.w, a.h, s, c));
+  st<true>(a.out, X, Y, filt<R, STEP, PRE, SW, FAST>(sA, sG, tx, ty, X, Y, a.w, a.h, s, c));
 }
 template <int R, int STEP, bool PRE, bool GUIDE_PACKED, int COMP, bool FAST>
 __global__ __launch_bounds__(NT) void k_atrous(AtrousArgs a) {
@@ -1222,10 +1229,10 @@ __device__ __forceinline__ void down(const TexU &F2, const TexF &GH, int QX, int
   for (int j = 0; j < 4; j++) {
     int hx = QX * 2 + (j & 1), hy = QY * 2 + (j >> 1);
     if (hx >= W2 || hy >= H2) continue;
-    const float4 gh = ldf(GH, hx, hy);
+    const float4 gh = ldf<true>(GH, hx, hy);
     float2 oz = pair_of(gh.w);
     if (oz.y <= 0.0f) continue;
-    uint4 g2 = ldu(F2, hx, hy);
+    uint4 g2 = ldu32(F2, hx, hy);
     float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
     float w = oz.y;
     N += w;
@@ -1261,10 +1268,10 @@ __device__ __forceinline__ void coarse1_tile(const Coarse1Args &a, int n, float4
     sG[i] = guide_tex<FAST>(g);
     if (lqx == qx && lqy == qy && lqx >= x0 && lqy >= y0 && lqx < x0 + TW && lqy < y0 + TH) {
       const uint2 gp = pack4(g);
-      a.outG.p[(size_t)qy * a.outG.pitch + qx] = gp;
+      *texel<true>(a.outG.p, a.outG.pitch, qx, qy) = gp;
       const float4 gs = unpack4(gp);
       const float3 d = oct_dec(gs.x, gs.y);
-      stf(a.outGD, qx, qy, make_float4(d.x, d.y, d.z, gs.z));
+      stf<true>(a.outGD, qx, qy, make_float4(d.x, d.y, d.z, gs.z));
     }
   }
   __syncthreads();
@@ -1272,8 +1279,8 @@ __device__ __forceinline__ void coarse1_tile(const Coarse1Args &a, int n, float4
   const int X = x0 + lx, Y = y0 + ly;
   if (!inside(X, Y, a.W4, a.H4)) return;
   const int tx = lx + AP, ty = ly + AP;
-  if (sG[ty * SW + tx].w < 0.0f) { st(a.outC, X, Y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
-  st(a.outC, X, Y, filt<2, 1, true, SW, FAST>(sA, sG, tx, ty, X, Y, a.W4, a.H4, a.s, 1.0f));
+  if (sG[ty * SW + tx].w < 0.0f) { st<true>(a.outC, X, Y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
+  st<true>(a.outC, X, Y, filt<2, 1, true, SW, FAST>(sA, sG, tx, ty, X, Y, a.W4, a.H4, a.s, 1.0f));
 }
 template <bool FAST>
 __global__ __launch_bounds__(NT) void k_coarse1(Coarse1Args a) {
