@@ -1210,7 +1210,10 @@ __device__ __forceinline__ void atrous_tile(const AtrousArgs &a, int n, float4 *
     s = COMP == 0 ? p1.x : COMP == 1 ? p1.y : COMP == 2 ? p1.z : p1.w;
     c = COMP == 0 ? p2.x : COMP == 1 ? p2.y : COMP == 2 ? p2.z : p2.w;
   }
-  st<true>(a.out, X, Y, filt<R, STEP, PRE, SW, FAST>(sA, sG, tx, ty, X, Y, a.w, a.h, s, c));
+  // interior tiles (every tap of every pixel inside the level) skip the per-tap bounds tests
+  const bool interior = x0 >= AP && y0 >= AP && x0 + TW + AP <= a.w && y0 + TH + AP <= a.h;
+  if (interior) st<true>(a.out, X, Y, filt<R, STEP, PRE, SW, FAST, false>(sA, sG, tx, ty, X, Y, a.w, a.h, s, c));
+  else st<true>(a.out, X, Y, filt<R, STEP, PRE, SW, FAST>(sA, sG, tx, ty, X, Y, a.w, a.h, s, c));
 }
 template <int R, int STEP, bool PRE, bool GUIDE_PACKED, int COMP, bool FAST>
 __global__ __launch_bounds__(NT) void k_atrous(AtrousArgs a) {
