@@ -1228,16 +1228,25 @@ __global__ __launch_bounds__(NT) void k_atrous(AtrousArgs a) {
 template <bool FAST>
 __device__ __forceinline__ void down(const TexU &F2, const TexF &GH, int QX, int QY, int W2, int H2, float4 &col, float4 &g) {
   float N = 0.0f, vr = 0.0f, z = 0.0f, clx = 0.0f, cly = 0.0f, clz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  // every tap's guide and level texel are loaded at once (clamped into the level); a tap outside the level or
+  // with no valid pixel takes weight 0 and adds exact zeros (the staged values are finite and every sum
+  // starts at +0), so the sums are those of the skipping form bit for bit
+  float4 ghv[4];
+  uint4 g2v[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    int hx = QX * 2 + (j & 1), hy = QY * 2 + (j >> 1);
-    if (hx >= W2 || hy >= H2) continue;
-    const float4 gh = ldf<true>(GH, hx, hy);
-    float2 oz = pair_of(gh.w);
-    if (oz.y <= 0.0f) continue;
-    uint4 g2 = ldu32(F2, hx, hy);
-    float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
-    float w = oz.y;
+    const int hx = mn(QX * 2 + (j & 1), W2 - 1), hy = mn(QY * 2 + (j >> 1), H2 - 1);
+    ghv[j] = ldf<true>(GH, hx, hy);
+    g2v[j] = ldu32(F2, hx, hy);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int hx = QX * 2 + (j & 1), hy = QY * 2 + (j >> 1);
+    const float4 gh = ghv[j];
+    const float2 oz = pair_of(gh.w);
+    const uint4 g2 = g2v[j];
+    const float2 a01 = unpack2(g2.z), a23 = unpack2(g2.w);
+    const float w = ((hx < W2) & (hy < H2) & (oz.y > 0.0f)) ? oz.y : 0.0f;
     N += w;
     clx += w * a01.x; cly += w * a01.y; clz += w * a23.x;
     vr += w * a23.y;
