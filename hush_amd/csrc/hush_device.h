@@ -264,7 +264,23 @@ __device__ __forceinline__ float pow128(float x) {                            //
   return x64 * x64;
 }
 
-template <bool FAST> __device__ __forceinline__ float sigmoid(float x) { return Math<FAST>::rcp(1.0f + Math<FAST>::exp(-x)); }
+// 1 / (1 + e^-x) (hush.js:357-366).  IEEE: the reduced exponential and reciprocal on -x clamped to [-30, 87]
+// (the constant first in mn / mx keeps a NaN), so 1 + e^t is normal and below 2^126.  Below -30, 1 + e^-x
+// rounds to 1 either way; above 87 the exact value is below 2^-125 and this one returns 1 / (1 + e^87),
+// about 1.6e-38, instead.  That difference never reaches a stored bit: 1/32 + (31/32) s and 0.5 + 7.5 s
+// round to 1/32 and 0.5, and s itself truncates to binary16 zero (hush_debug_math_selftest checks the
+// three stored activations against 1 / (1 + expf(-x)) for every binary32 x).
+template <bool FAST> __device__ __forceinline__ float sigmoid(float x) {
+  if constexpr (FAST) {
+    return Math<true>::rcp(1.0f + Math<true>::exp(-x));
+  } else {
+    const float t = mx(-30.0f, mn(87.0f, -x));
+    return Math<false>::rcpn(1.0f + Math<false>::expm(t));
+  }
+}
+// the activations of the first three MLP outputs (hush.js:357-366): alpha, gamma, and beta / kappa
+__device__ __forceinline__ float act_alpha(float s) { return 1.0f / 32.0f + (31.0f / 32.0f) * s; }
+__device__ __forceinline__ float act_gamma(float s) { return 0.5f + 7.5f * s; }
 
 // octEnc (hush.js:34-39)
 // The shared reciprocal RN(1 / d) comes from rcpn (exact for normal d below 2^126, which d >= 1e-20 is

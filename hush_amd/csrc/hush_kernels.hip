@@ -734,8 +734,8 @@ __device__ __forceinline__ void infer_outputs(const InferArgs &a, const float4 *
                                               int X, int Y, const float *o) {
   using F = Math<FAST>;
   constexpr int SW = TW + 2;
-  const float alpha = 1.0f / 32.0f + (31.0f / 32.0f) * sigmoid<FAST>(o[0]);
-  const float gamma = 0.5f + 7.5f * sigmoid<FAST>(o[1]);
+  const float alpha = act_alpha(sigmoid<FAST>(o[0]));
+  const float gamma = act_gamma(sigmoid<FAST>(o[1]));
   const float beta = sigmoid<FAST>(o[2]);
   const float kappa = sigmoid<FAST>(o[3]);
   float s[4], c[4];
@@ -1872,6 +1872,15 @@ __global__ __launch_bounds__(NT) void k_math_selftest(int which, uint32_t base, 
   } else if (which == 4) {
     if ((u & 0x7fffffffu) >= 1u && (u & 0x7fffffffu) < 0x00800000u) return;   // every input but a subnormal
     r = Math<false>::logn(x); ref = logf(x);
+  } else if (which == 6) {
+    // the stored activations (binary16, truncated) of sigmoid against 1 / (1 + expf(-x)), every x
+    const float s = sigmoid<false>(x), sr = 1.0f / (1.0f + expf(-x));
+    const uint32_t a = pack_rtz(act_alpha(s), act_gamma(s)), ar = pack_rtz(act_alpha(sr), act_gamma(sr));
+    const uint32_t b = pack_rtz(s, 0.0f), br = pack_rtz(sr, 0.0f);
+    if (a == ar && b == br) return;
+    atomicAdd(bad, 1ull);
+    atomicMin(first, u);
+    return;
   } else {
     // divs(x, d, RN(1/d)) against x / d for every numerator of its domain over divisors that cover its
     // domain: +-0, +-inf, NaN, the extremes 2^-126 and 2^126 - ulp, and significands with hard patterns at

@@ -8,7 +8,9 @@ non-positive edge-stopping and joint-bilateral exponents and of the MLP outputs 
 reciprocals of sums and counts bounded away from 0 and infinity (at least 1e-6, at most ~1.5e6);
 logarithms of 1 + x (zero or at least 2^-24 in magnitude near x = -1, never subnormal).  The shared-reciprocal
 division (Math<false>::divs) against IEEE division for every numerator of its domain over 24 divisors: the
-special ones (+-0, +-inf, NaN), the normal extremes and hard significands."""
+special ones (+-0, +-inf, NaN), the normal extremes and hard significands.  The reduced sigmoid of the MLP's
+first four outputs (hush_device.h: sigmoid<false>) through the activations as stored (1/32 + 31/32 s,
+0.5 + 7.5 s and s itself, truncated to binary16) against 1 / (1 + expf(-x)) for every binary32 x."""
 import ctypes
 
 import pytest
@@ -21,7 +23,8 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("which,name", [(0, "sqrt, x >= 2^-96, +0, +inf, NaN"), (1, "exp, x <= 0, NaN"),
                                         (2, "exp, -103.28 <= x <= 88.72"), (3, "rcp, normal |x| < 2^126, NaN"),
                                         (4, "log, all but subnormals"),
-                                        (5, "shared-reciprocal division, every numerator x 24 divisors")])
+                                        (5, "shared-reciprocal division, every numerator x 24 divisors"),
+                                        (6, "sigmoid's stored activations, every x")])
 def test_reduced_range_math_matches_toolchain(which, name):
     L = _lib.load()
     bad, first = ctypes.c_uint64(0), ctypes.c_uint32(0)
