@@ -4,9 +4,8 @@
 // Arithmetic mirrors the WGSL operation by operation (FMA contraction off for the whole library), with
 // correctly rounded division and square root.  Two choices go beyond the reference:
 //   - pow(x, sigmaN) with a positive integer exponent (the default 128) is evaluated by binary
-//     exponentiation in FP64, whose vector rate on CDNA4 equals non-packed FP32: the FP32 result is the
-//     correctly rounded x^n except when x^n lies within ~1e-14 (relative) of a rounding boundary;
-//     other exponents use the library powf;
+//     exponentiation in FP32 (x^128 is seven squarings, hush.js's own pow128), the products the oracle's
+//     FP32 mirror forms in the same order; other exponents use the library powf;
 //   - the resolve's linear sampler (svgf.js:248, 261) is evaluated as exact bilinear weights at
 //     uv * size - 0.5 with clamp-to-edge, the same formula as the oracle; a hardware sampler's sub-texel
 //     weight precision is implementation-defined.
@@ -17,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 #include "hush_device.h"
 #include "svgf.h"
@@ -42,7 +42,8 @@ struct SvVarianceArgs {
 
 struct SvAtrousArgs {
   Tex in, nd, out, fb;
-  int W, H, step, n_int;
+  int W, H, lg_step, n_int;
+  int n_items;                       // row-phase tiling: regions (of TX x TY s pixels) x s row phases
   float sigma_l, sigma_n, sigma_z;
 };
 
@@ -172,16 +173,52 @@ __global__ __launch_bounds__(NT) void k_sv_temporal(SvTemporalArgs a) {
 }
 
 // VARIANCE (svgf.js:114-148): temporal variance for long histories, a 7x7 depth / normal weighted
-// spatial estimate (scaled by 4 / history length) for histories shorter than 4 frames
+// spatial estimate (scaled by 4 / history length) for histories shorter than 4 frames.  A tile with any such
+// pixel stages its 38x14 window once in LDS (normal and depth, colour, moments; unpacked, clamp-to-edge, the
+// taps outside the image skipped); a tile without one skips the staging.  Each tap's arithmetic and the sum
+// order are those of the WGSL loop.
 template <bool P128>                 // sigmaN = 128
 __global__ __launch_bounds__(NT) void k_sv_variance(SvVarianceArgs a) {
-  const int2 p = pixel_of(a.W, a.H);
-  if (p.x >= a.W || p.y >= a.H) return;
-  const float4 nd = ld(a.nd, p.x, p.y);
-  if (nd.w < 0.0f) { st(a.out, p.x, p.y, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
-  const float4 c = ld(a.col, p.x, p.y), m = ld(a.mom, p.x, p.y);
+  constexpr int AP = 3, SW = TW + 2 * AP, SH = TH + 2 * AP, SN = SW * SH, NI = (SN + NT - 1) / NT;
+  __shared__ float4 sN[SN], sC[SN];
+  __shared__ float2 sM[SN];
+  const int tid = (int)threadIdx.x;
+  const int2 o = tile_origin<TW, TH>(a.W, a.H);
+  const int lx = tid & (TW - 1), ly = tid / TW, px = o.x + lx, py = o.y + ly;
+  const bool own = px < a.W && py < a.H;
+  const int cx = mn(px, a.W - 1), cy = mn(py, a.H - 1);
+  const float4 nd = ld(a.nd, cx, cy), c = ld(a.col, cx, cy), m = ld(a.mom, cx, cy);
+  const bool spatial = own && !(nd.w < 0.0f) && !(m.z >= 4.0f);
+  if (!__syncthreads_or(spatial)) {
+    if (!own) return;
+    if (nd.w < 0.0f) st(a.out, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    else st(a.out, px, py, make_float4(c.x, c.y, c.z, mx(0.0f, m.y - m.x * m.x)));
+    return;
+  }
+  uint2 nr[NI], cr[NI], mr[NI];
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    const int i = mn(tid + k * NT, SN - 1);
+    const int qx = clampi(o.x - AP + i % SW, 0, a.W - 1), qy = clampi(o.y - AP + i / SW, 0, a.H - 1);
+    nr[k] = a.nd.p[(size_t)qy * a.nd.pitch + qx];
+    cr[k] = a.col.p[(size_t)qy * a.col.pitch + qx];
+    mr[k] = a.mom.p[(size_t)qy * a.mom.pitch + qx];
+  }
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    const int i = tid + k * NT;
+    if (i < SN) {
+      sN[i] = unpack4(nr[k]);
+      sC[i] = unpack4(cr[k]);
+      const float4 mq = unpack4(mr[k]);
+      sM[i] = make_float2(mq.x, mq.y);
+    }
+  }
+  __syncthreads();
+  if (!own) return;
+  if (nd.w < 0.0f) { st(a.out, px, py, make_float4(0.0f, 0.0f, 0.0f, 0.0f)); return; }
   if (m.z >= 4.0f) {
-    st(a.out, p.x, p.y, make_float4(c.x, c.y, c.z, mx(0.0f, m.y - m.x * m.x)));
+    st(a.out, px, py, make_float4(c.x, c.y, c.z, mx(0.0f, m.y - m.x * m.x)));
     return;
   }
   // the depth term's divisor 0.02 z len + 1e-3, one per squared tap distance len^2 = dx^2 + dy^2 (0 .. 18)
@@ -193,62 +230,117 @@ __global__ __launch_bounds__(NT) void k_sv_variance(SvVarianceArgs a) {
     zrcp[d2] = sv_rcp(zdiv[d2]);
   }
   float sw = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sm0 = 0.0f, sm1 = 0.0f;
+  const int cc = (ly + AP) * SW + lx + AP;
+  // interior tiles (every tap of every pixel inside the image) skip the per-tap bounds tests
+  const bool interior = o.x >= AP && o.y >= AP && o.x + TW + AP <= a.W && o.y + TH + AP <= a.H;
+  auto taps = [&](auto in_tile) {
+    constexpr bool CHK = !decltype(in_tile)::value;
 #pragma unroll
-  for (int dy = -3; dy <= 3; dy++) {
-    const int qy = p.y + dy;
-    if (qy < 0 || qy >= a.H) continue;
+    for (int dy = -3; dy <= 3; dy++) {
 #pragma unroll
-    for (int dx = -3; dx <= 3; dx++) {
-      const int qx = p.x + dx;
-      if (qx < 0 || qx >= a.W) continue;
-      const float4 ndq = ld(a.nd, qx, qy);
-      if (ndq.w < 0.0f) continue;
-      const int d2 = dx * dx + dy * dy;
-      const float wz = Math<false>::expn(sv_div(-fabsf(ndq.w - nd.w), zdiv[d2], zrcp[d2]));
-      const float dn = mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z);
-      const float wn = P128 ? sv_pow128(dn) : sv_pow(dn, a.n_int, a.sigma_n);
-      const float w = wz * wn;
-      sw += w;
-      const float4 cq = ld(a.col, qx, qy), mq = ld(a.mom, qx, qy);
-      s0 += w * cq.x; s1 += w * cq.y; s2 += w * cq.z;
-      sm0 += w * mq.x; sm1 += w * mq.y;
+      for (int dx = -3; dx <= 3; dx++) {
+        if (CHK && !inside(px + dx, py + dy, a.W, a.H)) continue;
+        const int q = cc + dy * SW + dx;
+        const float4 ndq = sN[q];
+        if (ndq.w < 0.0f) continue;
+        const int d2 = dx * dx + dy * dy;
+        const float wz = Math<false>::expn(sv_div(-fabsf(ndq.w - nd.w), zdiv[d2], zrcp[d2]));
+        const float dn = mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z);
+        const float wn = P128 ? sv_pow128(dn) : sv_pow(dn, a.n_int, a.sigma_n);
+        const float w = wz * wn;
+        sw += w;
+        const float4 cq = sC[q];
+        const float2 mq = sM[q];
+        s0 += w * cq.x; s1 += w * cq.y; s2 += w * cq.z;
+        sm0 += w * mq.x; sm1 += w * mq.y;
+      }
     }
-  }
+  };
+  if (interior) taps(std::true_type{});
+  else taps(std::false_type{});
   const float d = mx(sw, 1e-6f);
   s0 /= d; s1 /= d; s2 /= d; sm0 /= d; sm1 /= d;
   const float v = mx(0.0f, sm1 - sm0 * sm0) * 4.0f / mx(m.z, 1.0f);
-  st(a.out, p.x, p.y, make_float4(s0, s1, s2, v));
+  st(a.out, px, py, make_float4(s0, s1, s2, v));
 }
 
-// ATROUS (svgf.js:157-216): one 5x5 B3-spline wavelet iteration with taps `step` texels apart, depth /
-// normal / luminance edge-stopping; iteration 0 also writes the colour feedback (the next frame's history)
+// ATROUS (svgf.js:157-216): one 5x5 B3-spline wavelet iteration with taps `step` = 2^LG texels apart, depth /
+// normal / luminance edge-stopping; iteration 0 also writes the colour feedback (the next frame's history).
+// Row-phase tiling: the rows y = ry + s v of one phase ry in [0, s) hold every vertical tap of their pixels,
+// so a workgroup filters TX contiguous columns x TY rows of one phase from a window of (TX + 4s) columns x
+// (TY + 4) such rows staged once in LDS (colour and variance, luminance, normal and depth, unpacked; clamp-to-edge,
+// the taps outside the image are skipped), in place of loading and unpacking every tap per pixel.  Window
+// rows and the pixels' own +-1 neighbourhoods are contiguous runs of a row, so every load is coalesced.
+// Workgroups walk regions of TX x TY s pixels x their s row phases, phases fastest, in one contiguous band per
+// XCD (tile_origin's mapping), so the phases of one region share an L2.  Each tap's arithmetic and the sum
+// order are those of the WGSL loop.
 // SHARED: the edge-stopping quotients as shared-reciprocal ones (sigmaL and sigmaZ in [0, 2^60]), else IEEE
-template <bool FIRST, bool P128, bool SHARED>     // P128: sigmaN = 128
+// tile of sv_tile_x(lg) x 256 / sv_tile_x(lg) pixels: wider and lower at the largest step, where the x apron
+// dominates the window
+constexpr int sv_tile_x(int lg) { return lg >= 4 ? 64 : 32; }
+template <int LG> struct SvWin {
+  static constexpr int S = 1 << LG, TX = sv_tile_x(LG), TY = NT / TX, AX = 2 * S, SW = TX + 4 * S, SH = TY + 4,
+                       SN = SW * SH, NI = (SN + NT - 1) / NT;
+};
+template <bool FIRST, int LG, bool P128, bool SHARED>     // P128: sigmaN = 128
 __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
+  using Wn = SvWin<LG>;
+  constexpr int S = Wn::S, TX = Wn::TX, TY = Wn::TY, AX = Wn::AX, SW = Wn::SW, SN = Wn::SN, NI = Wn::NI;
+  __shared__ float4 sC[SN], sN[SN];
+  __shared__ float sL[SN];
   auto quot = [](float x, float d, float rd) { return SHARED ? sv_div(x, d, rd) : x / d; };
-  const int2 p = pixel_of(a.W, a.H);
-  if (p.x >= a.W || p.y >= a.H) return;
-  const float4 nd = ld(a.nd, p.x, p.y);
+  const int tid = (int)threadIdx.x;
+  // workgroup -> (region, row phase)
+  const int T = a.n_items, q = T >> 3, r = T & 7, xcd = (int)blockIdx.x & 7, n = (int)blockIdx.x >> 3;
+  const int L = xcd * q + (xcd < r ? xcd : r) + n;
+  const int region = L >> LG, ry = L & (S - 1);
+  const int regions_x = (a.W + TX - 1) / TX;
+  const int x0 = (region % regions_x) * TX, y0 = (region / regions_x) * (TY * S) + ry;
+  // window texel (wx, wy) is image texel (x0 - 2s + wx, y0 + s (wy - 2))
+  uint2 cr[NI], nr[NI];
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    const int i = mn(tid + k * NT, SN - 1);
+    const int qx = clampi(x0 - AX + i % SW, 0, a.W - 1), qy = clampi(y0 + S * (i / SW - 2), 0, a.H - 1);
+    cr[k] = a.in.p[(size_t)qy * a.in.pitch + qx];
+    nr[k] = a.nd.p[(size_t)qy * a.nd.pitch + qx];
+  }
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    const int i = tid + k * NT;
+    if (i < SN) {
+      const float4 c = unpack4(cr[k]);
+      sC[i] = c;
+      sL[i] = lum(c.x, c.y, c.z);
+      sN[i] = unpack4(nr[k]);
+    }
+  }
+  __syncthreads();
+  const int lx = tid % TX, ly = tid / TX;
+  const int px = x0 + lx, py = y0 + S * ly;
+  if (px >= a.W || py >= a.H) return;
+  const int c = (ly + 2) * SW + lx + AX;
+  const float4 nd = sN[c];
   if (nd.w < 0.0f) {
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    st(a.out, p.x, p.y, zero);
-    if (FIRST) st(a.fb, p.x, p.y, zero);
+    st(a.out, px, py, zero);
+    if (FIRST) st(a.fb, px, py, zero);
     return;
   }
-  const float4 center = ld(a.in, p.x, p.y);
+  const float4 center = sC[c];
   const float k3[3] = {0.25f, 0.5f, 0.25f};
   float gv = 0.0f;
 #pragma unroll
   for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
     for (int dx = -1; dx <= 1; dx++)
-      gv += k3[dx + 1] * k3[dy + 1] * ld(a.in, clampi(p.x + dx, 0, a.W - 1), clampi(p.y + dy, 0, a.H - 1)).w;
-  const float lp = lum(center.x, center.y, center.z);
+      gv += k3[dx + 1] * k3[dy + 1] * ld(a.in, clampi(px + dx, 0, a.W - 1), clampi(py + dy, 0, a.H - 1)).w;
+  const float lp = sL[c];
   const float denomL = a.sigma_l * sqrtf(mx(gv, 0.0f)) + 1e-6f, rL = SHARED ? sv_rcp(denomL) : 0.0f;
-  // zGrad (svgf.js:157-165)
+  // zGrad (svgf.js:157-165): the pixel's own +-1 neighbours
   const float z = nd.w;
-  const float zl = ld(a.nd, clampi(p.x - 1, 0, a.W - 1), p.y).w, zr = ld(a.nd, clampi(p.x + 1, 0, a.W - 1), p.y).w;
-  const float zu = ld(a.nd, p.x, clampi(p.y - 1, 0, a.H - 1)).w, zd = ld(a.nd, p.x, clampi(p.y + 1, 0, a.H - 1)).w;
+  const float zl = ld(a.nd, clampi(px - 1, 0, a.W - 1), py).w, zr = ld(a.nd, clampi(px + 1, 0, a.W - 1), py).w;
+  const float zu = ld(a.nd, px, clampi(py - 1, 0, a.H - 1)).w, zd = ld(a.nd, px, clampi(py + 1, 0, a.H - 1)).w;
   const float gx = mn(zr > 0.0f ? fabsf(zr - z) : 1e9f, zl > 0.0f ? fabsf(zl - z) : 1e9f);
   const float gy = mn(zd > 0.0f ? fabsf(zd - z) : 1e9f, zu > 0.0f ? fabsf(zu - z) : 1e9f);
   const float zg = mn(mx(gx, gy), z);
@@ -258,39 +350,43 @@ __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
   float zdiv[9], zrcp[9];
 #pragma unroll
   for (int d2 = 1; d2 <= 8; d2++) {
-    zdiv[d2] = zden * (sqrtf((float)d2) * (float)a.step) + 1e-3f;
+    zdiv[d2] = zden * (sqrtf((float)d2) * (float)S) + 1e-3f;
     zrcp[d2] = SHARED ? sv_rcp(zdiv[d2]) : 0.0f;
   }
   float sw = h[2] * h[2];
   float s0 = center.x * sw, s1 = center.y * sw, s2 = center.z * sw;
   float sv = center.w * sw * sw;
-  const int step = a.step;
+  // interior tiles (every tap of every pixel inside the image) skip the per-tap bounds tests
+  const bool interior = x0 >= AX && y0 >= 2 * S && x0 + TX + AX <= a.W && y0 + S * (TY + 1) < a.H;
+  auto taps = [&](auto in_tile) {
+    constexpr bool CHK = !decltype(in_tile)::value;
 #pragma unroll
-  for (int dy = -2; dy <= 2; dy++) {
-    const int qy = p.y + dy * step;
-    if (qy < 0 || qy >= a.H) continue;
+    for (int dy = -2; dy <= 2; dy++) {
 #pragma unroll
-    for (int dx = -2; dx <= 2; dx++) {
-      if (dx == 0 && dy == 0) continue;
-      const int qx = p.x + dx * step;
-      if (qx < 0 || qx >= a.W) continue;
-      const float4 ndq = ld(a.nd, qx, qy);
-      if (ndq.w < 0.0f) continue;
-      const float4 cq = ld(a.in, qx, qy);
-      const int d2 = dx * dx + dy * dy;
-      const float wz = quot(fabsf(ndq.w - z), zdiv[d2], zrcp[d2]);
-      const float dn = mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z);
-      const float wn = P128 ? sv_pow128(dn) : sv_pow(dn, a.n_int, a.sigma_n);
-      const float wl = quot(fabsf(lum(cq.x, cq.y, cq.z) - lp), denomL, rL);
-      const float w = h[dx + 2] * h[dy + 2] * wn * Math<false>::expn(-wz - wl);
-      sw += w;
-      s0 += w * cq.x; s1 += w * cq.y; s2 += w * cq.z;
-      sv += w * w * cq.w;
+      for (int dx = -2; dx <= 2; dx++) {
+        if (dx == 0 && dy == 0) continue;
+        if (CHK && !inside(px + dx * S, py + dy * S, a.W, a.H)) continue;
+        const int o = c + dy * SW + dx * S;
+        const float4 ndq = sN[o];
+        if (ndq.w < 0.0f) continue;
+        const float4 cq = sC[o];
+        const int d2 = dx * dx + dy * dy;
+        const float wz = quot(fabsf(ndq.w - z), zdiv[d2], zrcp[d2]);
+        const float dn = mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z);
+        const float wn = P128 ? sv_pow128(dn) : sv_pow(dn, a.n_int, a.sigma_n);
+        const float wl = quot(fabsf(sL[o] - lp), denomL, rL);
+        const float w = h[dx + 2] * h[dy + 2] * wn * Math<false>::expn(-wz - wl);
+        sw += w;
+        s0 += w * cq.x; s1 += w * cq.y; s2 += w * cq.z;
+        sv += w * w * cq.w;
+      }
     }
-  }
+  };
+  if (interior) taps(std::true_type{});
+  else taps(std::false_type{});
   const float r0 = s0 / sw, r1 = s1 / sw, r2 = s2 / sw, r3 = sv / (sw * sw);
-  st(a.out, p.x, p.y, make_float4(r0, r1, r2, r3));
-  if (FIRST) st(a.fb, p.x, p.y, make_float4(r0, r1, r2, 0.0f));
+  st(a.out, px, py, make_float4(r0, r1, r2, r3));
+  if (FIRST) st(a.fb, px, py, make_float4(r0, r1, r2, 0.0f));
 }
 
 // tm / itm (svgf.js:237-238) and the YCoCg transform (svgf.js:232-235)
@@ -528,14 +624,24 @@ static int check_inputs(svgf_t *s, const svgf_inputs *in) {
 }
 
 // one pass at the current frame / reset state (svgf.js:359-374)
-template <bool FIRST>
-static void launch_atrous(bool p128, bool shared, dim3 g, dim3 b, hipStream_t st, const SvAtrousArgs &a) {
+template <bool FIRST, int LG>
+static void launch_atrous_lg(bool p128, bool shared, dim3 g, dim3 b, hipStream_t st, const SvAtrousArgs &a) {
   if (p128) {
-    if (shared) hipLaunchKernelGGL((k_sv_atrous<FIRST, true, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_sv_atrous<FIRST, true, false>), g, b, 0, st, a);
+    if (shared) hipLaunchKernelGGL((k_sv_atrous<FIRST, LG, true, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_sv_atrous<FIRST, LG, true, false>), g, b, 0, st, a);
   } else {
-    if (shared) hipLaunchKernelGGL((k_sv_atrous<FIRST, false, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_sv_atrous<FIRST, false, false>), g, b, 0, st, a);
+    if (shared) hipLaunchKernelGGL((k_sv_atrous<FIRST, LG, false, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_sv_atrous<FIRST, LG, false, false>), g, b, 0, st, a);
+  }
+}
+// iteration i runs at step 2^i; only iteration 0 writes the colour feedback
+static void launch_atrous(int i, bool p128, bool shared, dim3 g, dim3 b, hipStream_t st, const SvAtrousArgs &a) {
+  switch (i) {
+    case 0: launch_atrous_lg<true, 0>(p128, shared, g, b, st, a); break;
+    case 1: launch_atrous_lg<false, 1>(p128, shared, g, b, st, a); break;
+    case 2: launch_atrous_lg<false, 2>(p128, shared, g, b, st, a); break;
+    case 3: launch_atrous_lg<false, 3>(p128, shared, g, b, st, a); break;
+    default: launch_atrous_lg<false, 4>(p128, shared, g, b, st, a); break;
   }
 }
 
@@ -561,13 +667,14 @@ static hipError_t run_pass(svgf_t *s, int pass, const svgf_inputs *in, hipStream
     const int i = pass - 2, src = i & 1;
     SvAtrousArgs a;
     a.in = t[SV_AT0 + src]; a.nd = in_tex(in->normal_depth); a.out = t[SV_AT0 + (src ^ 1)]; a.fb = t[SV_HC0 + cur];
-    a.W = W; a.H = H; a.step = 1 << i; a.n_int = n_int;
+    a.W = W; a.H = H; a.lg_step = i; a.n_int = n_int;
+    const int tx = sv_tile_x(i), ty = NT / tx;
+    a.n_items = (((W + tx - 1) / tx) * ((H + (ty << i) - 1) / (ty << i))) << i;
     a.sigma_l = s->o.sigma_l; a.sigma_n = s->o.sigma_n; a.sigma_z = s->o.sigma_z;
     const bool p128 = a.n_int == 128;
     // the shared-reciprocal quotients' domain holds for sigmaL, sigmaZ in [0, 2^60] (NaN fails the test)
     const bool shared = a.sigma_l >= 0.0f && a.sigma_l <= 0x1p60f && a.sigma_z >= 0.0f && a.sigma_z <= 0x1p60f;
-    if (i == 0) launch_atrous<true>(p128, shared, g, b, st, a);
-    else launch_atrous<false>(p128, shared, g, b, st, a);
+    launch_atrous(i, p128, shared, dim3((unsigned)a.n_items), b, st, a);
   } else if (pass == 2 + s->o.iterations) {
     SvResolveArgs a;
     a.illum = t[SV_AT0 + (s->o.iterations & 1)]; a.mat = in_tex(in->material); a.emi = in_tex(in->emission);
