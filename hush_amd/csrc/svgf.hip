@@ -94,51 +94,71 @@ __device__ __forceinline__ float sv_rcp(float b) {
 }
 __device__ __forceinline__ float sv_div(float a, float b, float rb) { return Math<false>::divs(a, b, rb); }
 
-// consistent (svgf.js:32-40)
-__device__ __forceinline__ bool consistent(const SvTemporalArgs &a, int qx, int qy, float3 n, float zexp, float obj) {
-  if (!inside(qx, qy, a.W, a.H)) return false;
-  const float4 nd = ld(a.pnd, qx, qy);
+// consistent (svgf.js:32-40) of an in-image texel from its previous normal / depth and object id
+__device__ __forceinline__ bool consistent_at(float4 nd, float pobj, float3 n, float zexp, float obj) {
   if (nd.w < 0.0f) return false;
-  if (ld(a.pmv, qx, qy).z != obj) return false;
+  if (pobj != obj) return false;
   float tol = 0.05f * zexp;
   tol = tol + 0.01f;
   if (fabsf(nd.w - zexp) > tol) return false;
   if (nd.x * n.x + nd.y * n.y + nd.z * n.z < 0.9f) return false;
   return true;
 }
+__device__ __forceinline__ bool consistent(const SvTemporalArgs &a, int qx, int qy, float3 n, float zexp, float obj) {
+  if (!inside(qx, qy, a.W, a.H)) return false;
+  return consistent_at(ld(a.pnd, qx, qy), ld(a.pmv, qx, qy).z, n, zexp, obj);
+}
 
 // TEMPORAL (svgf.js:42-105): bilinear reprojection with per-tap consistency, 3x3 fallback search,
-// exponential moving average of colour and the first two luminance moments
+// exponential moving average of colour and the first two luminance moments.  The pixel's own texels and the
+// four bilinear taps' history texels (previous normal / depth and motion, colour, moments) are loaded at
+// clamped addresses before any test, so their latencies overlap; a tap that fails a test, or lies outside
+// the image, adds exact zeros (its texels are finite stored values), which leave every running sum unchanged
+// bit for bit (each starts at +0).
 __global__ __launch_bounds__(NT) void k_sv_temporal(SvTemporalArgs a) {
   const int2 p = pixel_of(a.W, a.H);
   if (p.x >= a.W || p.y >= a.H) return;
   const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  const float4 nd = ld(a.nd, p.x, p.y);
+  const uint2 rnd = a.nd.p[(size_t)p.y * a.nd.pitch + p.x], rmv = a.mv.p[(size_t)p.y * a.mv.pitch + p.x];
+  const uint2 ralb = a.mat.p[(size_t)p.y * a.mat.pitch + p.x], rrad = a.rad.p[(size_t)p.y * a.rad.pitch + p.x];
+  const float4 nd = unpack4(rnd);
   if (nd.w < 0.0f) {
     st(a.outC, p.x, p.y, zero);
     st(a.outM, p.x, p.y, zero);
     return;
   }
-  const float4 mv = ld(a.mv, p.x, p.y), alb = ld(a.mat, p.x, p.y), rad = ld(a.rad, p.x, p.y);
-  const float ir = rad.x / mx(alb.x, 1e-3f), ig = rad.y / mx(alb.y, 1e-3f), ib = rad.z / mx(alb.z, 1e-3f);
-  const float l = lum(ir, ig, ib);
+  const float4 mv = unpack4(rmv), alb = unpack4(ralb), rad = unpack4(rrad);
   const float ppx = ((float)p.x + 0.5f + mv.x) - 0.5f, ppy = ((float)p.y + 0.5f + mv.y) - 0.5f;
   const int bx = f2i(floorf(ppx)), by = f2i(floorf(ppy));
+  const bool nohist = a.reset != 0;
+  uint2 tnd[4], tmv[4], tpc[4], tpm[4];
+  if (!nohist) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int qx = clampi(addw(bx, i & 1), 0, a.W - 1), qy = clampi(addw(by, i >> 1), 0, a.H - 1);
+      tnd[i] = a.pnd.p[(size_t)qy * a.pnd.pitch + qx];
+      tmv[i] = a.pmv.p[(size_t)qy * a.pmv.pitch + qx];
+      tpc[i] = a.pcol.p[(size_t)qy * a.pcol.pitch + qx];
+      tpm[i] = a.pmom.p[(size_t)qy * a.pmom.pitch + qx];
+    }
+  }
+  const float ir = rad.x / mx(alb.x, 1e-3f), ig = rad.y / mx(alb.y, 1e-3f), ib = rad.z / mx(alb.z, 1e-3f);
+  const float l = lum(ir, ig, ib);
   const float fx = ppx - floorf(ppx), fy = ppy - floorf(ppy);
   const float3 n = make_float3(nd.x, nd.y, nd.z);
   float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
   float sumW = 0.0f, hc0 = 0.0f, hc1 = 0.0f, hc2 = 0.0f, hm0 = 0.0f, hm1 = 0.0f, hm2 = 0.0f;
-  const bool nohist = a.reset != 0;
   if (!nohist) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int qx = addw(bx, i & 1), qy = addw(by, i >> 1);
-      if (consistent(a, qx, qy, n, mv.w, mv.z) && wts[i] > 0.0f) {
-        sumW += wts[i];
-        const float4 pc = ld(a.pcol, qx, qy), pm = ld(a.pmom, qx, qy);
-        hc0 += wts[i] * pc.x; hc1 += wts[i] * pc.y; hc2 += wts[i] * pc.z;
-        hm0 += wts[i] * pm.x; hm1 += wts[i] * pm.y; hm2 += wts[i] * pm.z;
-      }
+      const bool ok = inside(qx, qy, a.W, a.H) && consistent_at(unpack4(tnd[i]), unpack4(tmv[i]).z, n, mv.w, mv.z) &&
+                      wts[i] > 0.0f;
+      const float w = ok ? wts[i] : 0.0f;
+      const float4 pc = unpack4(tpc[i]), pm = unpack4(tpm[i]);
+      sumW += w;
+      hc0 += w * pc.x; hc1 += w * pc.y; hc2 += w * pc.z;
+      hm0 += w * pm.x; hm1 += w * pm.y; hm2 += w * pm.z;
     }
     if (sumW < 0.01f) {
       const int cx = f2i(rintf(ppx)), cy = f2i(rintf(ppy));   // WGSL round(): half to even
@@ -389,14 +409,18 @@ __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
   if (FIRST) st(a.fb, px, py, make_float4(r0, r1, r2, 0.0f));
 }
 
-// tm / itm (svgf.js:237-238) and the YCoCg transform (svgf.js:232-235)
+// tm / itm (svgf.js:237-238) and the YCoCg transform (svgf.js:232-235).  The three quotients share one
+// reciprocal (sv_div): tm's divisor 1 + lum(c) is 0 or normal (near lum = -1 the sum is exact, at least
+// 2^-25 when not 0) or +-inf, itm's is at least 1e-4 or +inf; the numerators are 0, at least 2^-102 (the
+// remodulated colour is a sum of products of binary16 values, a multiple of 2^-48, and its tone-mapped
+// value a quotient of it by at most ~2^33) or non-finite, so every quotient is the IEEE one.
 __device__ __forceinline__ float3 tm3(float3 c) {
-  const float d = 1.0f + lum(c.x, c.y, c.z);
-  return make_float3(c.x / d, c.y / d, c.z / d);
+  const float d = 1.0f + lum(c.x, c.y, c.z), r = sv_rcp(d);
+  return make_float3(sv_div(c.x, d, r), sv_div(c.y, d, r), sv_div(c.z, d, r));
 }
 __device__ __forceinline__ float3 itm3(float3 c) {
-  const float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f);
-  return make_float3(c.x / d, c.y / d, c.z / d);
+  const float d = mx(1.0f - lum(c.x, c.y, c.z), 1e-4f), r = sv_rcp(d);
+  return make_float3(sv_div(c.x, d, r), sv_div(c.y, d, r), sv_div(c.z, d, r));
 }
 __device__ __forceinline__ float3 ycocg(float3 c) {
   return make_float3(c.x * 0.25f + c.y * 0.5f + c.z * 0.25f, c.x * 0.5f - c.z * 0.5f, -c.x * 0.25f + c.y * 0.5f - c.z * 0.25f);
@@ -424,12 +448,30 @@ __global__ __launch_bounds__(NT) void k_sv_resolve(SvResolveArgs a) {
     st(a.out, x, y, make_float4(out.x, out.y, out.z, 1.0f));
     return;
   }
-  for (int i = threadIdx.x; i < LW * LH; i += NT) {
-    const int ly = i / LW, lx = i - ly * LW;
-    const float3 t = tm3(remod(a, o.x + lx - 1, o.y + ly - 1));
-    const float3 c = ycocg(t);
-    s_t[ly][lx] = make_float4(t.x, t.y, t.z, 0.0f);
-    s_y[ly][lx] = make_float4(c.x, c.y, c.z, 0.0f);
+  // the motion vector, then the four history taps it addresses (clamped, loaded whether or not they are
+  // used), issued before the tile staging so that their latency overlaps it
+  const int cx = mn(x, a.W - 1), cy = mn(y, a.H - 1);
+  const float4 mv = ld(a.mv, cx, cy);
+  const float ux = ((float)x + 0.5f + mv.x) / (float)a.W, uy = ((float)y + 0.5f + mv.y) / (float)a.H;
+  const float hpx = ux * (float)a.W - 0.5f, hpy = uy * (float)a.H - 0.5f;
+  const float hbx = floorf(hpx), hby = floorf(hpy);
+  uint2 hr[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int qx = clampi(addw(f2i(hbx), j & 1), 0, a.W - 1), qy = clampi(addw(f2i(hby), j >> 1), 0, a.H - 1);
+    hr[j] = a.prev.p[(size_t)qy * a.prev.pitch + qx];
+  }
+  constexpr int NI = (LW * LH + NT - 1) / NT;
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    const int i = threadIdx.x + k * NT;
+    if (i < LW * LH) {
+      const int ly = i / LW, lx = i - ly * LW;
+      const float3 t = tm3(remod(a, o.x + lx - 1, o.y + ly - 1));
+      const float3 c = ycocg(t);
+      s_t[ly][lx] = make_float4(t.x, t.y, t.z, 0.0f);
+      s_y[ly][lx] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
   }
   __syncthreads();
   if (x >= a.W || y >= a.H) return;
@@ -446,19 +488,16 @@ __global__ __launch_bounds__(NT) void k_sv_resolve(SvResolveArgs a) {
     }
   m10 /= 9.0f; m11 /= 9.0f; m12 /= 9.0f; m20 /= 9.0f; m21 /= 9.0f; m22 /= 9.0f;
   const float sd0 = sqrtf(mx(m20 - m10 * m10, 0.0f)), sd1 = sqrtf(mx(m21 - m11 * m11, 0.0f)), sd2 = sqrtf(mx(m22 - m12 * m12, 0.0f));
-  const float4 mv = ld(a.mv, x, y);
-  const float ux = ((float)x + 0.5f + mv.x) / (float)a.W, uy = ((float)y + 0.5f + mv.y) / (float)a.H;
   float3 res = cur;
   if (ux >= 0.0f && uy >= 0.0f && ux <= 1.0f && uy <= 1.0f && !a.reset) {
     // linear sampler, clamp to edge
-    const float px = ux * (float)a.W - 0.5f, py = uy * (float)a.H - 0.5f;
-    const float bx = floorf(px), by = floorf(py), fx = px - bx, fy = py - by;
+    const float fx = hpx - hbx, fy = hpy - hby;
     float h0 = 0.0f, h1 = 0.0f, h2 = 0.0f;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int ox = j & 1, oy = j >> 1;
       const float wb = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy);
-      const float4 t = ld(a.prev, clampi(addw(f2i(bx), ox), 0, a.W - 1), clampi(addw(f2i(by), oy), 0, a.H - 1));
+      const float4 t = unpack4(hr[j]);
       h0 += wb * t.x; h1 += wb * t.y; h2 += wb * t.z;
     }
     float3 hh = ycocg(tm3(make_float3(h0, h1, h2)));
