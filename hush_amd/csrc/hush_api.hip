@@ -570,7 +570,7 @@ extern "C" int32_t hush_debug_wide_gathers(hush_t *h, int32_t enabled) {
 }
 
 extern "C" int32_t hush_debug_math_selftest(int32_t which, uint64_t *mismatches, uint32_t *first) {
-  if (which < 0 || which > 6 || !mismatches || !first)
+  if (which < 0 || which > 7 || !mismatches || !first)
     return fail(nullptr, HUSH_ERR_ARGUMENT, "hush_debug_math_selftest: bad argument");
   struct Counters { unsigned long long bad; uint32_t first, pad; } host = {0ull, 0xffffffffu, 0u};
   Counters *dev = nullptr;

@@ -217,6 +217,20 @@ template <> struct Math<false> {
     return x < __int_as_float(0xc2ce8ed0) ? 0.0f : r;        // -103.27893 = ln(2^-149)
   }
   static __device__ __forceinline__ float expm(float x) { return exp_core(x); }
+  // e^x for finite x <= 0 as a factor of a product whose other factor is at most 1/2 (SVGF's edge-stopping
+  // weights: kernel weight x normal term <= 3/32): x clamped at -104 in place of expn's underflow test.  For
+  // x >= ln(2^-149) = -103.28 it is expf(x) bit for bit; below, it is 0 or 2^-149 where expf is 0, and either
+  // times a factor of at most 1/2 rounds to 0 (2^-150 ties to even), so the product is expf's exactly.
+  static __device__ __forceinline__ float expw(float x) {
+    const float L2E = __int_as_float(0x3fb8aa3b), L2E_LO = __int_as_float(0x32a5705f);
+    const float xc = x > -104.0f ? x : -104.0f;
+    const float ph = xc * L2E;
+    float pl = __builtin_fmaf(xc, L2E, -ph);
+    const float e = __builtin_rintf(ph);              // in [-151, 0]: the conversion below is exact
+    pl = __builtin_fmaf(xc, L2E_LO, pl);
+    const float t = (ph - e) + pl;
+    return __builtin_ldexpf(__builtin_amdgcn_exp2f(t), (int)e);
+  }
   // ln x for every x except a subnormal: the hardware log2, times ln 2 carried as a two-float constant,
   // infinities and NaN passed through (logf without its rescaling of subnormal arguments).  log1p_'s
   // 1 + x is never subnormal: near -1 the sum is exact, so it is 0 or at least 2^-24 in magnitude.
@@ -238,6 +252,7 @@ template <> struct Math<true> {
   static __device__ __forceinline__ float rcpn(float b) { return __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float expn(float x) { return exp(x); }
   static __device__ __forceinline__ float expm(float x) { return exp(x); }
+  static __device__ __forceinline__ float expw(float x) { return exp(x); }
   // e^x = 2^t * 2^lo with t = fl(x log2 e) and lo its rounding error (x log2 e carried to ~48 bits)
   static __device__ __forceinline__ float exp(float x) {
     const float L2E = 1.44269502e+00f, L2E_LO = 1.92596299e-08f;

@@ -1872,6 +1872,17 @@ __global__ __launch_bounds__(NT) void k_math_selftest(int which, uint32_t base, 
   } else if (which == 4) {
     if ((u & 0x7fffffffu) >= 1u && (u & 0x7fffffffu) < 0x00800000u) return;   // every input but a subnormal
     r = Math<false>::logn(x); ref = logf(x);
+  } else if (which == 7) {
+    // expw over every finite x <= 0: expf's bits from ln(2^-149) up; below, a value that any factor of at
+    // most 1/2 takes to expf's 0
+    if (!(x <= 0.0f) || x == -__builtin_inff()) return;
+    const float r = Math<false>::expw(x), rf = expf(x);
+    const bool ok = x >= __int_as_float(0xc2ce8ed0) ? __float_as_uint(r) == __float_as_uint(rf)
+                                                   : (rf == 0.0f && r >= 0.0f && r * 0.5f == 0.0f);
+    if (ok) return;
+    atomicAdd(bad, 1ull);
+    atomicMin(first, u);
+    return;
   } else if (which == 6) {
     // the stored activations (binary16, truncated) of sigmoid against 1 / (1 + expf(-x)), every x
     const float s = sigmoid<false>(x), sr = 1.0f / (1.0f + expf(-x));

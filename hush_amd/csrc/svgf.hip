@@ -395,7 +395,9 @@ __global__ __launch_bounds__(NT) void k_sv_atrous(SvAtrousArgs a) {
         const float dn = mx(0.0f, ndq.x * nd.x + ndq.y * nd.y + ndq.z * nd.z);
         const float wn = P128 ? sv_pow128(dn) : sv_pow(dn, a.n_int, a.sigma_n);
         const float wl = quot(fabsf(sL[o] - lp), denomL, rL);
-        const float w = h[dx + 2] * h[dy + 2] * wn * Math<false>::expn(-wz - wl);
+        // (P128 and SHARED: the exponent is finite and <= 0 and h h wn <= 3/32, so expw's product is expn's)
+        const float x = -wz - wl;
+        const float w = h[dx + 2] * h[dy + 2] * wn * (P128 && SHARED ? Math<false>::expw(x) : Math<false>::expn(x));
         sw += w;
         s0 += w * cq.x; s1 += w * cq.y; s2 += w * cq.z;
         sv += w * w * cq.w;

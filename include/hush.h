@@ -184,7 +184,9 @@ int32_t hush_debug_wide_gathers(hush_t *h, int32_t enabled);
    shared-reciprocal division x / d for every x of its domain (0, +-inf, NaN, |x| >= 2^-102) over 24 divisors
    (+-0, +-inf, NaN and normal ones; finite pairs with a normal quotient below 2^127), counted per pair;
    6 the MLP's sigmoid over every x, compared as the three activations it feeds are stored (binary16,
-   truncated: 1/32 + 31/32 s, 0.5 + 7.5 s and s) against 1 / (1 + expf(-x)).
+   truncated: 1/32 + 31/32 s, 0.5 + 7.5 s and s) against 1 / (1 + expf(-x)); 7 SVGF's a-trous weight
+   exponential over every finite x <= 0 against expf: the same bits for x >= -103.27893, and below, a value
+   that halves to 0 where expf is 0.
    *mismatches = number of inputs whose results differ in any bit (any NaN equals any NaN), *first = the
    smallest such input's bits (0xffffffff when there is none).  Synchronous; allocates 16 bytes of device
    memory for the counters. */

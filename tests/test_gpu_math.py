@@ -10,7 +10,9 @@ logarithms of 1 + x (zero or at least 2^-24 in magnitude near x = -1, never subn
 division (Math<false>::divs) against IEEE division for every numerator of its domain over 24 divisors: the
 special ones (+-0, +-inf, NaN), the normal extremes and hard significands.  The reduced sigmoid of the MLP's
 first four outputs (hush_device.h: sigmoid<false>) through the activations as stored (1/32 + 31/32 s,
-0.5 + 7.5 s and s itself, truncated to binary16) against 1 / (1 + expf(-x)) for every binary32 x."""
+0.5 + 7.5 s and s itself, truncated to binary16) against 1 / (1 + expf(-x)) for every binary32 x.  SVGF's a-trous weight exponential
+(Math<false>::expw) against expf over every finite x <= 0: the same bits from ln(2^-149) up, and below, a
+value that any factor of at most 1/2 (the kernel weight times the normal term) takes to expf's 0."""
 import ctypes
 
 import pytest
@@ -24,7 +26,8 @@ pytestmark = pytest.mark.gpu
                                         (2, "exp, -103.28 <= x <= 88.72"), (3, "rcp, normal |x| < 2^126, NaN"),
                                         (4, "log, all but subnormals"),
                                         (5, "shared-reciprocal division, every numerator x 24 divisors"),
-                                        (6, "sigmoid's stored activations, every x")])
+                                        (6, "sigmoid's stored activations, every x"),
+                                        (7, "exp as a weight factor, finite x <= 0")])
 def test_reduced_range_math_matches_toolchain(which, name):
     L = _lib.load()
     bad, first = ctypes.c_uint64(0), ctypes.c_uint32(0)
