@@ -583,7 +583,8 @@ def main():
     if not args.no_extras:
         extras["mlp_stage_1080p"] = mlp_stage(hs, hip, W, H)
         print(f"[bench] MLP stage: {extras['mlp_stage_1080p']}", file=sys.stderr, flush=True)
-        # the whole frame with the FP16-MFMA MLP, and with the FAST math policy (the other policy than --math)
+        # the whole frame with the FP16-MFMA MLP, with the f16 MLP, and with the FAST math policy (the other
+        # policy than --math)
         def frame_ms(opts):
             hm = hush_amd.Hush(0, dict({"width": W, "height": H}, **opts))
             pos[0] = 0
@@ -600,6 +601,8 @@ def main():
             return round(e0.elapsed_ms(e1) / args.steps, 5)
 
         extras["fp16_mfma_ms_per_frame"] = frame_ms({"precision": "fp16-mfma", "ieee_math": args.math == "ieee"})
+        # the reference's f16 option (binary16 arithmetic in the MLP, hush.js:690)
+        extras["f16_ms_per_frame"] = frame_ms({"precision": "f16", "ieee_math": args.math == "ieee"})
         other = "fast" if args.math == "ieee" else "ieee"
         extras[f"{other}_math_ms_per_frame"] = frame_ms({"precision": args.precision, "ieee_math": other == "ieee"})
         extras[f"{other}_math_fp16_mfma_ms_per_frame"] = frame_ms({"precision": "fp16-mfma", "ieee_math": other == "ieee"})

@@ -22,6 +22,9 @@ struct MfmaPack {
   uint32_t xclamp;         // binary16 pair (448 sx[0], 448 sx[0]), capped at 65504: the layer-1 input clamp
   float qc[3];             // sx[L] sw[L] / sx[L+1] (L < 2), sx[2] sw[2]: E4M3 sums -> next E4M3 input / output
   float qb1[16], qb2[16];  // b1 / sx[1], b2 / sx[2]
+  // binary16 pairs of the shader-f16 form (units 2p, 2p + 1 of one input, for packed half arithmetic):
+  // W1 [8][20], b1 [8], W2 [8][16], b2 [8], W3 [6][16], b3 [6]
+  uint32_t hp[406];
 };
 
 constexpr int TEMPORAL_AUX = 20;

@@ -728,6 +728,50 @@ __device__ __forceinline__ void mlp_f16(const float *__restrict__ w, int h, cons
   }
 }
 
+// mlp_f16 at hidden width 16 on packed binary16 arithmetic: two units per instruction (v_pk_mul_f16,
+// v_pk_add_f16), each half its own product and sum rounded to nearest even in the same order as mlp_f16's,
+// so the same bits; the weights as binary16 pairs (MfmaPack::hp)
+typedef _Float16 half2x __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void mlp_f16_16(const MfmaPack *__restrict__ P, const float *f, float *o) {
+  const uint32_t *__restrict__ hp = P->hp;
+  auto hw = [&](int k) { return __builtin_bit_cast(half2x, hp[k]); };
+  _Float16 h1[16], h2[16];
+#pragma unroll
+  for (int p = 0; p < 8; p++) {
+    half2x acc = hw(160 + p);
+#pragma unroll
+    for (int i = 0; i < 20; i++) {
+      const _Float16 x = (_Float16)f[i];
+      const half2x prod = half2x{x, x} * hw(p * 20 + i);
+      acc = acc + prod;
+    }
+    h1[2 * p] = acc.x > (_Float16)0 ? acc.x : (_Float16)0;
+    h1[2 * p + 1] = acc.y > (_Float16)0 ? acc.y : (_Float16)0;
+  }
+#pragma unroll
+  for (int p = 0; p < 8; p++) {
+    half2x acc = hw(296 + p);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const half2x prod = half2x{h1[i], h1[i]} * hw(168 + p * 16 + i);
+      acc = acc + prod;
+    }
+    h2[2 * p] = acc.x > (_Float16)0 ? acc.x : (_Float16)0;
+    h2[2 * p + 1] = acc.y > (_Float16)0 ? acc.y : (_Float16)0;
+  }
+#pragma unroll
+  for (int p = 0; p < 6; p++) {
+    half2x acc = hw(400 + p);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const half2x prod = half2x{h2[i], h2[i]} * hw(304 + p * 16 + i);
+      acc = acc + prod;
+    }
+    o[2 * p] = (float)acc.x;
+    o[2 * p + 1] = (float)acc.y;
+  }
+}
+
 // activations (hush.js:357-366) and the four half-res outputs of one texel
 template <bool FAST>
 __device__ __forceinline__ void infer_outputs(const InferArgs &a, const float4 *sA, const float4 *sG, int tx, int ty,
@@ -791,6 +835,7 @@ __device__ __forceinline__ void infer_tile(const InferArgs &a, int n, float4 *ld
   unpack_features(ldu32(a.F0, X, Y), ldu32(a.F1, X, Y), ldu32(a.F2, X, Y), f);
   float o[12];
   if (MODE == 0) mlp_fp32<HID>(a.w, a.hidden, f, o);
+  else if (HID == 16) mlp_f16_16((const MfmaPack *)a.wq, f, o);
   else mlp_f16(a.w, a.hidden, f, o);
   infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
 }
@@ -851,6 +896,16 @@ __global__ __launch_bounds__(64) void k_pack_mlp(const float *__restrict__ w, Mf
   // fp8: layer 1 takes k = 8kb .. 8kb+7 as the f16 form does; layers 2 and 3 put hidden unit 4kb+j at
   // k = 8kb+j (j < 4) and zero weights at 8kb+4 .. 8kb+7, so each lane's B fragment is its own D fragment of
   // the layer before, padded with zeros
+  for (int k = l; k < 406; k += 64) {
+    float u, v;
+    if (k < 160) { u = W1[(k / 20) * 40 + k % 20]; v = W1[(k / 20) * 40 + 20 + k % 20]; }
+    else if (k < 168) { u = B1[2 * (k - 160)]; v = B1[2 * (k - 160) + 1]; }
+    else if (k < 296) { const int q = k - 168; u = W2[(q / 16) * 32 + q % 16]; v = W2[(q / 16) * 32 + 16 + q % 16]; }
+    else if (k < 304) { u = B2[2 * (k - 296)]; v = B2[2 * (k - 296) + 1]; }
+    else if (k < 400) { const int q = k - 304; u = W3[(q / 16) * 32 + q % 16]; v = W3[(q / 16) * 32 + 16 + q % 16]; }
+    else { u = B3[2 * (k - 400)]; v = B3[2 * (k - 400) + 1]; }
+    p->hp[k] = pack_h2(u, v);                      // binary16 weights widened to fp32: exact
+  }
   const float r1 = 1.0f / sw1, r2 = 1.0f / sw2, r3 = 1.0f / sw3;     // powers of two: exact
   float q[3][8];
 #pragma unroll
@@ -1120,7 +1175,7 @@ __global__ __launch_bounds__(NT) void k_mlp(const uint32_t *__restrict__ feat, f
       f[4 * j] = a.x; f[4 * j + 1] = a.y; f[4 * j + 2] = b.x; f[4 * j + 3] = b.y;
     }
     if (MODE == MLP_FP32) mlp_fp32<16>(w, 16, f, o);
-    else mlp_f16(w, 16, f, o);
+    else mlp_f16_16((const MfmaPack *)wq, f, o);
     float4 *dst = (float4 *)(out + r * 12);
     dst[0] = make_float4(o[0], o[1], o[2], o[3]);
     dst[1] = make_float4(o[4], o[5], o[6], o[7]);
@@ -1689,6 +1744,7 @@ static void launch_pass_kernel(K kernel, dim3 grid, hipStream_t s, Args... args)
 #define K_INFER16(F) k_infer<16, 0, F>
 #define K_INFER(F) k_infer<0, 0, F>
 #define K_INFER_F16(F) k_infer<0, 1, F>
+#define K_INFER16_F16(F) k_infer<16, 1, F>
 #define K_INFER_H(F) k_infer_mfma<MLP_FP16_MFMA, F>
 #define K_INFER_B(F) k_infer_mfma<MLP_BF16_MFMA, F>
 #define K_INFER_Q(F) k_infer_mfma<MLP_FP8_MFMA, F>
@@ -1735,7 +1791,8 @@ hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s) 
     if (a.hidden == 16) HUSH_LAUNCH(K_INFER16, g, a);
     else HUSH_LAUNCH(K_INFER, g, a);
   } else if (mode == MLP_F16) {
-    HUSH_LAUNCH(K_INFER_F16, g, a);
+    if (a.hidden == 16 && a.wq) HUSH_LAUNCH(K_INFER16_F16, g, a);
+    else HUSH_LAUNCH(K_INFER_F16, g, a);
   } else if (a.hidden != 16 || !a.wq) {
     return hipErrorInvalidValue;
   } else if (mode == MLP_FP16_MFMA) {
@@ -1800,6 +1857,7 @@ hipError_t launch_coarse2(const AtrousArgs &a, bool fast, hipStream_t s) {
 #define K_C1_INFER16(F) k_coarse1_infer<16, 0, F>
 #define K_C1_INFER(F) k_coarse1_infer<0, 0, F>
 #define K_C1_INFER_F16(F) k_coarse1_infer<0, 1, F>
+#define K_C1_INFER16_F16(F) k_coarse1_infer<16, 1, F>
 #define K_C1_INFER_H(F) k_coarse1_infer_mfma<MLP_FP16_MFMA, F>
 #define K_C1_INFER_B(F) k_coarse1_infer_mfma<MLP_BF16_MFMA, F>
 #define K_C1_INFER_Q(F) k_coarse1_infer_mfma<MLP_FP8_MFMA, F>
@@ -1812,7 +1870,8 @@ hipError_t launch_coarse1_infer(const Coarse1Args &c, const InferArgs &a, int mo
     if (a.hidden == 16) HUSH_LAUNCH_N(K_C1_INFER16, g, c, a, nA, nApad);
     else HUSH_LAUNCH_N(K_C1_INFER, g, c, a, nA, nApad);
   } else if (mode == MLP_F16) {
-    HUSH_LAUNCH_N(K_C1_INFER_F16, g, c, a, nA, nApad);
+    if (a.hidden == 16 && a.wq) HUSH_LAUNCH_N(K_C1_INFER16_F16, g, c, a, nA, nApad);
+    else HUSH_LAUNCH_N(K_C1_INFER_F16, g, c, a, nA, nApad);
   } else if (a.hidden != 16 || !a.wq) {
     return hipErrorInvalidValue;
   } else if (mode == MLP_FP16_MFMA) {
