@@ -209,6 +209,37 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
     hush_destroy(h);
     return HUSH_ERR_DEVICE;
   }
+  // The FP32 forms run in registers at 8, 16 or 32 units: a blob of another width up to 32 is zero-padded to
+  // the next of them.  A padded unit's sums are +0 (zero weights and bias), so its ReLU output is 0, and a
+  // padded input adds +0 products after a real unit's own terms; that can only turn a -0 sum into +0, which
+  // the ReLU maps to the same 0 and the activations (sigmoid, exp) map to the same value, so every output
+  // is the unpadded MLP's bit for bit.
+  h->hpad = hid <= 8 ? 8 : hid <= 16 ? 16 : hid <= 32 ? 32 : 0;
+  if (h->hpad == hid) {
+    h->d_wpad = h->d_w;
+  } else if (h->hpad) {
+    const int H = h->hpad, n = hid;
+    std::vector<float> wp((size_t)H * 20 + H + (size_t)H * H + H + 12 * H + 12, 0.0f);
+    const float *W1 = wf.data(), *B1 = W1 + n * 20, *W2 = B1 + n, *B2 = W2 + n * n, *W3 = B2 + n, *B3 = W3 + 12 * n;
+    float *P1 = wp.data(), *Q1 = P1 + H * 20, *P2 = Q1 + H, *Q2 = P2 + H * H, *P3 = Q2 + H, *Q3 = P3 + 12 * H;
+    for (int j = 0; j < n; j++) {
+      for (int i = 0; i < 20; i++) P1[j * 20 + i] = W1[j * 20 + i];
+      Q1[j] = B1[j];
+      for (int i = 0; i < n; i++) P2[j * H + i] = W2[j * n + i];
+      Q2[j] = B2[j];
+    }
+    for (int j = 0; j < 12; j++) {
+      for (int i = 0; i < n; i++) P3[j * H + i] = W3[j * n + i];
+      Q3[j] = B3[j];
+    }
+    e = hipMalloc((void **)&h->d_wpad, wp.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(h->d_wpad, wp.data(), wp.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hip_fail(nullptr, e, "padded weights upload");
+      hush_destroy(h);
+      return HUSH_ERR_DEVICE;
+    }
+  }
   if (hid == 16) {   // fragment-order weights of the MFMA forms
     h->fp8_sw[0] = fp8_scale(wf.data(), 320); h->fp8_sw[1] = fp8_scale(wf.data() + 336, 256);
     h->fp8_sw[2] = fp8_scale(wf.data() + 608, 192);
@@ -243,6 +274,7 @@ extern "C" void hush_destroy(hush_t *h) {
   (void)hipSetDevice(h->device);
   (void)hipDeviceSynchronize();
   free_surfaces(h);
+  if (h->d_wpad && h->d_wpad != h->d_w) (void)hipFree(h->d_wpad);
   if (h->d_w) (void)hipFree(h->d_w);
   if (h->d_wq) (void)hipFree(h->d_wq);
   if (h->d_wscale) (void)hipFree(h->d_wscale);
@@ -368,6 +400,7 @@ static InferArgs infer_args(const hush_t *h) {
   a.F0 = h->F[0]; a.F1 = h->F[1]; a.F2 = h->F[2]; a.GH = h->gh;
   a.out0 = t[S_PAR0_0 + cur]; a.out1 = t[S_PAR1]; a.out2 = t[S_PAR2]; a.outMid = t[S_BH0];
   a.W2 = h->W2; a.H2 = h->H2; a.hidden = h->hidden; a.w = h->d_w; a.wq = h->d_wq; a.wscale = h->d_wscale;
+  a.hpad = h->hpad; a.wp = h->d_wpad;
   return a;
 }
 static AtrousArgs mid_args(const hush_t *h, int pass) {      // pass 2 (mid2) or 3 (mid3)

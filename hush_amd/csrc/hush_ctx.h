@@ -23,6 +23,8 @@ struct hush_ctx {
   int hidden = 0;
   int precision = HUSH_MLP_FP32;
   float *d_w = nullptr;
+  float *d_wpad = nullptr;       // the weights zero-padded to hpad units (d_w itself when hpad == hidden)
+  int hpad = 0;
   void *d_wq = nullptr;
   float *d_wscale = nullptr;
   float fp8_sw[3] = {1.0f, 1.0f, 1.0f};   // fp8 form: per-layer weight scales (from the weights)

@@ -834,7 +834,7 @@ __device__ __forceinline__ void infer_tile(const InferArgs &a, int n, float4 *ld
   float f[20];
   unpack_features(ldu32(a.F0, X, Y), ldu32(a.F1, X, Y), ldu32(a.F2, X, Y), f);
   float o[12];
-  if (MODE == 0) mlp_fp32<HID>(a.w, a.hidden, f, o);
+  if (MODE == 0) mlp_fp32<HID>(HID > 0 ? a.wp : a.w, a.hidden, f, o);
   else if (HID == 16) mlp_f16_16((const MfmaPack *)a.wq, f, o);
   else mlp_f16(a.w, a.hidden, f, o);
   infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
@@ -1742,6 +1742,8 @@ static void launch_pass_kernel(K kernel, dim3 grid, hipStream_t s, Args... args)
   } while (0)
 
 #define K_INFER16(F) k_infer<16, 0, F>
+#define K_INFER8(F) k_infer<8, 0, F>
+#define K_INFER32(F) k_infer<32, 0, F>
 #define K_INFER(F) k_infer<0, 0, F>
 #define K_INFER_F16(F) k_infer<0, 1, F>
 #define K_INFER16_F16(F) k_infer<16, 1, F>
@@ -1788,7 +1790,9 @@ hipError_t launch_temporal(const TemporalArgs &a, bool, hipStream_t s) {
 hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s) {
   dim3 g = grid_for(a.W2, a.H2);
   if (mode == MLP_FP32) {
-    if (a.hidden == 16) HUSH_LAUNCH(K_INFER16, g, a);
+    if (a.hpad == 16) HUSH_LAUNCH(K_INFER16, g, a);
+    else if (a.hpad == 8) HUSH_LAUNCH(K_INFER8, g, a);
+    else if (a.hpad == 32) HUSH_LAUNCH(K_INFER32, g, a);
     else HUSH_LAUNCH(K_INFER, g, a);
   } else if (mode == MLP_F16) {
     if (a.hidden == 16 && a.wq) HUSH_LAUNCH(K_INFER16_F16, g, a);
@@ -1855,6 +1859,8 @@ hipError_t launch_coarse2(const AtrousArgs &a, bool fast, hipStream_t s) {
     else launch_pass_kernel(KERNEL(false), GRID, s, __VA_ARGS__);                       \
   } while (0)
 #define K_C1_INFER16(F) k_coarse1_infer<16, 0, F>
+#define K_C1_INFER8(F) k_coarse1_infer<8, 0, F>
+#define K_C1_INFER32(F) k_coarse1_infer<32, 0, F>
 #define K_C1_INFER(F) k_coarse1_infer<0, 0, F>
 #define K_C1_INFER_F16(F) k_coarse1_infer<0, 1, F>
 #define K_C1_INFER16_F16(F) k_coarse1_infer<16, 1, F>
@@ -1867,7 +1873,9 @@ hipError_t launch_coarse1_infer(const Coarse1Args &c, const InferArgs &a, int mo
   const int nA = (int)grid_for(c.W4, c.H4).x, nApad = (nA + 7) & ~7;
   const dim3 g((unsigned)nApad + grid_for(a.W2, a.H2).x);
   if (mode == MLP_FP32) {
-    if (a.hidden == 16) HUSH_LAUNCH_N(K_C1_INFER16, g, c, a, nA, nApad);
+    if (a.hpad == 16) HUSH_LAUNCH_N(K_C1_INFER16, g, c, a, nA, nApad);
+    else if (a.hpad == 8) HUSH_LAUNCH_N(K_C1_INFER8, g, c, a, nA, nApad);
+    else if (a.hpad == 32) HUSH_LAUNCH_N(K_C1_INFER32, g, c, a, nA, nApad);
     else HUSH_LAUNCH_N(K_C1_INFER, g, c, a, nA, nApad);
   } else if (mode == MLP_F16) {
     if (a.hidden == 16 && a.wq) HUSH_LAUNCH_N(K_C1_INFER16_F16, g, c, a, nA, nApad);
