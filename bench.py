@@ -455,6 +455,8 @@ def main():
                     help="after generating the inputs, time only the differentiable frame (the training leg: 1080p "
                          "forward + loss + backward, and the 8 x 16 x 512^2 step) and print it as one JSON line; for "
                          "profiling the training kernels on their own")
+    ap.add_argument("--svgf-only", action="store_true",
+                    help="only the SVGF baseline over the resident sequence (ms/frame and per-pass ms), one JSON line")
     ap.add_argument("--post-only", action="store_true",
                     help="time only the display post (Post.encode of two resident seeded HDR images into a "
                          "width x height bgra8 target, --steps encodes) and print it as one JSON line")
@@ -493,6 +495,10 @@ def main():
 
     if args.train_only:
         print(json.dumps({"train_fp32": train_leg(dev, W, H), "width": W, "height": H}), flush=True)
+        return
+    if args.svgf_only:
+        leg = svgf_leg(dev, hip.Stream(), hip, W, H, min(args.steps, 64), args.warmup)
+        print(json.dumps({"svgf": leg, "width": W, "height": H}), flush=True)
         return
     hs = hush_amd.Hush(0, {"width": W, "height": H, "precision": args.precision, "ieee_math": args.math == "ieee"})
     stream = hip.Stream()
