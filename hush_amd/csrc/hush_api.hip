@@ -209,12 +209,12 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
     hush_destroy(h);
     return HUSH_ERR_DEVICE;
   }
-  // The FP32 forms run in registers at 8, 16 or 32 units: a blob of another width up to 32 is zero-padded to
-  // the next of them.  A padded unit's sums are +0 (zero weights and bias), so its ReLU output is 0, and a
+  // The FP32 forms run in registers at 8, 16, 32 or 64 units: a blob of another width up to 64 is zero-padded
+  // to the next of them.  A padded unit's sums are +0 (zero weights and bias), so its ReLU output is 0, and a
   // padded input adds +0 products after a real unit's own terms; that can only turn a -0 sum into +0, which
   // the ReLU maps to the same 0 and the activations (sigmoid, exp) map to the same value, so every output
   // is the unpadded MLP's bit for bit.
-  h->hpad = hid <= 8 ? 8 : hid <= 16 ? 16 : hid <= 32 ? 32 : 0;
+  h->hpad = hid <= 8 ? 8 : hid <= 16 ? 16 : hid <= 32 ? 32 : hid <= 64 ? 64 : 0;
   if (h->hpad == hid) {
     h->d_wpad = h->d_w;
   } else if (h->hpad) {

@@ -1744,6 +1744,7 @@ static void launch_pass_kernel(K kernel, dim3 grid, hipStream_t s, Args... args)
 #define K_INFER16(F) k_infer<16, 0, F>
 #define K_INFER8(F) k_infer<8, 0, F>
 #define K_INFER32(F) k_infer<32, 0, F>
+#define K_INFER64(F) k_infer<64, 0, F>
 #define K_INFER(F) k_infer<0, 0, F>
 #define K_INFER_F16(F) k_infer<0, 1, F>
 #define K_INFER16_F16(F) k_infer<16, 1, F>
@@ -1793,6 +1794,7 @@ hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s) 
     if (a.hpad == 16) HUSH_LAUNCH(K_INFER16, g, a);
     else if (a.hpad == 8) HUSH_LAUNCH(K_INFER8, g, a);
     else if (a.hpad == 32) HUSH_LAUNCH(K_INFER32, g, a);
+    else if (a.hpad == 64) HUSH_LAUNCH(K_INFER64, g, a);
     else HUSH_LAUNCH(K_INFER, g, a);
   } else if (mode == MLP_F16) {
     if (a.hidden == 16 && a.wq) HUSH_LAUNCH(K_INFER16_F16, g, a);
@@ -1861,6 +1863,7 @@ hipError_t launch_coarse2(const AtrousArgs &a, bool fast, hipStream_t s) {
 #define K_C1_INFER16(F) k_coarse1_infer<16, 0, F>
 #define K_C1_INFER8(F) k_coarse1_infer<8, 0, F>
 #define K_C1_INFER32(F) k_coarse1_infer<32, 0, F>
+#define K_C1_INFER64(F) k_coarse1_infer<64, 0, F>
 #define K_C1_INFER(F) k_coarse1_infer<0, 0, F>
 #define K_C1_INFER_F16(F) k_coarse1_infer<0, 1, F>
 #define K_C1_INFER16_F16(F) k_coarse1_infer<16, 1, F>
@@ -1876,6 +1879,7 @@ hipError_t launch_coarse1_infer(const Coarse1Args &c, const InferArgs &a, int mo
     if (a.hpad == 16) HUSH_LAUNCH_N(K_C1_INFER16, g, c, a, nA, nApad);
     else if (a.hpad == 8) HUSH_LAUNCH_N(K_C1_INFER8, g, c, a, nA, nApad);
     else if (a.hpad == 32) HUSH_LAUNCH_N(K_C1_INFER32, g, c, a, nA, nApad);
+    else if (a.hpad == 64) HUSH_LAUNCH_N(K_C1_INFER64, g, c, a, nA, nApad);
     else HUSH_LAUNCH_N(K_C1_INFER, g, c, a, nA, nApad);
   } else if (mode == MLP_F16) {
     if (a.hidden == 16 && a.wq) HUSH_LAUNCH_N(K_C1_INFER16_F16, g, c, a, nA, nApad);
