@@ -214,10 +214,34 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
   // padded input adds +0 products after a real unit's own terms; that can only turn a -0 sum into +0, which
   // the ReLU maps to the same 0 and the activations (sigmoid, exp) map to the same value, so every output
   // is the unpadded MLP's bit for bit.
-  h->hpad = hid <= 8 ? 8 : hid <= 16 ? 16 : hid <= 32 ? 32 : hid <= 64 ? 64 : 0;
-  if (h->hpad == hid) {
+  // Wider blobs run the chunked form (units in chunks of 32) over the weights zero-padded to a multiple of 32
+  // and stored input-major.
+  h->hpad = hid <= 8 ? 8 : hid <= 16 ? 16 : hid <= 32 ? 32 : hid <= 64 ? 64 : (hid + 31) / 32 * 32;
+  if (h->hpad == hid && hid <= 64) {
     h->d_wpad = h->d_w;
-  } else if (h->hpad) {
+  } else if (h->hpad > 64) {
+    const int H = h->hpad, n = hid;
+    std::vector<float> wp((size_t)H * 20 + H + (size_t)H * H + H + 12 * H + 12, 0.0f);
+    const float *W1 = wf.data(), *B1 = W1 + n * 20, *W2 = B1 + n, *B2 = W2 + n * n, *W3 = B2 + n, *B3 = W3 + 12 * n;
+    float *P1 = wp.data(), *Q1 = P1 + H * 20, *P2 = Q1 + H, *Q2 = P2 + H * H, *P3 = Q2 + H, *Q3 = P3 + 12 * H;
+    for (int j = 0; j < n; j++) {
+      for (int i = 0; i < 20; i++) P1[i * H + j] = W1[j * 20 + i];
+      Q1[j] = B1[j];
+      for (int i = 0; i < n; i++) P2[i * H + j] = W2[j * n + i];
+      Q2[j] = B2[j];
+    }
+    for (int j = 0; j < 12; j++) {
+      for (int i = 0; i < n; i++) P3[i * 12 + j] = W3[j * n + i];
+      Q3[j] = B3[j];
+    }
+    e = hipMalloc((void **)&h->d_wpad, wp.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(h->d_wpad, wp.data(), wp.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hip_fail(nullptr, e, "padded weights upload");
+      hush_destroy(h);
+      return HUSH_ERR_DEVICE;
+    }
+  } else {
     const int H = h->hpad, n = hid;
     std::vector<float> wp((size_t)H * 20 + H + (size_t)H * H + H + 12 * H + 12, 0.0f);
     const float *W1 = wf.data(), *B1 = W1 + n * 20, *W2 = B1 + n, *B2 = W2 + n * n, *W3 = B2 + n, *B3 = W3 + 12 * n;

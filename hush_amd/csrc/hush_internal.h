@@ -53,8 +53,8 @@ struct InferArgs {
   Tex out0, out1, out2, outMid;      // par0[cur], par1, par2, Bh[0]
   int W2, H2, hidden;
   const float *w;                    // MLP weights widened to fp32 (hush.js:744)
-  // the FP32 forms' register width (8, 16, 32 or 64; 0: the generic form over w) and the weights zero-padded
-  // to it
+  // the FP32 forms' register width (8, 16, 32 or 64; above 64 the chunked form's width, a multiple of 32) and
+  // the weights zero-padded to it (input-major for the chunked form)
   int hpad;
   const float *wp;
   const void *wq;                    // packed low-precision weights for the MFMA forms

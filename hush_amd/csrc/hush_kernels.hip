@@ -705,6 +705,48 @@ __device__ __forceinline__ void mlp_fp32(const float *__restrict__ w, int hid, c
   }
 }
 
+// The FP32 MLP for hidden widths above 64, over the weights zero-padded to H (a multiple of 32) and stored
+// input-major (W1t [20][H], b1 [H], W2t [H][H], b2 [H], W3t [H][12], b3 [12]): the units in chunks of 32
+// accumulators held in registers, the hidden activations in per-lane memory, each read once per chunk.
+// Every unit's sum runs over the same terms in the same order as mlp_fp32's (bias, then inputs ascending).
+__device__ __forceinline__ void mlp_fp32_chunked(const float *__restrict__ w, int H, const float *f, float *o) {
+  const float *W1 = w, *B1 = W1 + H * 20, *W2 = B1 + H, *B2 = W2 + H * H, *W3 = B2 + H, *B3 = W3 + 12 * H;
+  float h1[256], h2[256];
+  for (int j0 = 0; j0 < H; j0 += 32) {
+    float acc[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) acc[k] = B1[j0 + k];
+#pragma unroll
+    for (int i = 0; i < 20; i++)
+#pragma unroll
+      for (int k = 0; k < 32; k++) acc[k] = __builtin_fmaf(f[i], W1[i * H + j0 + k], acc[k]);
+#pragma unroll
+    for (int k = 0; k < 32; k++) h1[j0 + k] = mx(acc[k], 0.0f);
+  }
+  for (int j0 = 0; j0 < H; j0 += 32) {
+    float acc[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) acc[k] = B2[j0 + k];
+    for (int i = 0; i < H; i++) {
+      const float x = h1[i];
+#pragma unroll
+      for (int k = 0; k < 32; k++) acc[k] = __builtin_fmaf(x, W2[i * H + j0 + k], acc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 32; k++) h2[j0 + k] = mx(acc[k], 0.0f);
+  }
+  float acc[12];
+#pragma unroll
+  for (int j = 0; j < 12; j++) acc[j] = B3[j];
+  for (int i = 0; i < H; i++) {
+    const float x = h2[i];
+#pragma unroll
+    for (int j = 0; j < 12; j++) acc[j] = __builtin_fmaf(x, W3[i * 12 + j], acc[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 12; j++) o[j] = acc[j];
+}
+
 // shader-f16 MLP of the reference's f16 option (hush.js:690, 699-702): binary16 inputs, weights,
 // products and sums, each rounded to nearest even
 __device__ __forceinline__ void mlp_f16(const float *__restrict__ w, int h, const float *f, float *o) {
@@ -834,8 +876,9 @@ __device__ __forceinline__ void infer_tile(const InferArgs &a, int n, float4 *ld
   float f[20];
   unpack_features(ldu32(a.F0, X, Y), ldu32(a.F1, X, Y), ldu32(a.F2, X, Y), f);
   float o[12];
-  if (MODE == 0) mlp_fp32<HID>(HID > 0 ? a.wp : a.w, a.hidden, f, o);
-  else if (HID == 16) mlp_f16_16((const MfmaPack *)a.wq, f, o);
+  if constexpr (MODE == 0 && HID < 0) mlp_fp32_chunked(a.wp, a.hpad, f, o);
+  else if constexpr (MODE == 0) mlp_fp32<HID>(HID > 0 ? a.wp : a.w, a.hidden, f, o);
+  else if constexpr (HID == 16) mlp_f16_16((const MfmaPack *)a.wq, f, o);
   else mlp_f16(a.w, a.hidden, f, o);
   infer_outputs<FAST>(a, sA, sG, tx, ty, X, Y, o);
 }
@@ -1745,7 +1788,7 @@ static void launch_pass_kernel(K kernel, dim3 grid, hipStream_t s, Args... args)
 #define K_INFER8(F) k_infer<8, 0, F>
 #define K_INFER32(F) k_infer<32, 0, F>
 #define K_INFER64(F) k_infer<64, 0, F>
-#define K_INFER(F) k_infer<0, 0, F>
+#define K_INFERC(F) k_infer<-1, 0, F>
 #define K_INFER_F16(F) k_infer<0, 1, F>
 #define K_INFER16_F16(F) k_infer<16, 1, F>
 #define K_INFER_H(F) k_infer_mfma<MLP_FP16_MFMA, F>
@@ -1795,7 +1838,7 @@ hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s) 
     else if (a.hpad == 8) HUSH_LAUNCH(K_INFER8, g, a);
     else if (a.hpad == 32) HUSH_LAUNCH(K_INFER32, g, a);
     else if (a.hpad == 64) HUSH_LAUNCH(K_INFER64, g, a);
-    else HUSH_LAUNCH(K_INFER, g, a);
+    else HUSH_LAUNCH(K_INFERC, g, a);
   } else if (mode == MLP_F16) {
     if (a.hidden == 16 && a.wq) HUSH_LAUNCH(K_INFER16_F16, g, a);
     else HUSH_LAUNCH(K_INFER_F16, g, a);
@@ -1864,7 +1907,7 @@ hipError_t launch_coarse2(const AtrousArgs &a, bool fast, hipStream_t s) {
 #define K_C1_INFER8(F) k_coarse1_infer<8, 0, F>
 #define K_C1_INFER32(F) k_coarse1_infer<32, 0, F>
 #define K_C1_INFER64(F) k_coarse1_infer<64, 0, F>
-#define K_C1_INFER(F) k_coarse1_infer<0, 0, F>
+#define K_C1_INFERC(F) k_coarse1_infer<-1, 0, F>
 #define K_C1_INFER_F16(F) k_coarse1_infer<0, 1, F>
 #define K_C1_INFER16_F16(F) k_coarse1_infer<16, 1, F>
 #define K_C1_INFER_H(F) k_coarse1_infer_mfma<MLP_FP16_MFMA, F>
@@ -1880,7 +1923,7 @@ hipError_t launch_coarse1_infer(const Coarse1Args &c, const InferArgs &a, int mo
     else if (a.hpad == 8) HUSH_LAUNCH_N(K_C1_INFER8, g, c, a, nA, nApad);
     else if (a.hpad == 32) HUSH_LAUNCH_N(K_C1_INFER32, g, c, a, nA, nApad);
     else if (a.hpad == 64) HUSH_LAUNCH_N(K_C1_INFER64, g, c, a, nA, nApad);
-    else HUSH_LAUNCH_N(K_C1_INFER, g, c, a, nA, nApad);
+    else HUSH_LAUNCH_N(K_C1_INFERC, g, c, a, nA, nApad);
   } else if (mode == MLP_F16) {
     if (a.hidden == 16 && a.wq) HUSH_LAUNCH_N(K_C1_INFER16_F16, g, c, a, nA, nApad);
     else HUSH_LAUNCH_N(K_C1_INFER_F16, g, c, a, nA, nApad);

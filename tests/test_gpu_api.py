@@ -2,8 +2,8 @@
 against the FP32-mirror oracle run through the same calls: resize in the middle of a sequence (hush.js:
 764-794: reallocates and resets; same size is a no-op), reset (hush.js:801), an exposure change between
 frames, sizes down to 1x1 and sizes one texel past a tile edge, MLP hidden widths other than 16
-(hiddenOf, hush.js:22-25: widths up to 64 zero-padded to a register form, wider ones on the generic-width
-kernel), the shader-f16 MLP variant (hush.js:348-356 with f16: true), and the reference's input errors
+(hiddenOf, hush.js:22-25: widths up to 64 zero-padded to a register form, wider ones in chunks of 32
+units), the shader-f16 MLP variant (hush.js:348-356 with f16: true), and the reference's input errors
 (hush.js:813-822).
 
 Tolerance (as the end-to-end parity test): HDR output within 1e-3 relative (max(|ref|, 1e-3)) for
@@ -184,7 +184,7 @@ def test_small_and_ragged_sizes_match_oracle(w, h):
 @pytest.mark.parametrize("hidden", [8, 12, 32, 40, 72])
 def test_other_hidden_widths_match_oracle(hidden):
     """widths 8 and 32 run the register form at their own width, 12 and 40 the width-16 and width-64 forms over
-    zero-padded weights, 72 the generic form (hidden activations in scratch memory)"""
+    zero-padded weights, 72 the chunked form over weights padded to 96"""
     from hush_amd.train import random_weights, to_f16_blob
     wts = to_f16_blob(random_weights(40 + hidden, hidden))
     assert hush_amd.hiddenOf(wts.size) == hidden
