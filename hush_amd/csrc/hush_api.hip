@@ -264,12 +264,47 @@ extern "C" int32_t hush_create(int32_t device, int32_t width, int32_t height, fl
       return HUSH_ERR_DEVICE;
     }
   }
-  if (hid == 16) {   // fragment-order weights of the MFMA forms
-    h->fp8_sw[0] = fp8_scale(wf.data(), 320); h->fp8_sw[1] = fp8_scale(wf.data() + 336, 256);
-    h->fp8_sw[2] = fp8_scale(wf.data() + 608, 192);
-    fp8_default_input_scales(wf.data(), h->fp8_sx);
+  // fragment-order weights of the MFMA forms (width 16), and the binary16 pairs of the f16 option's packed
+  // form, which also runs narrower blobs over their weights zero-padded to 16 (the MFMA forms refuse them)
+  if (hid <= 16) {
+    float *w16 = h->d_w;
+    if (hid == 16) {
+      h->fp8_sw[0] = fp8_scale(wf.data(), 320); h->fp8_sw[1] = fp8_scale(wf.data() + 336, 256);
+      h->fp8_sw[2] = fp8_scale(wf.data() + 608, 192);
+      fp8_default_input_scales(wf.data(), h->fp8_sx);
+    } else if (h->hpad == 16) {
+      w16 = h->d_wpad;
+    } else {                                      // padded to 8 for the FP32 forms: a width-16 copy to pack
+      std::vector<float> wp(812, 0.0f);
+      const int n = hid;
+      const float *W1 = wf.data(), *B1 = W1 + n * 20, *W2 = B1 + n, *B2 = W2 + n * n, *W3 = B2 + n, *B3 = W3 + 12 * n;
+      float *P1 = wp.data(), *Q1 = P1 + 320, *P2 = Q1 + 16, *Q2 = P2 + 256, *P3 = Q2 + 16, *Q3 = P3 + 192;
+      for (int j = 0; j < n; j++) {
+        for (int i = 0; i < 20; i++) P1[j * 20 + i] = W1[j * 20 + i];
+        Q1[j] = B1[j];
+        for (int i = 0; i < n; i++) P2[j * 16 + i] = W2[j * n + i];
+        Q2[j] = B2[j];
+      }
+      for (int j = 0; j < 12; j++) {
+        for (int i = 0; i < n; i++) P3[j * 16 + i] = W3[j * n + i];
+        Q3[j] = B3[j];
+      }
+      w16 = nullptr;
+      e = hipMalloc((void **)&w16, wp.size() * sizeof(float));
+      if (e == hipSuccess) e = hipMemcpy(w16, wp.data(), wp.size() * sizeof(float), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        if (w16) (void)hipFree(w16);
+        hip_fail(nullptr, e, "padded weights upload");
+        hush_destroy(h);
+        return HUSH_ERR_DEVICE;
+      }
+    }
     e = hipMalloc(&h->d_wq, sizeof(MfmaPack));
-    if (e == hipSuccess) e = launch_pack_mlp(h->d_w, (MfmaPack *)h->d_wq, h->fp8_sw, h->fp8_sx, nullptr);
+    if (e == hipSuccess) e = launch_pack_mlp(w16, (MfmaPack *)h->d_wq, h->fp8_sw, h->fp8_sx, nullptr);
+    if (w16 != h->d_w && w16 != h->d_wpad) {
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      (void)hipFree(w16);
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
       hip_fail(nullptr, e, "MFMA weight packing");
@@ -347,7 +382,7 @@ extern "C" int32_t hush_set_ieee_math(hush_t *h, int32_t on) {
 
 extern "C" int32_t hush_set_fp8_scales(hush_t *h, const float *scales) {
   if (!h || !scales) return fail(h, HUSH_ERR_ARGUMENT, "hush_set_fp8_scales: null argument");
-  if (!h->d_wq) return fail(h, HUSH_ERR_UNSUPPORTED, "hush_set_fp8_scales: the fp8 form needs hidden width 16");
+  if (!h->d_wq || h->hidden != 16) return fail(h, HUSH_ERR_UNSUPPORTED, "hush_set_fp8_scales: the fp8 form needs hidden width 16");
   for (int i = 0; i < 3; i++)
     if (!fp8_scale_ok(scales[i]))
       return fail(h, HUSH_ERR_ARGUMENT, "hush_set_fp8_scales: scale %d (%g) is not a power of two in [2^-20, 2^20]", i,

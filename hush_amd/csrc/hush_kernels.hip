@@ -1840,7 +1840,7 @@ hipError_t launch_infer(const InferArgs &a, int mode, bool fast, hipStream_t s) 
     else if (a.hpad == 64) HUSH_LAUNCH(K_INFER64, g, a);
     else HUSH_LAUNCH(K_INFERC, g, a);
   } else if (mode == MLP_F16) {
-    if (a.hidden == 16 && a.wq) HUSH_LAUNCH(K_INFER16_F16, g, a);
+    if (a.hidden <= 16 && a.wq) HUSH_LAUNCH(K_INFER16_F16, g, a);
     else HUSH_LAUNCH(K_INFER_F16, g, a);
   } else if (a.hidden != 16 || !a.wq) {
     return hipErrorInvalidValue;
@@ -1925,7 +1925,7 @@ hipError_t launch_coarse1_infer(const Coarse1Args &c, const InferArgs &a, int mo
     else if (a.hpad == 64) HUSH_LAUNCH_N(K_C1_INFER64, g, c, a, nA, nApad);
     else HUSH_LAUNCH_N(K_C1_INFERC, g, c, a, nA, nApad);
   } else if (mode == MLP_F16) {
-    if (a.hidden == 16 && a.wq) HUSH_LAUNCH_N(K_C1_INFER16_F16, g, c, a, nA, nApad);
+    if (a.hidden <= 16 && a.wq) HUSH_LAUNCH_N(K_C1_INFER16_F16, g, c, a, nA, nApad);
     else HUSH_LAUNCH_N(K_C1_INFER_F16, g, c, a, nA, nApad);
   } else if (a.hidden != 16 || !a.wq) {
     return hipErrorInvalidValue;

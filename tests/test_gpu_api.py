@@ -210,6 +210,22 @@ def test_f16_mlp_variant_matches_oracle():
     _check(stats, "f16")
 
 
+@pytest.mark.parametrize("hidden", [8, 12, 24])
+def test_f16_mlp_other_widths_match_oracle(hidden):
+    """the f16 option at widths 8 and 12 (the packed width-16 form over zero-padded weights) and 24 (the
+    generic f16 form)"""
+    from hush_amd.train import random_weights, to_f16_blob
+    wts = to_f16_blob(random_weights(60 + hidden, hidden))
+    hs, o = _pair(96, 64, precision="f16", weights=wts)
+    gen = _scene(96, 64, 38)
+    stats, surfs = [], None
+    for f in range(3):
+        st, surfs = _step(hs, o, gen.render(f), surfs)
+        stats.append(st)
+    hs.destroy()
+    _check(stats, f"f16 hidden {hidden}")
+
+
 def test_input_errors_match_reference():
     from hush_amd.hip import Surface
     from tests.parity import INPUT_KEYS
