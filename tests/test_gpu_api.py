@@ -181,10 +181,11 @@ def test_small_and_ragged_sizes_match_oracle(w, h):
     _check(stats, f"{w}x{h}")
 
 
-@pytest.mark.parametrize("hidden", [8, 12, 32, 40, 72])
+@pytest.mark.parametrize("hidden", [1, 8, 12, 32, 40, 72, 256])
 def test_other_hidden_widths_match_oracle(hidden):
-    """widths 8 and 32 run the register form at their own width, 12 and 40 the width-16 and width-64 forms over
-    zero-padded weights, 72 the chunked form over weights padded to 96"""
+    """widths 8 and 32 run the register form at their own width, 1, 12 and 40 the width-8, -16 and -64 forms
+    over zero-padded weights, 72 and 256 (the largest hiddenOf admits) the chunked form over weights padded to
+    96 and 256"""
     from hush_amd.train import random_weights, to_f16_blob
     wts = to_f16_blob(random_weights(40 + hidden, hidden))
     assert hush_amd.hiddenOf(wts.size) == hidden
@@ -210,9 +211,9 @@ def test_f16_mlp_variant_matches_oracle():
     _check(stats, "f16")
 
 
-@pytest.mark.parametrize("hidden", [8, 12, 24])
+@pytest.mark.parametrize("hidden", [1, 8, 12, 24])
 def test_f16_mlp_other_widths_match_oracle(hidden):
-    """the f16 option at widths 8 and 12 (the packed width-16 form over zero-padded weights) and 24 (the
+    """the f16 option at widths 1, 8 and 12 (the packed width-16 form over zero-padded weights) and 24 (the
     generic f16 form)"""
     from hush_amd.train import random_weights, to_f16_blob
     wts = to_f16_blob(random_weights(60 + hidden, hidden))
